@@ -1,0 +1,110 @@
+/*
+ * mc3d: MI355X-native (gfx950) per-timestep hot path of a compact-difference
+ * curvilinear compressible Navier-Stokes solver, exposed as a plain C ABI.
+ *
+ * Every entry point returns 0 on success or a negative MC3D_E* code; the text
+ * of the last failure on the calling thread is available from
+ * mc3d_last_error().  The library never aborts the process.
+ *
+ * Conventions
+ *  - Fields are device-resident FP64 arrays of Nx*Ny*Nz values in one layout,
+ *    x fastest: index = (k*Ny + j)*Nx + i.  The reference keeps three pencil
+ *    layouts and transposes between them; per-line arithmetic does not depend
+ *    on the layout, so this library keeps one and solves strided lines in place.
+ *  - "Lines" of direction d are enumerated by the two other indices with the
+ *    lower direction fastest (x-lines: q = k*Ny + j; y-lines: q = k*Nx + i;
+ *    z-lines: q = j*Nx + i).  Per-line side arrays (transformed-periodic wrap
+ *    values) use this order.
+ *  - Calls are asynchronous on the given stream (a hipStream_t passed as
+ *    void*; NULL is the default stream).  The caller owns all field memory;
+ *    handles own their precomputed factor tables.
+ *  - Direction ids: 0 = xi (x), 1 = eta (y), 2 = zeta (z).
+ *    Boundary types: MC3D_PERIODIC, MC3D_DIRICHLET.
+ */
+#ifndef MC3D_H
+#define MC3D_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MC3D_OK 0
+#define MC3D_EINVAL (-1)      /* bad argument or unsupported size */
+#define MC3D_EHIP (-2)        /* HIP runtime failure */
+#define MC3D_ENOMEM (-3)      /* allocation failure */
+#define MC3D_ENUMERIC (-4)    /* factorisation failure (zero pivot) */
+#define MC3D_ECONFIG (-5)     /* options file / configuration error */
+
+enum { MC3D_PERIODIC = 0, MC3D_DIRICHLET = 1 };
+/* derivative schemes: src/Options.hpp:22 (FDType) */
+enum { MC3D_CD2 = 0, MC3D_PADE6 = 1, MC3D_PENTA10 = 2 };
+/* filter kinds: src/Options.hpp:23 (FilterType) plus the two DSM test filters */
+enum { MC3D_COMPACT8 = 0, MC3D_COMPACT10 = 1, MC3D_COMMN5PADE = 2, MC3D_COMMN5EXP = 3 };
+
+typedef struct mc3d_lineop mc3d_lineop; /* opaque derivative or filter operator */
+
+const char *mc3d_last_error(void);
+int mc3d_device_count(int *count);
+
+/* ---------------------------------------------------------------------------
+ * First-derivative operator along one direction.
+ * Replaces: AbstractDerivatives (src/AbstractDerivatives.hpp:13-53) and its
+ * constructors Pade6(Domain*, BCType, Direct) src/Pade6.hpp:39,
+ * Penta10(...) src/Penta10.hpp:38, CD2(...) src/CD2.hpp:25.
+ * dims = {Nx, Ny, Nz}; dd = grid spacing along dir in computational space.
+ */
+int mc3d_deriv_create(int scheme, int dir, int bc, const int dims[3], double dd,
+                      mc3d_lineop **out);
+
+/* out[f] = d(in[f])/d(xi_dir) for f < nfields (nfields <= 16; in and out
+ * must not alias).  Replaces calc1stDerivField(in, out)
+ * src/AbstractDerivatives.hpp:41 / src/Pade6.cpp:3-45. */
+int mc3d_deriv_apply(const mc3d_lineop *op, const double *const *in, double *const *out,
+                     int nfields, void *stream);
+
+/* Transformed-periodic variant: the wrapped stencil values are taken from
+ * wrap[0..5] = {Nm3, Nm2, Nm1, Np1, Np2, Np3} (one value per line, in the line
+ * order above; entries beyond the scheme's half-width may be NULL).
+ * Replaces calc1stDerivField_TPB(...) src/AbstractDerivatives.hpp:45-47. */
+int mc3d_deriv_apply_tpb(const mc3d_lineop *op, const double *in, double *out,
+                         const double *const wrap[6], void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Compact / test filter along one direction.  bc3 = boundary types of the
+ * x, y, z directions: lines lying on a Dirichlet face of another direction are
+ * copied unfiltered (src/Compact8Filter.cpp:142-230).
+ * Replaces: AbstractFilter (src/AbstractFilter.hpp:13-35) and
+ * Compact8Filter(alphaF, Domain*, BC*, BCType, Direct) src/Compact8Filter.hpp:28,
+ * Compact10Filter src/Compact10Filter.hpp:30, CommN5PadeTestFilter
+ * src/CommN5PadeTestFilter.hpp:28, CommN5ExpTestFilter src/CommN5ExpTestFilter.hpp:24.
+ */
+int mc3d_filter_create(int kind, double alphaF, int dir, const int bc3[3], const int dims[3],
+                       mc3d_lineop **out);
+
+/* Replaces filterField(in, out) src/AbstractFilter.hpp:34.  in may equal out. */
+int mc3d_filter_apply(const mc3d_lineop *op, const double *const *in, double *const *out,
+                      int nfields, void *stream);
+
+void mc3d_lineop_destroy(mc3d_lineop *op);
+
+/* Diagnostics of an operator's partitioned solve: chunks per line, rows per
+ * chunk, lines per workgroup. */
+int mc3d_lineop_info(const mc3d_lineop *op, int *chunks, int *chunk_rows, int *lines_per_block);
+
+/* Diagnostic (no counterpart in the reference): builds, on the host only,
+ * the partition tables an operator would upload, for verification of the
+ * factorisation without a GPU.  cls 0 = derivative (kind = scheme), 1 = filter
+ * (kind = filter kind).  sizes[8] receives {N, W, H, P, NI, nbl, nbr, int_off};
+ * each non-NULL table pointer must have room for the sizes documented in
+ * mpi_3dcompact_amd/csrc/lineop.h (lu: P*16*5, sp: P*16*2W, G: P*2W*NI,
+ * rows: (nbl+nbr)*19, interior: 11, cstart/clen: P). */
+int mc3d_plan_tables(int cls, int kind, int bc, int n, double dd, double alphaF, int sizes[8],
+                     double *lu, double *sp, double *G, double *rows, double *interior,
+                     int *cstart, int *clen);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

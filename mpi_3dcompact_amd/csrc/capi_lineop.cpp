@@ -1,0 +1,219 @@
+// C ABI of the derivative and filter operators (include/mc3d.h).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+
+#include "capi_common.h"
+#include "lineop_launch.h"
+#include "lineop_plan.h"
+#include "mc3d.h"
+
+struct mc3d_lineop {
+    mc3d::Plan plan;
+    mc3d::OpDev dev{};
+    int cls = 0, kind = 0, dir = 0, bc = 0;
+    int dims[3] = {0, 0, 0};
+    void *dmem = nullptr;
+    int device = 0;
+};
+
+namespace {
+
+int upload(mc3d_lineop *op) {
+    const mc3d::Plan &p = op->plan;
+    const size_t nr = p.rows.size(), nl = p.lu.size(), ns = p.sp.size(), ng = p.G.size();
+    const size_t bytes = sizeof(double) * (nr + nl + ns + ng) + sizeof(int) * 2 * p.P + 256;
+    if (hipGetDevice(&op->device) != hipSuccess) return mc3d::fail(MC3D_EHIP, "hipGetDevice failed");
+    if (hipMalloc(&op->dmem, bytes) != hipSuccess) return mc3d::fail(MC3D_ENOMEM, "hipMalloc of operator tables failed");
+    char *base = static_cast<char *>(op->dmem);
+    size_t off = 0;
+    auto put = [&](const void *src, size_t n) -> void * {
+        void *dst = base + off;
+        if (n && hipMemcpy(dst, src, n, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+        off += (n + 15) & ~size_t(15);
+        return dst;
+    };
+    double dummy = 0.0;
+    mc3d::OpDev &d = op->dev;
+    d.rows = static_cast<const double *>(put(nr ? p.rows.data() : &dummy, sizeof(double) * (nr ? nr : 1)));
+    d.lu = static_cast<const double *>(put(p.lu.data(), sizeof(double) * nl));
+    d.sp = static_cast<const double *>(put(p.sp.data(), sizeof(double) * ns));
+    d.G = static_cast<const double *>(put(p.G.data(), sizeof(double) * ng));
+    d.cstart = static_cast<const int *>(put(p.cstart.data(), sizeof(int) * p.P));
+    d.clen = static_cast<const int *>(put(p.clen.data(), sizeof(int) * p.P));
+    if (!d.rows || !d.lu || !d.sp || !d.G || !d.cstart || !d.clen)
+        return mc3d::fail(MC3D_EHIP, "upload of operator tables failed");
+    return MC3D_OK;
+}
+
+int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[3], double dd,
+           double alphaF, mc3d_lineop **out) {
+    if (!out) return mc3d::fail(MC3D_EINVAL, "null output handle");
+    *out = nullptr;
+    if (!dims || dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0)
+        return mc3d::fail(MC3D_EINVAL, "dims must be positive");
+    if (dir < 0 || dir > 2) return mc3d::fail(MC3D_EINVAL, "dir must be 0, 1 or 2");
+    if (bc != MC3D_PERIODIC && bc != MC3D_DIRICHLET) return mc3d::fail(MC3D_EINVAL, "bad boundary type");
+    if (cls == mc3d::kDeriv && !(dd > 0.0)) return mc3d::fail(MC3D_EINVAL, "dd must be positive");
+    const long npts = (long)dims[0] * dims[1] * dims[2];
+    if (npts > (long)2147483647) return mc3d::fail(MC3D_EINVAL, "field larger than 2^31 points");
+    mc3d::PlanRequest rq{cls, kind, bc, dims[dir], dd, alphaF};
+    auto *op = new mc3d_lineop();
+    std::string err;
+    int rc = mc3d::build_plan(rq, op->plan, err);
+    if (rc) {
+        delete op;
+        return mc3d::fail(rc, err);
+    }
+    op->cls = cls; op->kind = kind; op->dir = dir; op->bc = bc;
+    std::memcpy(op->dims, dims, sizeof(op->dims));
+    const mc3d::Plan &p = op->plan;
+    mc3d::OpDev &d = op->dev;
+    d.N = p.N; d.dir = dir; d.W = p.W; d.H = p.H; d.P = p.P; d.L = p.L; d.NI = p.NI;
+    d.ext = p.ext; d.periodic = p.periodic; d.nbl = p.nbl; d.nbr = p.nbr;
+    d.int_off = p.int_off; d.int_w = (int)p.int_c.size();
+    for (int k = 0; k < d.int_w; ++k) d.int_c[k] = p.int_c[k];
+    d.dd = p.dd; d.divide = p.divide ? 1 : 0;
+    d.passA = d.passB = 0;
+    if (cls == mc3d::kFilter && bc3) {
+        const int da = dir == 0 ? 1 : 0, db = dir == 2 ? 1 : 2;
+        d.passA = bc3[da] == MC3D_DIRICHLET;
+        d.passB = bc3[db] == MC3D_DIRICHLET;
+        d.NA = dims[da];
+        d.NB = dims[db];
+    }
+    if (mc3d::lineop_lds_bytes(d) > 65536) {
+        delete op;
+        return mc3d::fail(MC3D_EINVAL, "line too long for the LDS tile");
+    }
+    rc = upload(op);
+    if (rc) {
+        if (op->dmem) (void)hipFree(op->dmem);
+        delete op;
+        return rc;
+    }
+    *out = op;
+    return MC3D_OK;
+}
+
+int apply(const mc3d_lineop *op, const double *const *in, double *const *out, int nfields,
+          int mode, const double *const wrap[6], int accumulate, void *stream) {
+    if (!op) return mc3d::fail(MC3D_EINVAL, "null operator");
+    if (nfields < 1 || nfields > mc3d::kMaxFields) return mc3d::fail(MC3D_EINVAL, "nfields must be 1..16");
+    if (!in || !out) return mc3d::fail(MC3D_EINVAL, "null field list");
+    mc3d::LaunchArgs a{};
+    for (int f = 0; f < nfields; ++f) {
+        if (!in[f] || !out[f]) return mc3d::fail(MC3D_EINVAL, "null field pointer");
+        a.in[f] = in[f];
+        a.out[f] = out[f];
+    }
+    for (int k = 0; k < 6; ++k) a.wrap[k] = wrap ? wrap[k] : nullptr;
+    a.Nx = op->dims[0]; a.Ny = op->dims[1]; a.Nz = op->dims[2];
+    a.nlines = (int)((long)a.Nx * a.Ny * a.Nz / op->dims[op->dir]);
+    a.accumulate = accumulate;
+    hipError_t e = mc3d::lineop_launch(op->dev, a, nfields, mode, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return mc3d::fail(MC3D_EHIP, std::string("line operator launch: ") + hipGetErrorString(e));
+    return MC3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mc3d_deriv_create(int scheme, int dir, int bc, const int dims[3], double dd, mc3d_lineop **out) {
+    if (scheme < MC3D_CD2 || scheme > MC3D_PENTA10) return mc3d::fail(MC3D_EINVAL, "unknown derivative scheme");
+    return create(mc3d::kDeriv, scheme, dir, bc, nullptr, dims, dd, 0.0, out);
+}
+
+int mc3d_deriv_apply(const mc3d_lineop *op, const double *const *in, double *const *out, int nfields,
+                     void *stream) {
+    if (op && op->cls != mc3d::kDeriv) return mc3d::fail(MC3D_EINVAL, "operator is not a derivative");
+    if (op && nfields >= 1 && in && out)
+        for (int f = 0; f < nfields && f < mc3d::kMaxFields; ++f)
+            if (in[f] == out[f]) return mc3d::fail(MC3D_EINVAL, "derivative input and output must not alias");
+    const int mode = op && op->bc == MC3D_PERIODIC ? mc3d::kModePlainPeriodic : mc3d::kModePlainDirichlet;
+    return apply(op, in, out, nfields, mode, nullptr, 0, stream);
+}
+
+int mc3d_deriv_apply_tpb(const mc3d_lineop *op, const double *in, double *out,
+                         const double *const wrap[6], void *stream) {
+    if (!op || op->cls != mc3d::kDeriv) return mc3d::fail(MC3D_EINVAL, "operator is not a derivative");
+    if (op->bc != MC3D_PERIODIC)
+        return mc3d::fail(MC3D_EINVAL, "transformed-periodic derivative needs a periodic operator");
+    if (!wrap) return mc3d::fail(MC3D_EINVAL, "null wrap list");
+    for (int k = 3 - op->plan.H; k < 3 + op->plan.H; ++k)
+        if (!wrap[k]) return mc3d::fail(MC3D_EINVAL, "missing wrap array for the scheme's half width");
+    if (in == out) return mc3d::fail(MC3D_EINVAL, "derivative input and output must not alias");
+    return apply(op, &in, &out, 1, mc3d::kModeTPB, wrap, 0, stream);
+}
+
+int mc3d_filter_create(int kind, double alphaF, int dir, const int bc3[3], const int dims[3],
+                       mc3d_lineop **out) {
+    if (kind < MC3D_COMPACT8 || kind > MC3D_COMMN5EXP) return mc3d::fail(MC3D_EINVAL, "unknown filter kind");
+    if (!bc3) return mc3d::fail(MC3D_EINVAL, "null bc3");
+    if (dir < 0 || dir > 2) return mc3d::fail(MC3D_EINVAL, "dir must be 0, 1 or 2");
+    return create(mc3d::kFilter, kind, dir, bc3[dir], bc3, dims, 1.0, alphaF, out);
+}
+
+int mc3d_filter_apply(const mc3d_lineop *op, const double *const *in, double *const *out, int nfields,
+                      void *stream) {
+    if (op && op->cls != mc3d::kFilter) return mc3d::fail(MC3D_EINVAL, "operator is not a filter");
+    const int mode = op && op->bc == MC3D_PERIODIC ? mc3d::kModePlainPeriodic : mc3d::kModePlainDirichlet;
+    return apply(op, in, out, nfields, mode, nullptr, 0, stream);
+}
+
+void mc3d_lineop_destroy(mc3d_lineop *op) {
+    if (!op) return;
+    if (op->dmem) {
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        (void)hipSetDevice(op->device);
+        (void)hipFree(op->dmem);
+        (void)hipSetDevice(cur);
+    }
+    delete op;
+}
+
+int mc3d_lineop_info(const mc3d_lineop *op, int *chunks, int *chunk_rows, int *lines_per_block) {
+    if (!op) return mc3d::fail(MC3D_EINVAL, "null operator");
+    if (chunks) *chunks = op->plan.P;
+    if (chunk_rows) *chunk_rows = op->plan.clen.empty() ? 0 : op->plan.clen[0];
+    if (lines_per_block) *lines_per_block = op->plan.L;
+    return MC3D_OK;
+}
+
+int mc3d_plan_tables(int cls, int kind, int bc, int n, double dd, double alphaF, int sizes[8],
+                     double *lu, double *sp, double *G, double *rows, double *interior,
+                     int *cstart, int *clen) {
+    if (!sizes) return mc3d::fail(MC3D_EINVAL, "null sizes");
+    mc3d::PlanRequest rq{cls, kind, bc, n, dd, alphaF};
+    mc3d::Plan p;
+    std::string err;
+    int rc = mc3d::build_plan(rq, p, err);
+    if (rc) return mc3d::fail(rc, err);
+    const int v[8] = {p.N, p.W, p.H, p.P, p.NI, p.nbl, p.nbr, p.int_off};
+    std::memcpy(sizes, v, sizeof(v));
+    if (lu) std::memcpy(lu, p.lu.data(), sizeof(double) * p.lu.size());
+    if (sp) std::memcpy(sp, p.sp.data(), sizeof(double) * p.sp.size());
+    if (G) std::memcpy(G, p.G.data(), sizeof(double) * p.G.size());
+    if (rows && !p.rows.empty()) std::memcpy(rows, p.rows.data(), sizeof(double) * p.rows.size());
+    if (interior) {
+        for (int k = 0; k < 2 * mc3d::kHMax + 1; ++k)
+            interior[k] = k < (int)p.int_c.size() ? p.int_c[k] : 0.0;
+    }
+    if (cstart) std::memcpy(cstart, p.cstart.data(), sizeof(int) * p.P);
+    if (clen) std::memcpy(clen, p.clen.data(), sizeof(int) * p.P);
+    return MC3D_OK;
+}
+
+}  // extern "C"
+
+// internal accessor used by the solver
+namespace mc3d {
+int lineop_launch_accumulate(const mc3d_lineop *op, const double *const *in, double *const *out,
+                             int nfields, int accumulate, void *stream) {
+    const int mode = op->bc == MC3D_PERIODIC ? kModePlainPeriodic : kModePlainDirichlet;
+    return apply(op, in, out, nfields, mode, nullptr, accumulate, stream);
+}
+}  // namespace mc3d

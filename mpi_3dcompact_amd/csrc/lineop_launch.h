@@ -1,0 +1,25 @@
+// Launch interface of the line-operator kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "lineop.h"
+
+namespace mc3d {
+
+enum LaunchMode { kModePlainPeriodic = 0, kModePlainDirichlet = 1, kModeTPB = 2 };
+
+struct LaunchArgs {
+    const double *in[kMaxFields];
+    double *out[kMaxFields];
+    const double *wrap[6];  // TPB wrap values {Nm3, Nm2, Nm1, Np1, Np2, Np3}
+    int Nx, Ny, Nz;
+    int nlines;
+    int accumulate;  // out += result instead of out = result
+};
+
+size_t lineop_lds_bytes(const OpDev &op);
+hipError_t lineop_launch(const OpDev &op, const LaunchArgs &a, int nfields, int mode,
+                         hipStream_t st);
+
+}  // namespace mc3d
