@@ -93,6 +93,41 @@ void mc3d_lineop_destroy(mc3d_lineop *op);
  * chunk, lines per workgroup. */
 int mc3d_lineop_info(const mc3d_lineop *op, int *chunks, int *chunk_rows, int *lines_per_block);
 
+/* ---------------------------------------------------------------------------
+ * Whole solver step (CurvilinearCSolver + RK + SGS), configured by the
+ * reference's options file text (MPI_3DCompact.in: sections [TIMESTEPPING],
+ * [BC], [DOMAIN], [SOLVER], [RESTART], [STATS], [LES], ...).
+ * Replaces the construction sequence of src/MPI_3DCompact.cpp:34-316
+ * (Options, Domain, CurvilinearCSolver, AbstractRK selection) and
+ * AbstractRK::executeSolverLoop (src/AbstractRK.hpp:14).
+ * The solver owns its device fields.  Host arrays are x-fastest.
+ */
+typedef struct mc3d_solver mc3d_solver;
+
+int mc3d_solver_create(const char *options_text, mc3d_solver **out);
+/* metrics[10] = {J, J11, J12, J13, J21, J22, J23, J31, J32, J33} (host arrays),
+ * the fields produced by src/AbstractSingleBlockMesh.cpp:23-1439. */
+int mc3d_solver_set_metrics(mc3d_solver *s, const double *const metrics[10]);
+/* primitive initial condition; performs setInitialConditions
+ * (src/CurvilinearCSolver_BC.cpp:3-63). */
+int mc3d_solver_set_initial(mc3d_solver *s, const double *rho0, const double *U0,
+                            const double *V0, const double *W0, const double *p0);
+/* advance nsteps full time steps (one executeSolverLoop iteration each) */
+int mc3d_solver_step(mc3d_solver *s, int nsteps);
+/* run until MAXTIMESTEP / MAXTIME, like executeSolverLoop */
+int mc3d_solver_run(mc3d_solver *s);
+/* copy a named field to host memory / get its device address.  Names:
+ * rho rhoU rhoV rhoW rhoE (the *1 registers), rhok.. rhoEk, rhok2.. rhoEk2,
+ * rho2.. rhoE2, U V W p T mu sos Ucurv Vcurv Wcurv mu_sgs k_sgs taukk,
+ * J J11 .. J33. */
+int mc3d_solver_get_field(mc3d_solver *s, const char *name, double *host_out);
+int mc3d_solver_field_device(mc3d_solver *s, const char *name, double **dev_ptr);
+int mc3d_solver_scalars(mc3d_solver *s, double *time, double *dt, int *timestep);
+/* the solver's HIP stream (as void*), for timing with events */
+int mc3d_solver_stream(mc3d_solver *s, void **stream);
+int mc3d_solver_sync(mc3d_solver *s);
+void mc3d_solver_destroy(mc3d_solver *s);
+
 /* Diagnostic (no counterpart in the reference): builds, on the host only,
  * the partition tables an operator would upload, for verification of the
  * factorisation without a GPU.  cls 0 = derivative (kind = scheme), 1 = filter

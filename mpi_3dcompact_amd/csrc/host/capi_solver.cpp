@@ -1,0 +1,143 @@
+// C ABI of the whole solver (include/mc3d.h).  Exceptions never cross it.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "../capi_common.h"
+#include "mc3d.h"
+#include "solver.hpp"
+
+struct mc3d_solver {
+    hipStream_t stream = nullptr;
+    mc3d::CurvilinearCSolver *cs = nullptr;
+    bool metrics = false, initial = false;
+};
+
+namespace {
+
+template <class F>
+int guarded(F &&f) {
+    try {
+        f();
+        return MC3D_OK;
+    } catch (const mc3d::Error &e) {
+        return mc3d::fail(e.code, e.what());
+    } catch (const std::bad_alloc &) {
+        return mc3d::fail(MC3D_ENOMEM, "host allocation failed");
+    } catch (const std::exception &e) {
+        return mc3d::fail(MC3D_EINVAL, e.what());
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int mc3d_solver_create(const char *options_text, mc3d_solver **out) {
+    if (!out) return mc3d::fail(MC3D_EINVAL, "null output handle");
+    *out = nullptr;
+    if (!options_text) return mc3d::fail(MC3D_EINVAL, "null options text");
+    mc3d::Options o;
+    std::string err;
+    int rc = mc3d::Options::parse(options_text, o, err);
+    if (rc) return mc3d::fail(rc, "options: " + err);
+    auto *s = new mc3d_solver();
+    rc = guarded([&] {
+        mc3d::hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "stream creation");
+        s->cs = new mc3d::CurvilinearCSolver(o, s->stream);
+    });
+    if (rc) {
+        delete s->cs;
+        if (s->stream) (void)hipStreamDestroy(s->stream);
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return MC3D_OK;
+}
+
+int mc3d_solver_set_metrics(mc3d_solver *s, const double *const metrics[10]) {
+    if (!s || !metrics) return mc3d::fail(MC3D_EINVAL, "null argument");
+    for (int k = 0; k < 10; ++k)
+        if (!metrics[k]) return mc3d::fail(MC3D_EINVAL, "null metric array");
+    return guarded([&] { s->cs->set_metrics_host(metrics); s->metrics = true; });
+}
+
+int mc3d_solver_set_initial(mc3d_solver *s, const double *rho0, const double *U0, const double *V0,
+                            const double *W0, const double *p0) {
+    if (!s || !rho0 || !U0 || !V0 || !W0 || !p0) return mc3d::fail(MC3D_EINVAL, "null argument");
+    if (!s->metrics) return mc3d::fail(MC3D_EINVAL, "set the metrics before the initial condition");
+    return guarded([&] { s->cs->setInitialConditions_host(rho0, U0, V0, W0, p0); s->initial = true; });
+}
+
+int mc3d_solver_step(mc3d_solver *s, int nsteps) {
+    if (!s) return mc3d::fail(MC3D_EINVAL, "null solver");
+    if (!s->initial) return mc3d::fail(MC3D_EINVAL, "initial condition not set");
+    if (nsteps < 0) return mc3d::fail(MC3D_EINVAL, "negative step count");
+    return guarded([&] {
+        for (int i = 0; i < nsteps; ++i) s->cs->rk->step();
+    });
+}
+
+int mc3d_solver_run(mc3d_solver *s) {
+    if (!s) return mc3d::fail(MC3D_EINVAL, "null solver");
+    if (!s->initial) return mc3d::fail(MC3D_EINVAL, "initial condition not set");
+    return guarded([&] { s->cs->rk->executeSolverLoop(); });
+}
+
+int mc3d_solver_field_device(mc3d_solver *s, const char *name, double **dev_ptr) {
+    if (!s || !name || !dev_ptr) return mc3d::fail(MC3D_EINVAL, "null argument");
+    double *p = s->cs->field_by_name(name);
+    if (!p) return mc3d::fail(MC3D_EINVAL, std::string("unknown field ") + name);
+    *dev_ptr = p;
+    return MC3D_OK;
+}
+
+int mc3d_solver_get_field(mc3d_solver *s, const char *name, double *host_out) {
+    double *p = nullptr;
+    int rc = mc3d_solver_field_device(s, name, &p);
+    if (rc) return rc;
+    if (!host_out) return mc3d::fail(MC3D_EINVAL, "null output");
+    return guarded([&] {
+        mc3d::hip_check(hipMemcpyAsync(host_out, p, sizeof(double) * s->cs->N, hipMemcpyDeviceToHost,
+                                       s->stream), "field download");
+        mc3d::hip_check(hipStreamSynchronize(s->stream), "field download");
+    });
+}
+
+int mc3d_solver_scalars(mc3d_solver *s, double *time, double *dt, int *timestep) {
+    if (!s) return mc3d::fail(MC3D_EINVAL, "null solver");
+    return guarded([&] {
+        mc3d::StepScalars h;
+        mc3d::hip_check(hipMemcpyAsync(&h, s->cs->sc, sizeof(h), hipMemcpyDeviceToHost, s->stream), "scalar read");
+        mc3d::hip_check(hipStreamSynchronize(s->stream), "scalar read");
+        if (time) *time = h.time;
+        if (dt) *dt = h.dt;
+        if (timestep) *timestep = s->cs->timeStep;
+    });
+}
+
+int mc3d_solver_stream(mc3d_solver *s, void **stream) {
+    if (!s || !stream) return mc3d::fail(MC3D_EINVAL, "null argument");
+    *stream = s->stream;
+    return MC3D_OK;
+}
+
+int mc3d_solver_sync(mc3d_solver *s) {
+    if (!s) return mc3d::fail(MC3D_EINVAL, "null solver");
+    return guarded([&] { mc3d::hip_check(hipStreamSynchronize(s->stream), "synchronize"); });
+}
+
+void mc3d_solver_destroy(mc3d_solver *s) {
+    if (!s) return;
+    try {
+        delete s->cs;
+    } catch (...) {
+    }
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+}  // extern "C"

@@ -1,0 +1,65 @@
+// Options file (INI) of the solver: the reference's sections, keys, defaults,
+// enum spellings and validation (src/Options.hpp:133-426, src/Options.cpp).
+// Parsing errors are returned as text instead of aborting the process.
+#pragma once
+
+#include <map>
+#include <string>
+
+namespace mc3d {
+
+struct Options {
+    enum TimeSteppingType { CONST_DT, CONST_CFL };
+    enum BCType { PERIODIC_SOLVE, DIRICHLET_SOLVE };
+    enum BCKind { PERIODIC, INTERNALLY_PERIODIC, SPONGE, ADIABATIC_WALL, CONST_T_WALL,
+                  MOVING_ADIABATIC_WALL, INLET };
+    enum SpongeKind { RECTILINEAR, CYLINDRICAL };
+    enum FDType { CD2, PADE6, PENTA10 };
+    enum FilterType { COMPACT8, COMPACT10 };
+    enum LESModel { NOMODEL, VREMAN, DSM };
+    enum StatsAvgType { NONE, XI1_AVG, XI2_AVG, XI3_AVG, LOCAL, GLOBAL };
+    enum RKType { TVDRK3, RK4, KENRK4, LSLDDRK4 };
+
+    // [TIMESTEPPING]
+    TimeSteppingType TSType = CONST_CFL;
+    double CFL = 0.5, dt = -1.0, maxTime = 1000;
+    int maxTimeStep = 10000, filterStep = 1, checkStep = 1, dumpStep = 1000;
+    bool subStepFiltering = false;
+    // [BC]
+    BCType bcXType = PERIODIC_SOLVE, bcYType = PERIODIC_SOLVE, bcZType = PERIODIC_SOLVE;
+    BCKind bcX0 = PERIODIC, bcX1 = PERIODIC, bcY0 = PERIODIC, bcY1 = PERIODIC, bcZ0 = PERIODIC,
+           bcZ1 = PERIODIC;
+    double periodicDisp[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    // [SPONGE]
+    SpongeKind spongeKind = RECTILINEAR;
+    double spongeP = 1.0 / 1.4, spongeAvgT = 1.0, spongeStrength = 1.0;
+    // [DOMAIN]
+    int Nx = 0, Ny = 0, Nz = 0;
+    // [PENCILDECOMP]
+    int pRow = 0, pCol = 0;
+    // [SOLVER]
+    double alphaF = 0.45, mu_ref = 0.0;
+    bool useTiming = false;
+    FDType xFDType = PADE6, yFDType = PADE6, zFDType = PADE6;
+    FilterType filterType = COMPACT10;
+    RKType rkType = TVDRK3;
+    // [RESTART]
+    bool fromRestart = false, onlyGridFromRestart = false, spongeFromRestart = false;
+    std::string filename, sponge_filename;
+    int timeStep = 0;
+    double time = 0.0;
+    // [STATS]
+    StatsAvgType statsAvgType = NONE;
+    bool velocityStats = false, thermoStats = false;
+    int stats_interval = 5;
+    // [LES]
+    StatsAvgType musgsAvgType = NONE;
+    LESModel lesModel = NOMODEL;
+    bool dumpCoeffRange = false, useTaukk = true;
+    int testFilterType = 1;
+
+    // Parses INI text.  Returns 0, or a negative code with a message.
+    static int parse(const std::string &text, Options &out, std::string &err);
+};
+
+}  // namespace mc3d

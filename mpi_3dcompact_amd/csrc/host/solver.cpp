@@ -1,0 +1,484 @@
+// Host orchestration of the solver step (see solver.hpp).
+#include "solver.hpp"
+
+#include <cmath>
+#include <cstring>
+
+#include "../lineop_api.h"
+
+namespace mc3d {
+
+void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw Error(MC3D_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+static void mc3d_check(int rc, const char *what) {
+    if (rc != MC3D_OK) throw Error(rc, std::string(what) + ": " + mc3d_last_error());
+}
+
+Domain::Domain(const Options &o) {
+    gNx = o.Nx; gNy = o.Ny; gNz = o.Nz;
+    gN = (long)gNx * gNy * gNz;
+    // x[i] = i*L/N (periodic) or i/(N-1)*L, and dx = x[1]-x[0] with L = 1
+    auto spacing = [](int n, Options::BCType t) {
+        double x0 = 0.0, x1 = (t == Options::PERIODIC_SOLVE) ? 1.0 * 1.0 / (double)n
+                                                             : ((1.0) / ((double)n - 1.0)) * 1.0;
+        return x1 - x0;
+    };
+    dx = spacing(gNx, o.bcXType);
+    dy = spacing(gNy, o.bcYType);
+    dz = spacing(gNz, o.bcZType);
+}
+
+// ---------------------------------------------------------------------------
+LineDerivative::LineDerivative(int scheme, const Domain &d, Options::BCType bc, Direct dir,
+                               hipStream_t st)
+    : scheme_(scheme), st_(st) {
+    Nx = d.gNx; Ny = d.gNy; Nz = d.gNz;
+    currentDir = dir;
+    bcType = bc;
+    N = dir == DIRX ? Nx : dir == DIRY ? Ny : Nz;
+    dd = dir == DIRX ? d.dx : dir == DIRY ? d.dy : d.dz;
+    lhsBandwidth = scheme == MC3D_PENTA10 ? BW5 : scheme == MC3D_PADE6 ? BW3 : BW3;
+    rhsBandwidth = scheme == MC3D_PENTA10 ? BW7 : scheme == MC3D_PADE6 ? BW5 : BW3;
+    const int dims[3] = {Nx, Ny, Nz};
+    mc3d_check(mc3d_deriv_create(scheme, (int)dir, bc == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET,
+                                 dims, dd, &op_),
+               "derivative setup");
+}
+
+LineDerivative::~LineDerivative() { mc3d_lineop_destroy(op_); }
+
+void LineDerivative::calc1stDerivField(const double *in, double *out) {
+    calc1stDerivFields(1, &in, &out, false);
+}
+
+void LineDerivative::calc1stDerivField_TPB(const double *in, double *out, const double *Nm2,
+                                           const double *Nm1, const double *Np1, const double *Np2) {
+    const double *w[6] = {nullptr, Nm2, Nm1, Np1, Np2, nullptr};
+    mc3d_check(mc3d_deriv_apply_tpb(op_, in, out, w, st_), "transformed-periodic derivative");
+}
+
+void LineDerivative::calc1stDerivField_TPB(const double *in, double *out, const double *Nm3,
+                                           const double *Nm2, const double *Nm1, const double *Np1,
+                                           const double *Np2, const double *Np3) {
+    const double *w[6] = {Nm3, Nm2, Nm1, Np1, Np2, Np3};
+    mc3d_check(mc3d_deriv_apply_tpb(op_, in, out, w, st_), "transformed-periodic derivative");
+}
+
+// src/Pade6.cpp:395-397, src/Penta10.cpp:483-485, src/CD2.cpp:183-185
+double LineDerivative::calcNeumann(const double *f) {
+    if (scheme_ == MC3D_CD2) return (f[0] * 4.0 - f[1] * 1.0) / 3.0;
+    return (f[0] * 360.0 - f[1] * 450.0 + f[2] * 400.0 - f[3] * 225.0 + f[4] * 72.0 - f[5] * 10.0) / 147.0;
+}
+
+void LineDerivative::calc1stDerivFields(int n, const double *const *in, double *const *out,
+                                        bool accumulate) {
+    mc3d_check(lineop_launch_accumulate(op_, in, out, n, accumulate ? 1 : 0, st_), "derivative");
+}
+
+LineFilter::LineFilter(int kind, double alphaF, const Domain &d, const int bc3[3],
+                       AbstractDerivatives::Direct dir, hipStream_t st)
+    : st_(st) {
+    Nx = d.gNx; Ny = d.gNy; Nz = d.gNz;
+    currentDir = dir;
+    bcType = bc3[dir] == MC3D_PERIODIC ? Options::PERIODIC_SOLVE : Options::DIRICHLET_SOLVE;
+    N = dir == AbstractDerivatives::DIRX ? Nx : dir == AbstractDerivatives::DIRY ? Ny : Nz;
+    const int dims[3] = {Nx, Ny, Nz};
+    mc3d_check(mc3d_filter_create(kind, alphaF, (int)dir, bc3, dims, &op_), "filter setup");
+}
+
+LineFilter::~LineFilter() { mc3d_lineop_destroy(op_); }
+
+void LineFilter::filterField(const double *in, double *out) { filterFields(1, &in, &out); }
+
+void LineFilter::filterFields(int n, const double *const *in, double *const *out) {
+    mc3d_check(lineop_launch_accumulate(op_, in, out, n, 0, st_), "filter");
+}
+
+// ---------------------------------------------------------------------------
+// src/AbstractSGS.cpp:13-164
+void AbstractSGS::doAveraging(const double *phi, double *phiF) {
+    const Domain &d = cs->dom;
+    switch (musgsAvgType) {
+        case Options::LOCAL: {
+            double *t = cs->tmp[9];
+            filtY->filterField(phi, t);
+            filtZ->filterField(t, t);
+            filtX->filterField(t, phiF);
+            break;
+        }
+        case Options::GLOBAL:
+            hip_check(launch_global_mean(phi, phiF, cs->cfl_partial, cs->N, cs->st), "global mean");
+            break;
+        case Options::XI1_AVG:
+        case Options::XI2_AVG:
+        case Options::XI3_AVG: {
+            const int dir = musgsAvgType == Options::XI1_AVG ? 0 : musgsAvgType == Options::XI2_AVG ? 1 : 2;
+            hip_check(launch_xi_average(phi, phiF, dir, d.gNx, d.gNy, d.gNz, cs->st), "line average");
+            break;
+        }
+        default:
+            if (phi != phiF)
+                hip_check(hipMemcpyAsync(phiF, phi, sizeof(double) * cs->N, hipMemcpyDeviceToDevice, cs->st),
+                          "copy");
+            break;
+    }
+}
+
+// src/VremanSGS.hpp:15-41
+VremanSGS::VremanSGS(CurvilinearCSolver *c) {
+    cs = c;
+    musgsAvgType = c->opt.musgsAvgType;
+    const int bc3[3] = {c->opt.bcXType == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET,
+                        c->opt.bcYType == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET,
+                        c->opt.bcZType == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET};
+    filtX.reset(new Compact10Filter(-0.49, c->dom, bc3, AbstractDerivatives::DIRX, c->st));
+    filtY.reset(new Compact10Filter(-0.49, c->dom, bc3, AbstractDerivatives::DIRY, c->st));
+    filtZ.reset(new Compact10Filter(-0.49, c->dom, bc3, AbstractDerivatives::DIRZ, c->st));
+    mu_sgs = c->alloc_field();
+}
+
+void VremanSGS::getSGSViscosity(const double *const dU[3], const double *const dV[3],
+                                const double *const dW[3], const double *rho) {
+    hip_check(launch_vreman(dU, dV, dW, cs->msh, rho, mu_sgs, cs->dom.dx, cs->dom.dy, cs->dom.dz,
+                            cs->N, cs->st),
+              "vreman");
+    doAveraging(mu_sgs, mu_sgs);
+}
+
+// ---------------------------------------------------------------------------
+void AbstractRK::executeSolverLoop() {
+    while (!cs->endFlag) step();
+}
+
+void CurvilinearCSolver::run_stage(AbstractRK *r, int s, int S) {
+    rkStep = s;
+    if (s == S) rkLast = true;
+    preSubStep();
+    solveEqnSet();
+    postSubStep();
+    r->updateConservedData();
+    updateData();
+}
+
+// src/TVDRK3.hpp:28-37 coefficients, :50-176 stages
+void TVDRK3::step() {
+    cs->rkLast = false;
+    cs->preStep();
+    for (int s = 1; s <= 3; ++s) cs->run_stage(this, s, 3);
+    cs->postStep();
+}
+void TVDRK3::updateConservedData() {
+    switch (cs->rkStep) {
+        case 1: cs->stage_update(kTvd1, 0.0, 0.0, 1.0); break;
+        case 2: cs->stage_update(kTvd2, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0); break;
+        default: cs->stage_update(kTvd3, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0); break;
+    }
+}
+
+// src/RK4.hpp:27-34, :47-178
+void RK4::step() {
+    cs->rkLast = false;
+    cs->preStep();
+    for (int s = 1; s <= 4; ++s) cs->run_stage(this, s, 4);
+    cs->postStep();
+}
+void RK4::updateConservedData() {
+    const double a[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+    const double b[3] = {1.0 / 2.0, 1.0 / 2.0, 1.0};
+    const int s = cs->rkStep;
+    if (s == 1) cs->stage_update(kRk4First, a[0], b[0], 0.0);
+    else if (s < 4) cs->stage_update(kRk4Mid, a[s - 1], b[s - 1], 0.0);
+    else cs->stage_update(kRk4Last, a[3], 0.0, 0.0);
+}
+
+// src/KenRK4.hpp:30-40, :53-215
+static const double kKenA[5] = {0.0, -6234157559845.0 / 12983515589748.0,
+                                -6194124222391.0 / 4410992767914.0,
+                                -31623096876824.0 / 15682348800105.0,
+                                -12251185447671.0 / 11596622555746.0};
+static const double kKenB[5] = {494393426753.0 / 4806282396855.0, 4047970641027.0 / 5463924506627.0,
+                                9795748752853.0 / 13190207949281.0, 4009051133189.0 / 8539092990294.0,
+                                1348533437543.0 / 7166442652324.0};
+
+static void lowstorage_update(CurvilinearCSolver *cs, const double *A, const double *B, int S) {
+    const int s = cs->rkStep;
+    if (s == 1) cs->stage_update(kLsFirst, 0.0, B[0], 0.0);
+    else if (s < S) cs->stage_update(kLsMid, A[s - 1], B[s - 1], 0.0);
+    else cs->stage_update(kLsLast, A[s - 1], B[s - 1], 0.0);
+}
+
+void KenRK4::step() {
+    cs->rkLast = false;
+    cs->preStep();
+    for (int s = 1; s <= 5; ++s) cs->run_stage(this, s, 5);
+    cs->postStep();
+}
+void KenRK4::updateConservedData() { lowstorage_update(cs, kKenA, kKenB, 5); }
+
+// src/LSLDDRK4.hpp:40-78, :93-460
+static const double kLs1A[5] = {0.0, -0.60512264332862261228, -2.04375640234761394333,
+                                -0.74069990637544192841, -4.42317651302968168941};
+static const double kLs1B[5] = {0.26874543888713438496, 0.8014706973220802933, 0.50515704269422722538,
+                                0.56235680379000296407, 0.05900655127758823335};
+static const double kLs2A[6] = {0.0, -0.44127377153877382565, -1.073982008079781868,
+                                -1.7063570791256758809, -2.7979293162682443056, -4.0913537120919160454};
+static const double kLs2B[6] = {0.11584888181285561688, 0.37287699051652864918, 0.73795368921435295698,
+                                0.57981109366311039538, 1.031284991300145194, 0.15};
+
+void LSLDDRK4::step() {
+    const int S = stages();
+    cs->rkLast = false;
+    cs->preStep();
+    for (int s = 1; s <= S; ++s) cs->run_stage(this, s, S);
+    cs->postStep();
+    rkDualStepStep = rkDualStepStep == 1 ? 2 : 1;
+}
+void LSLDDRK4::updateConservedData() {
+    if (rkDualStepStep == 1) lowstorage_update(cs, kLs1A, kLs1B, 5);
+    else lowstorage_update(cs, kLs2A, kLs2B, 6);
+}
+
+// ---------------------------------------------------------------------------
+// src/CurvilinearCSolver.hpp:120-245
+CurvilinearCSolver::CurvilinearCSolver(const Options &o, hipStream_t s)
+    : opt(o), dom(o), st(s) {
+    N = dom.gN;
+    timeStep = opt.timeStep;
+    // src/IdealGas.hpp:35-48
+    ig.gamma = 1.4;
+    ig.Pr = 0.7;
+    const double p_ref = 1.0 / ig.gamma, rho_ref = 1.0;
+    ig.T_ref = 1.0;
+    ig.mu_ref = opt.mu_ref;
+    ig.R_gas = p_ref / rho_ref / ig.T_ref;
+    ig.cp = ig.R_gas * ig.gamma / (ig.gamma - 1.0);
+    if (opt.bcXType != Options::PERIODIC_SOLVE || opt.bcYType != Options::PERIODIC_SOLVE ||
+        opt.bcZType != Options::PERIODIC_SOLVE)
+        throw Error(MC3D_ECONFIG, "the solver step supports periodic boundaries in all directions; "
+                                  "wall and sponge boundaries are not implemented");
+    // field slab
+    const int nfields = 10 + 20 + 10 + 12 + 15 + 5 + 10 + 8;
+    slab_bytes_ = sizeof(double) * (size_t)N * nfields + 4096;
+    hip_check(hipMalloc(&slab_, slab_bytes_), "field allocation");
+    for (int k = 0; k < 10; ++k) Jm[k] = alloc_field();
+    msh = Metrics{Jm[0], Jm[1], Jm[2], Jm[3], Jm[4], Jm[5], Jm[6], Jm[7], Jm[8], Jm[9]};
+    for (int q = 0; q < 5; ++q) { r1.q[q] = alloc_field(); rk_.q[q] = alloc_field(); k2.q[q] = alloc_field(); r2.q[q] = alloc_field(); }
+    pr = Prim{alloc_field(), alloc_field(), alloc_field(), alloc_field(), alloc_field(),
+              alloc_field(), alloc_field(), alloc_field(), alloc_field(), alloc_field()};
+    for (int d = 0; d < 3; ++d) { dU[d] = alloc_field(); dV[d] = alloc_field(); dW[d] = alloc_field(); dT[d] = alloc_field(); }
+    for (int e = 0; e < 15; ++e) pre[e] = alloc_field();
+    for (int e = 0; e < 5; ++e) div[e] = alloc_field();
+    for (int e = 0; e < 10; ++e) tmp[e] = alloc_field();
+    // zero everything: k2 registers must start at zero (momentum increments accumulate)
+    hip_check(hipMemsetAsync(slab_, 0, slab_bytes_, st), "field clear");
+    void *p = nullptr;
+    hip_check(hipMalloc(&p, sizeof(double) * 4096), "scratch allocation");
+    extra_.push_back(p);
+    cfl_partial = static_cast<double *>(p);
+    hip_check(hipMalloc(&p, sizeof(StepScalars)), "scalar allocation");
+    extra_.push_back(p);
+    sc = static_cast<StepScalars *>(p);
+    StepScalars s0{opt.dt, opt.CFL, opt.time, 0.0};
+    hip_check(hipMemcpyAsync(sc, &s0, sizeof(s0), hipMemcpyHostToDevice, st), "scalar init");
+
+    // src/CurvilinearCSolver.hpp:162-217
+    const Options::BCType bt[3] = {opt.bcXType, opt.bcYType, opt.bcZType};
+    const Options::FDType ft[3] = {opt.xFDType, opt.yFDType, opt.zFDType};
+    std::unique_ptr<AbstractDerivatives> *dv[3] = {&derivX, &derivY, &derivZ};
+    for (int d = 0; d < 3; ++d) {
+        auto dir = (AbstractDerivatives::Direct)d;
+        if (ft[d] == Options::CD2) dv[d]->reset(new CD2(dom, bt[d], dir, st));
+        else if (ft[d] == Options::PADE6) dv[d]->reset(new Pade6(dom, bt[d], dir, st));
+        else dv[d]->reset(new Penta10(dom, bt[d], dir, st));
+    }
+    const int bc3[3] = {bt[0] == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET,
+                        bt[1] == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET,
+                        bt[2] == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET};
+    std::unique_ptr<AbstractFilter> *fv[3] = {&filtX, &filtY, &filtZ};
+    for (int d = 0; d < 3; ++d) {
+        auto dir = (AbstractDerivatives::Direct)d;
+        if (opt.filterType == Options::COMPACT8) fv[d]->reset(new Compact8Filter(opt.alphaF, dom, bc3, dir, st));
+        else fv[d]->reset(new Compact10Filter(opt.alphaF, dom, bc3, dir, st));
+    }
+    // src/CurvilinearCSolver.hpp:232-242
+    if (opt.lesModel == Options::VREMAN) les.reset(new VremanSGS(this));
+    else if (opt.lesModel == Options::DSM)
+        throw Error(MC3D_ECONFIG, "LESMODEL DSM is not available in this build");
+    // src/MPI_3DCompact.cpp:164-176
+    switch (opt.rkType) {
+        case Options::TVDRK3: rk.reset(new TVDRK3(this)); break;
+        case Options::RK4: rk.reset(new RK4(this)); break;
+        case Options::KENRK4: rk.reset(new KenRK4(this)); break;
+        default: rk.reset(new LSLDDRK4(this)); break;
+    }
+}
+
+CurvilinearCSolver::~CurvilinearCSolver() {
+    les.reset();
+    (void)hipStreamSynchronize(st);
+    for (void *p : extra_) (void)hipFree(p);
+    if (slab_) (void)hipFree(slab_);
+}
+
+double *CurvilinearCSolver::alloc_field() {
+    const size_t bytes = sizeof(double) * (size_t)N;
+    const size_t off = (slab_used_ + 255) & ~size_t(255);
+    if (off + bytes > slab_bytes_) {
+        void *p = nullptr;
+        hip_check(hipMalloc(&p, bytes), "field allocation");
+        hip_check(hipMemsetAsync(p, 0, bytes, st), "field clear");
+        extra_.push_back(p);
+        return static_cast<double *>(p);
+    }
+    slab_used_ = off + bytes;
+    return reinterpret_cast<double *>(static_cast<char *>(slab_) + off);
+}
+
+void CurvilinearCSolver::set_metrics_host(const double *const m[10]) {
+    for (int k = 0; k < 10; ++k)
+        hip_check(hipMemcpyAsync(Jm[k], m[k], sizeof(double) * N, hipMemcpyHostToDevice, st), "metric upload");
+    hip_check(hipStreamSynchronize(st), "metric upload");
+}
+
+// src/CurvilinearCSolver_BC.cpp:3-63 (periodic: no wall / sponge branches)
+void CurvilinearCSolver::setInitialConditions_host(const double *rho0, const double *U0,
+                                                   const double *V0, const double *W0,
+                                                   const double *p0) {
+    const double *src[5] = {rho0, U0, V0, W0, p0};
+    double *d[5] = {tmp[0], tmp[1], tmp[2], tmp[3], tmp[4]};
+    for (int k = 0; k < 5; ++k)
+        hip_check(hipMemcpyAsync(d[k], src[k], sizeof(double) * N, hipMemcpyHostToDevice, st), "IC upload");
+    hip_check(launch_init_state(d[0], d[1], d[2], d[3], d[4], r1, pr, msh, ig, N, st), "initial state");
+    hip_check(hipStreamSynchronize(st), "initial state");
+}
+
+// calcDtFromCFL, src/CurvilinearCSolver_Core.cpp:189-237
+void CurvilinearCSolver::preStep() {
+    hip_check(launch_cfl(pr, msh, dom.dx, dom.dy, dom.dz, cfl_partial, N, st), "cfl");
+    const bool first = (timeStep == opt.timeStep);
+    hip_check(launch_cfl_finish(cfl_partial, cfl_blocks(N), sc, opt.TSType == Options::CONST_CFL,
+                                opt.CFL, first ? 1 : 0, opt.time, st),
+              "cfl finish");
+    timeStep++;
+}
+
+// src/CurvilinearCSolver_Core.cpp:468-787
+void CurvilinearCSolver::preSubStep() {
+    const Cons &s = state();
+    AbstractDerivatives *dv[3] = {derivX.get(), derivY.get(), derivZ.get()};
+    // gradients of U, V, W and T per direction (T does not depend on the SGS
+    // model here; the DSM branch, which modifies T first, is not built)
+    for (int d = 0; d < 3; ++d) {
+        const double *in[4] = {pr.U, pr.V, pr.W, pr.T};
+        double *out[4] = {dU[d], dV[d], dW[d], dT[d]};
+        dv[d]->calc1stDerivFields(4, in, out, false);
+    }
+    if (les) les->getSGSViscosity(dU, dV, dW, s.q[0]);
+    FluxArgs fa{};
+    for (int d = 0; d < 3; ++d) { fa.dU[d] = dU[d]; fa.dV[d] = dV[d]; fa.dW[d] = dW[d]; fa.dT[d] = dT[d]; }
+    fa.m = msh;
+    fa.U = pr.U; fa.V = pr.V; fa.W = pr.W; fa.p = pr.p; fa.mu = pr.mu;
+    fa.mu_sgs = les ? les->mu_sgs : nullptr;
+    fa.k_sgs = les ? les->k_sgs : nullptr;
+    fa.rho = s.q[0]; fa.rhoU = s.q[1]; fa.rhoV = s.q[2]; fa.rhoW = s.q[3]; fa.rhoE = s.q[4];
+    fa.Ucurv = pr.Ucurv; fa.Vcurv = pr.Vcurv; fa.Wcurv = pr.Wcurv;
+    for (int e = 0; e < 15; ++e) fa.pre[e] = pre[e];
+    fa.les = (opt.lesModel == Options::NOMODEL) ? 0 : (opt.lesModel == Options::DSM ? 2 : 1);
+    fa.Pr_t = Pr_t;
+    hip_check(launch_flux(fa, ig, N, st), "flux");
+    // divergence: xi of the _1 fluxes, then eta of _2 and zeta of _3 added in
+    // (src/CurvilinearCSolver_Core.cpp:763-773 with the sums of :810,839,...)
+    const double *p1[5] = {pre[0], pre[1], pre[2], pre[3], pre[4]};
+    const double *p2[5] = {pre[5], pre[6], pre[7], pre[8], pre[9]};
+    const double *p3[5] = {pre[10], pre[11], pre[12], pre[13], pre[14]};
+    derivX->calc1stDerivFields(5, p1, div, false);
+    derivY->calc1stDerivFields(5, p2, div, true);
+    derivZ->calc1stDerivFields(5, p3, div, true);
+}
+
+void CurvilinearCSolver::stage_update(int op, double a, double b, double c) {
+    RkArgs ra{};
+    ra.r1 = r1; ra.rk = rk_; ra.k2 = k2; ra.r2 = r2;
+    for (int e = 0; e < 5; ++e) ra.div[e] = div[e];
+    ra.J = msh.J;
+    ra.sc = sc;
+    ra.op = op;
+    ra.a = a; ra.b = b; ra.c = c;
+    hip_check(launch_rhs_rk(ra, N, st), "stage update");
+}
+
+// src/CurvilinearCSolver.hpp:346-353
+void CurvilinearCSolver::updateData() {
+    if (rkLast || opt.subStepFiltering) filterConservedData();
+    updateNonConservedData();
+}
+
+// src/CurvilinearCSolver_Core.cpp:949-1155
+void CurvilinearCSolver::filterConservedData() {
+    const Cons &in = rkLast ? r2 : rk_;
+    const Cons &out = rkLast ? r1 : rk_;
+    if (timeStep % opt.filterStep == 0) {
+        filterTimeStep++;
+        AbstractFilter *f[3] = {filtX.get(), filtY.get(), filtZ.get()};
+        int order[3];
+        if (filterTimeStep % 3 == 1) { order[0] = 1; order[1] = 2; order[2] = 0; }       // Y, Z, X
+        else if (filterTimeStep % 3 == 2) { order[0] = 2; order[1] = 0; order[2] = 1; }  // Z, X, Y
+        else { order[0] = 0; order[1] = 1; order[2] = 2; }                                // X, Y, Z
+        const double *src[5];
+        double *t[5], *dst[5];
+        for (int q = 0; q < 5; ++q) { src[q] = in.q[q]; t[q] = tmp[q]; dst[q] = out.q[q]; }
+        const double *tc[5] = {t[0], t[1], t[2], t[3], t[4]};
+        f[order[0]]->filterFields(5, src, t);
+        f[order[1]]->filterFields(5, tc, t);
+        f[order[2]]->filterFields(5, tc, dst);
+    } else if (rkLast) {
+        for (int q = 0; q < 5; ++q)
+            hip_check(hipMemcpyAsync(r1.q[q], r2.q[q], sizeof(double) * N, hipMemcpyDeviceToDevice, st), "copy");
+    }
+}
+
+void CurvilinearCSolver::updateNonConservedData() {
+    hip_check(launch_eos(rkLast ? r1 : rk_, pr, msh, ig, N, st), "eos");
+}
+
+// checkEnd without the kill file: src/CurvilinearCSolver_Core.cpp:1798-1819
+void CurvilinearCSolver::postStep() {
+    if (timeStep >= opt.maxTimeStep) endFlag = true;
+    if (!endFlag) {
+        StepScalars h;
+        hip_check(hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, st), "scalar read");
+        hip_check(hipStreamSynchronize(st), "scalar read");
+        if (h.time >= opt.maxTime) endFlag = true;
+    }
+}
+
+double *CurvilinearCSolver::field_by_name(const std::string &n) {
+    static const char *cons[5] = {"rho", "rhoU", "rhoV", "rhoW", "rhoE"};
+    for (int q = 0; q < 5; ++q) {
+        if (n == cons[q]) return r1.q[q];
+        if (n == std::string(cons[q]) + "k") return rk_.q[q];
+        if (n == std::string(cons[q]) + "k2") return k2.q[q];
+        if (n == std::string(cons[q]) + "2") return r2.q[q];
+    }
+    if (n == "U") return pr.U;
+    if (n == "V") return pr.V;
+    if (n == "W") return pr.W;
+    if (n == "p") return pr.p;
+    if (n == "T") return pr.T;
+    if (n == "mu") return pr.mu;
+    if (n == "sos") return pr.sos;
+    if (n == "Ucurv") return pr.Ucurv;
+    if (n == "Vcurv") return pr.Vcurv;
+    if (n == "Wcurv") return pr.Wcurv;
+    if (n == "mu_sgs") return les ? les->mu_sgs : nullptr;
+    if (n == "k_sgs") return les ? les->k_sgs : nullptr;
+    if (n == "taukk") return les ? les->taukk : nullptr;
+    static const char *met[10] = {"J", "J11", "J12", "J13", "J21", "J22", "J23", "J31", "J32", "J33"};
+    for (int k = 0; k < 10; ++k)
+        if (n == met[k]) return Jm[k];
+    return nullptr;
+}
+
+}  // namespace mc3d

@@ -1,0 +1,289 @@
+// Host C++ side of the hot path: the reference's plugin interfaces
+// (AbstractDerivatives, AbstractFilter, AbstractRK, AbstractSGS) re-declared
+// over device pointers, their HIP-backed implementations, and the
+// CurvilinearCSolver orchestrator that calls them in the reference's order.
+// Every operator runs on the solver's HIP stream; nothing here touches field
+// data on the host.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../pointwise.h"
+#include "mc3d.h"
+#include "options.hpp"
+
+namespace mc3d {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char *what);
+
+// Computational domain with unit length per direction (src/Domain.hpp:31-89,
+// src/MPI_3DCompact.cpp:132): spacing 1/N periodic, 1/(N-1) Dirichlet.
+struct Domain {
+    int gNx, gNy, gNz;
+    long gN;
+    double dx, dy, dz;
+    explicit Domain(const Options &o);
+};
+
+// ---------------------------------------------------------------------------
+// Derivatives, src/AbstractDerivatives.hpp:13-53
+// ---------------------------------------------------------------------------
+class AbstractDerivatives {
+  public:
+    enum Direct { DIRX, DIRY, DIRZ };
+    enum BW { BW3, BW5, BW7 };
+    int Nx, Ny, Nz, N;
+    double dd;
+    Direct currentDir;
+    Options::BCType bcType;
+    BW lhsBandwidth, rhsBandwidth;
+
+    virtual ~AbstractDerivatives() = default;
+    // d(dataIn)/d(xi_dir) into dataOut (device pointers)
+    virtual void calc1stDerivField(const double *dataIn, double *dataOut) = 0;
+    // transformed periodic boundaries, rhs bandwidth 5 and 7
+    virtual void calc1stDerivField_TPB(const double *dataIn, double *dataOut, const double *Nm2,
+                                       const double *Nm1, const double *Np1, const double *Np2) = 0;
+    virtual void calc1stDerivField_TPB(const double *dataIn, double *dataOut, const double *Nm3,
+                                       const double *Nm2, const double *Nm1, const double *Np1,
+                                       const double *Np2, const double *Np3) = 0;
+    // one-sided wall value (host values, as the reference evaluates it)
+    virtual double calcNeumann(const double *f) = 0;
+    // batched form: n fields in one launch; accumulate adds into dataOut
+    virtual void calc1stDerivFields(int n, const double *const *in, double *const *out,
+                                    bool accumulate) = 0;
+};
+
+class LineDerivative : public AbstractDerivatives {
+  public:
+    LineDerivative(int scheme, const Domain &dom, Options::BCType bc, Direct dir, hipStream_t st);
+    ~LineDerivative() override;
+    void calc1stDerivField(const double *in, double *out) override;
+    void calc1stDerivField_TPB(const double *in, double *out, const double *Nm2, const double *Nm1,
+                               const double *Np1, const double *Np2) override;
+    void calc1stDerivField_TPB(const double *in, double *out, const double *Nm3, const double *Nm2,
+                               const double *Nm1, const double *Np1, const double *Np2,
+                               const double *Np3) override;
+    double calcNeumann(const double *f) override;
+    void calc1stDerivFields(int n, const double *const *in, double *const *out, bool accumulate) override;
+
+  protected:
+    int scheme_;
+    mc3d_lineop *op_ = nullptr;
+    hipStream_t st_;
+};
+
+// src/Pade6.hpp:9-259, src/Penta10.hpp:9-197, src/CD2.hpp:9-83
+struct Pade6 : LineDerivative {
+    Pade6(const Domain &d, Options::BCType bc, Direct dir, hipStream_t st)
+        : LineDerivative(MC3D_PADE6, d, bc, dir, st) {}
+};
+struct Penta10 : LineDerivative {
+    Penta10(const Domain &d, Options::BCType bc, Direct dir, hipStream_t st)
+        : LineDerivative(MC3D_PENTA10, d, bc, dir, st) {}
+};
+struct CD2 : LineDerivative {
+    CD2(const Domain &d, Options::BCType bc, Direct dir, hipStream_t st)
+        : LineDerivative(MC3D_CD2, d, bc, dir, st) {}
+};
+
+// ---------------------------------------------------------------------------
+// Filters, src/AbstractFilter.hpp:13-35
+// ---------------------------------------------------------------------------
+class AbstractFilter {
+  public:
+    int Nx, Ny, Nz, N;
+    AbstractDerivatives::Direct currentDir;
+    Options::BCType bcType;
+    virtual ~AbstractFilter() = default;
+    virtual void filterField(const double *dataIn, double *dataOut) = 0;
+    // batched form (in may equal out)
+    virtual void filterFields(int n, const double *const *in, double *const *out) = 0;
+};
+
+class LineFilter : public AbstractFilter {
+  public:
+    LineFilter(int kind, double alphaF, const Domain &dom, const int bc3[3],
+               AbstractDerivatives::Direct dir, hipStream_t st);
+    ~LineFilter() override;
+    void filterField(const double *in, double *out) override;
+    void filterFields(int n, const double *const *in, double *const *out) override;
+
+  protected:
+    mc3d_lineop *op_ = nullptr;
+    hipStream_t st_;
+};
+
+// src/Compact8Filter.hpp:9-139, src/Compact10Filter.hpp:9-308,
+// src/CommN5PadeTestFilter.hpp, src/CommN5ExpTestFilter.hpp
+struct Compact8Filter : LineFilter {
+    Compact8Filter(double af, const Domain &d, const int bc3[3], AbstractDerivatives::Direct dir, hipStream_t st)
+        : LineFilter(MC3D_COMPACT8, af, d, bc3, dir, st) {}
+};
+struct Compact10Filter : LineFilter {
+    Compact10Filter(double af, const Domain &d, const int bc3[3], AbstractDerivatives::Direct dir, hipStream_t st)
+        : LineFilter(MC3D_COMPACT10, af, d, bc3, dir, st) {}
+};
+struct CommN5PadeTestFilter : LineFilter {
+    CommN5PadeTestFilter(const Domain &d, const int bc3[3], AbstractDerivatives::Direct dir, hipStream_t st)
+        : LineFilter(MC3D_COMMN5PADE, 0.0, d, bc3, dir, st) {}
+};
+struct CommN5ExpTestFilter : LineFilter {
+    CommN5ExpTestFilter(const Domain &d, const int bc3[3], AbstractDerivatives::Direct dir, hipStream_t st)
+        : LineFilter(MC3D_COMMN5EXP, 0.0, d, bc3, dir, st) {}
+};
+
+class CurvilinearCSolver;
+
+// ---------------------------------------------------------------------------
+// SGS models, src/AbstractSGS.hpp:9-42
+// ---------------------------------------------------------------------------
+class AbstractSGS {
+  public:
+    CurvilinearCSolver *cs = nullptr;
+    Options::StatsAvgType musgsAvgType = Options::NONE;
+    std::unique_ptr<AbstractFilter> filtX, filtY, filtZ;
+    double *mu_sgs = nullptr, *taukk = nullptr, *k_sgs = nullptr;
+    bool coeffRangeFlag = false;
+    double *C = nullptr, *CI = nullptr, *Prt = nullptr;
+    virtual ~AbstractSGS() = default;
+    // gradients are passed as the computational-space derivatives dU[3], dV[3],
+    // dW[3]; the Cartesian tensor is formed in the kernels that use it.
+    virtual void getSGSViscosity(const double *const dU[3], const double *const dV[3],
+                                 const double *const dW[3], const double *rho) = 0;
+    virtual void calcMuSGSTaukk(const double *const dU[3], const double *const dV[3],
+                                const double *const dW[3], const double *rho, const double *rhoU,
+                                const double *rhoV, const double *rhoW) = 0;
+    virtual void calcKSGS(const double *const dT[3], const double *rho, const double *rhoU,
+                          const double *rhoV, const double *rhoW, const double *T) = 0;
+    // src/AbstractSGS.cpp:13-164
+    void doAveraging(const double *phi, double *phiF);
+};
+
+class VremanSGS : public AbstractSGS {
+  public:
+    explicit VremanSGS(CurvilinearCSolver *cs);
+    void getSGSViscosity(const double *const dU[3], const double *const dV[3],
+                         const double *const dW[3], const double *rho) override;
+    void calcMuSGSTaukk(const double *const *, const double *const *, const double *const *,
+                        const double *, const double *, const double *, const double *) override {}
+    void calcKSGS(const double *const *, const double *, const double *, const double *,
+                  const double *, const double *) override {}
+};
+
+// ---------------------------------------------------------------------------
+// RK integrators, src/AbstractRK.hpp:6-17
+// ---------------------------------------------------------------------------
+class AbstractRK {
+  public:
+    CurvilinearCSolver *cs = nullptr;
+    virtual ~AbstractRK() = default;
+    // runs full steps until the solver's end condition (maxTimeStep / maxTime)
+    virtual void executeSolverLoop();
+    // one full time step (all stages)
+    virtual void step() = 0;
+    // register update of the current stage (fused with the k2 assembly)
+    virtual void updateConservedData() = 0;
+    virtual int stages() const = 0;
+};
+
+class TVDRK3 : public AbstractRK {  // src/TVDRK3.hpp
+  public:
+    explicit TVDRK3(CurvilinearCSolver *c) { cs = c; }
+    void step() override;
+    void updateConservedData() override;
+    int stages() const override { return 3; }
+};
+class RK4 : public AbstractRK {  // src/RK4.hpp
+  public:
+    explicit RK4(CurvilinearCSolver *c) { cs = c; }
+    void step() override;
+    void updateConservedData() override;
+    int stages() const override { return 4; }
+};
+class KenRK4 : public AbstractRK {  // src/KenRK4.hpp
+  public:
+    explicit KenRK4(CurvilinearCSolver *c) { cs = c; }
+    void step() override;
+    void updateConservedData() override;
+    int stages() const override { return 5; }
+};
+class LSLDDRK4 : public AbstractRK {  // src/LSLDDRK4.hpp: alternating 5- and 6-stage steps
+  public:
+    explicit LSLDDRK4(CurvilinearCSolver *c) { cs = c; }
+    void step() override;
+    void updateConservedData() override;
+    int stages() const override { return rkDualStepStep == 1 ? 5 : 6; }
+    int rkDualStepStep = 1;
+};
+
+// ---------------------------------------------------------------------------
+// Solver, src/CurvilinearCSolver.hpp:25-379
+// ---------------------------------------------------------------------------
+class CurvilinearCSolver {
+  public:
+    CurvilinearCSolver(const Options &opt, hipStream_t st);
+    ~CurvilinearCSolver();
+
+    Options opt;
+    Domain dom;
+    Gas ig;
+    hipStream_t st;
+    long N;
+    int timeStep, filterTimeStep = 0, rkStep = 0;
+    bool rkLast = false, endFlag = false, firstStep = true;
+    double Pr_t = 0.9;
+
+    std::unique_ptr<AbstractDerivatives> derivX, derivY, derivZ;
+    std::unique_ptr<AbstractFilter> filtX, filtY, filtZ;
+    std::unique_ptr<AbstractSGS> les;
+    std::unique_ptr<AbstractRK> rk;
+
+    // device fields (one slab)
+    Metrics msh{};
+    double *Jm[10] = {};
+    Cons r1{}, rk_{}, k2{}, r2{};
+    Prim pr{};
+    double *dU[3] = {}, *dV[3] = {}, *dW[3] = {}, *dT[3] = {};
+    double *pre[15] = {}, *div[5] = {};
+    double *tmp[10] = {};
+    double *cfl_partial = nullptr;
+    StepScalars *sc = nullptr;
+
+    double *alloc_field();
+    void set_metrics_host(const double *const m[10]);
+    void setInitialConditions_host(const double *rho0, const double *U0, const double *V0,
+                                   const double *W0, const double *p0);
+
+    // stage methods, in the reference's call order
+    void preStep();               // calcDtFromCFL
+    void preSubStep();            // preStepDerivatives
+    void solveEqnSet() {}         // k2 assembly is fused into the RK stage kernel
+    void postSubStep() {}         // periodic boundaries: no boundary fix-ups
+    void updateData();            // filter (last stage / substeps) + EOS
+    void postStep();              // end conditions
+    void stage_update(int op, double a, double b, double c);
+    void filterConservedData();
+    void updateNonConservedData();
+    void run_stage(AbstractRK *r, int s, int S);
+
+    const Cons &state() const { return rkStep == 1 ? r1 : rk_; }
+    double *field_by_name(const std::string &name);
+
+  private:
+    void *slab_ = nullptr;
+    size_t slab_bytes_ = 0, slab_used_ = 0;
+    std::vector<void *> extra_;
+};
+
+}  // namespace mc3d

@@ -1,0 +1,83 @@
+// Pointwise kernels of the solver step (launch interface).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace mc3d {
+
+// Ideal gas, src/IdealGas.hpp:35-55
+struct Gas {
+    double gamma, Pr, R_gas, cp, T_ref, mu_ref;
+};
+
+struct Metrics {
+    const double *J, *J11, *J12, *J13, *J21, *J22, *J23, *J31, *J32, *J33;
+};
+
+struct Cons {  // one register set of conserved variables: rho, rhoU, rhoV, rhoW, rhoE
+    double *q[5];
+};
+
+struct Prim {  // non-conserved data
+    double *U, *V, *W, *p, *T, *mu, *sos, *Ucurv, *Vcurv, *Wcurv;
+};
+
+// time-step scalars kept on the device so whole steps can be captured
+struct StepScalars {
+    double dt;
+    double cfl;
+    double time;
+    double maxspeed;
+};
+
+enum RkOp {
+    kTvd1 = 0, kTvd2, kTvd3,          // TVDRK3 stages
+    kRk4First, kRk4Mid, kRk4Last,     // classic RK4
+    kLsFirst, kLsMid, kLsLast         // 2N low-storage (KenRK4, LSLDDRK4)
+};
+
+struct RkArgs {
+    Cons r1, rk, k2, r2;
+    const double *div[5];  // summed flux divergences (cont, mom1..3, engy)
+    const double *J;
+    const StepScalars *sc;
+    int op;
+    double a, b, c;        // stage coefficients
+};
+
+struct FluxArgs {
+    const double *dU[3], *dV[3], *dW[3], *dT[3];
+    Metrics m;
+    const double *U, *V, *W, *p, *mu, *mu_sgs, *k_sgs;
+    const double *rho, *rhoU, *rhoV, *rhoW, *rhoE;
+    const double *Ucurv, *Vcurv, *Wcurv;
+    double *pre[15];
+    int les;  // 0 none, 1 Vreman-type (k from mu_sgs/Pr_t), 2 DSM (k_sgs)
+    double Pr_t;
+};
+
+hipError_t launch_init_state(const double *rho0, const double *U0, const double *V0,
+                             const double *W0, const double *p0, Cons r1, Prim pr, Metrics m,
+                             Gas g, long n, hipStream_t st);
+hipError_t launch_eos(Cons in, Prim pr, Metrics m, Gas g, long n, hipStream_t st);
+// CFL: partial maxima into partial[nblocks], then a single-block finish that
+// sets dt (const CFL) or cfl (const dt) and advances time.
+int cfl_blocks(long n);
+hipError_t launch_cfl(Prim pr, Metrics m, double dx, double dy, double dz, double *partial,
+                      long n, hipStream_t st);
+hipError_t launch_cfl_finish(const double *partial, int nblocks, StepScalars *sc, int const_cfl,
+                             double cfl_target, int first_step, double time0, hipStream_t st);
+hipError_t launch_flux(const FluxArgs &a, Gas g, long n, hipStream_t st);
+hipError_t launch_rhs_rk(const RkArgs &a, long n, hipStream_t st);
+hipError_t launch_vreman(const double *const dU[3], const double *const dV[3],
+                         const double *const dW[3], Metrics m, const double *rho, double *mu_sgs,
+                         double dx, double dy, double dz, long n, hipStream_t st);
+hipError_t launch_fill(double *p, double v, long n, hipStream_t st);
+// doAveraging XI1/XI2/XI3: average along direction dir (in place allowed)
+hipError_t launch_xi_average(const double *phi, double *phiF, int dir, int Nx, int Ny, int Nz,
+                             hipStream_t st);
+// doAveraging GLOBAL: phiF = mean(phi); partial needs cfl_blocks(n) doubles
+hipError_t launch_global_mean(const double *phi, double *phiF, double *partial, long n,
+                              hipStream_t st);
+
+}  // namespace mc3d

@@ -1,0 +1,418 @@
+// Pointwise kernels of the solver step: initial conversion, equation of state,
+// CFL reduction, curvilinear flux assembly, RHS + Runge-Kutta register update,
+// Vreman eddy viscosity.
+//
+// All are HBM-bound streaming kernels.  This file is compiled with
+// -ffp-contract=off: separately rounded products and sums, in the reference's
+// evaluation order, make every pointwise result reproduce the reference's
+// rounding; FMA buys nothing at this arithmetic intensity.
+#include <hip/hip_runtime.h>
+
+#include "pointwise.h"
+
+namespace mc3d {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline dim3 grid_for(long n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
+
+__device__ __forceinline__ double solve_mu(const Gas &g, double T) {
+    return g.mu_ref * pow(T / g.T_ref, 0.76);
+}
+__device__ __forceinline__ double solve_p(const Gas &g, double rho, double rhoE, double U, double V,
+                                          double W) {
+    return (g.gamma - 1) * (rhoE - 0.5 * rho * (U * U + V * V + W * W));
+}
+__device__ __forceinline__ double solve_T(const Gas &g, double rho, double p) {
+    return p / (rho * g.R_gas);
+}
+__device__ __forceinline__ double solve_sos(const Gas &g, double rho, double p) {
+    return sqrt(g.gamma * p / rho);
+}
+
+// setInitialConditions, src/CurvilinearCSolver_BC.cpp:30-63
+__global__ void k_init_state(const double *__restrict__ rho0, const double *__restrict__ U0,
+                             const double *__restrict__ V0, const double *__restrict__ W0,
+                             const double *__restrict__ p0, Cons r1, Prim pr, Metrics m, Gas g,
+                             long n) {
+    const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= n) return;
+    const double U = U0[ip], V = V0[ip], W = W0[ip], p = p0[ip], rho = rho0[ip];
+    pr.U[ip] = U; pr.V[ip] = V; pr.W[ip] = W; pr.p[ip] = p;
+    r1.q[0][ip] = rho;
+    r1.q[1][ip] = rho * U;
+    r1.q[2][ip] = rho * V;
+    r1.q[3][ip] = rho * W;
+    pr.Ucurv[ip] = m.J11[ip] * U + m.J12[ip] * V + m.J13[ip] * W;
+    pr.Vcurv[ip] = m.J21[ip] * U + m.J22[ip] * V + m.J23[ip] * W;
+    pr.Wcurv[ip] = m.J31[ip] * U + m.J32[ip] * V + m.J33[ip] * W;
+    r1.q[4][ip] = p / (g.gamma - 1.0) + 0.5 * rho * (U * U + V * V + W * W);
+    const double T = solve_T(g, rho, p);
+    pr.T[ip] = T;
+    pr.mu[ip] = solve_mu(g, T);
+    pr.sos[ip] = solve_sos(g, rho, p);
+}
+
+// updateNonConservedData, src/CurvilinearCSolver_Core.cpp:1158-1198
+__global__ void k_eos(Cons c, Prim pr, Metrics m, Gas g, long n) {
+    const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= n) return;
+    const double rho = c.q[0][ip];
+    const double U = c.q[1][ip] / rho, V = c.q[2][ip] / rho, W = c.q[3][ip] / rho;
+    const double p = solve_p(g, rho, c.q[4][ip], U, V, W);
+    const double T = solve_T(g, rho, p);
+    pr.U[ip] = U; pr.V[ip] = V; pr.W[ip] = W;
+    pr.p[ip] = p;
+    pr.T[ip] = T;
+    pr.mu[ip] = solve_mu(g, T);
+    pr.sos[ip] = solve_sos(g, rho, p);
+    pr.Ucurv[ip] = m.J11[ip] * U + m.J12[ip] * V + m.J13[ip] * W;
+    pr.Vcurv[ip] = m.J21[ip] * U + m.J22[ip] * V + m.J23[ip] * W;
+    pr.Wcurv[ip] = m.J31[ip] * U + m.J32[ip] * V + m.J33[ip] * W;
+}
+
+// calcDtFromCFL wave-speed maximum, src/CurvilinearCSolver_Core.cpp:197-207
+__global__ __launch_bounds__(kBlock) void k_cfl(Prim pr, Metrics m, double dx, double dy,
+                                                double dz, double *partial, long n) {
+    __shared__ double red[kBlock];
+    double v = -100000.0;
+    for (long ip = (long)blockIdx.x * blockDim.x + threadIdx.x; ip < n;
+         ip += (long)gridDim.x * blockDim.x) {
+        const double s = pr.sos[ip];
+        const double u = m.J11[ip] * (fabs(pr.U[ip]) + s) / dx + m.J22[ip] * (fabs(pr.V[ip]) + s) / dy +
+                         m.J33[ip] * (fabs(pr.W[ip]) + s) / dz;
+        v = u > v ? u : v;
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_cfl_finish(const double *partial, int nb,
+                                                       StepScalars *sc, int const_cfl,
+                                                       double cfl_target, int first_step,
+                                                       double time0) {
+    __shared__ double red[kBlock];
+    double v = -100000.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) v = fmax(v, partial[i]);
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double mx = red[0];
+        sc->maxspeed = mx;
+        if (const_cfl) sc->dt = cfl_target / mx;
+        else sc->cfl = sc->dt * mx;
+        if (first_step) sc->time = time0;
+        else sc->time = sc->time + sc->dt;
+    }
+}
+
+// Stress, heat flux and contravariant fluxes / J, src/CurvilinearCSolver_Core.cpp:548-561,615-752
+__global__ __launch_bounds__(kBlock) void k_flux(FluxArgs a, Gas g, long n) {
+    const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= n) return;
+    const Metrics &m = a.m;
+    const double J11 = m.J11[ip], J12 = m.J12[ip], J13 = m.J13[ip];
+    const double J21 = m.J21[ip], J22 = m.J22[ip], J23 = m.J23[ip];
+    const double J31 = m.J31[ip], J32 = m.J32[ip], J33 = m.J33[ip];
+    // velocity gradient tensor, gradU[i][j] = d u_i / d x_j
+    const double dU1 = a.dU[0][ip], dU2 = a.dU[1][ip], dU3 = a.dU[2][ip];
+    const double dV1 = a.dV[0][ip], dV2 = a.dV[1][ip], dV3 = a.dV[2][ip];
+    const double dW1 = a.dW[0][ip], dW2 = a.dW[1][ip], dW3 = a.dW[2][ip];
+    const double dUdx = J11 * dU1 + J21 * dU2 + J31 * dU3;
+    const double dUdy = J12 * dU1 + J22 * dU2 + J32 * dU3;
+    const double dUdz = J13 * dU1 + J23 * dU2 + J33 * dU3;
+    const double dVdx = J11 * dV1 + J21 * dV2 + J31 * dV3;
+    const double dVdy = J12 * dV1 + J22 * dV2 + J32 * dV3;
+    const double dVdz = J13 * dV1 + J23 * dV2 + J33 * dV3;
+    const double dWdx = J11 * dW1 + J21 * dW2 + J31 * dW3;
+    const double dWdy = J12 * dW1 + J22 * dW2 + J32 * dW3;
+    const double dWdz = J13 * dW1 + J23 * dW2 + J33 * dW3;
+    double T11 = (4.0 / 3.0) * dUdx - (2.0 / 3.0) * dVdy - (2.0 / 3.0) * dWdz;
+    double T22 = -(2.0 / 3.0) * dUdx + (4.0 / 3.0) * dVdy - (2.0 / 3.0) * dWdz;
+    double T33 = -(2.0 / 3.0) * dUdx - (2.0 / 3.0) * dVdy + (4.0 / 3.0) * dWdz;
+    double T12 = dUdy + dVdx, T13 = dUdz + dWdx, T23 = dVdz + dWdy;
+    const double mu = a.mu[ip];
+    double mu_eff, k_eff;
+    if (a.les) {
+        mu_eff = mu + a.mu_sgs[ip];
+        if (a.les == 2) k_eff = g.cp * mu / g.Pr + a.k_sgs[ip];
+        else k_eff = g.cp * (mu / g.Pr + a.mu_sgs[ip] / a.Pr_t);
+        if (mu_eff < 0.0) mu_eff = 0.0;
+        if (k_eff < 0.0) k_eff = 0.0;
+    } else {
+        mu_eff = mu;
+        k_eff = (g.cp / g.Pr) * mu;
+    }
+    T11 *= mu_eff; T22 *= mu_eff; T33 *= mu_eff;
+    T12 *= mu_eff; T13 *= mu_eff; T23 *= mu_eff;
+    const double T21 = T12, T31 = T13, T32 = T23;
+    const double dT1 = a.dT[0][ip], dT2 = a.dT[1][ip], dT3 = a.dT[2][ip];
+    const double q1 = k_eff * (J11 * dT1 + J21 * dT2 + J31 * dT3);
+    const double q2 = k_eff * (J12 * dT1 + J22 * dT2 + J32 * dT3);
+    const double q3 = k_eff * (J13 * dT1 + J23 * dT2 + J33 * dT3);
+    const double U = a.U[ip], V = a.V[ip], W = a.W[ip], p = a.p[ip];
+    const double b1 = U * T11 + V * T12 + W * T13 + q1;
+    const double b2 = U * T21 + V * T22 + W * T23 + q2;
+    const double b3 = U * T31 + V * T32 + W * T33 + q3;
+    const double iJ = 1.0 / m.J[ip];
+    const double Uc = a.Ucurv[ip], Vc = a.Vcurv[ip], Wc = a.Wcurv[ip];
+    const double rho = a.rho[ip], rhoU = a.rhoU[ip], rhoV = a.rhoV[ip], rhoW = a.rhoW[ip];
+    const double rhoE = a.rhoE[ip];
+    a.pre[0][ip] = iJ * (rho * Uc);
+    a.pre[5][ip] = iJ * (rho * Vc);
+    a.pre[10][ip] = iJ * (rho * Wc);
+    double F1, F2, F3, G1, G2, G3;
+    F1 = rhoU * Uc + J11 * p; F2 = rhoU * Vc + J21 * p; F3 = rhoU * Wc + J31 * p;
+    G1 = J11 * T11 + J12 * T21 + J13 * T31;
+    G2 = J21 * T11 + J22 * T21 + J23 * T31;
+    G3 = J31 * T11 + J32 * T21 + J33 * T31;
+    a.pre[1][ip] = iJ * (F1 - G1); a.pre[6][ip] = iJ * (F2 - G2); a.pre[11][ip] = iJ * (F3 - G3);
+    F1 = rhoV * Uc + J12 * p; F2 = rhoV * Vc + J22 * p; F3 = rhoV * Wc + J32 * p;
+    G1 = J11 * T12 + J12 * T22 + J13 * T32;
+    G2 = J21 * T12 + J22 * T22 + J23 * T32;
+    G3 = J31 * T12 + J32 * T22 + J33 * T32;
+    a.pre[2][ip] = iJ * (F1 - G1); a.pre[7][ip] = iJ * (F2 - G2); a.pre[12][ip] = iJ * (F3 - G3);
+    F1 = rhoW * Uc + J13 * p; F2 = rhoW * Vc + J23 * p; F3 = rhoW * Wc + J33 * p;
+    G1 = J11 * T13 + J12 * T23 + J13 * T33;
+    G2 = J21 * T13 + J22 * T23 + J23 * T33;
+    G3 = J31 * T13 + J32 * T23 + J33 * T33;
+    a.pre[3][ip] = iJ * (F1 - G1); a.pre[8][ip] = iJ * (F2 - G2); a.pre[13][ip] = iJ * (F3 - G3);
+    F1 = (rhoE + p) * Uc; F2 = (rhoE + p) * Vc; F3 = (rhoE + p) * Wc;
+    G1 = J11 * b1 + J12 * b2 + J13 * b3;
+    G2 = J21 * b1 + J22 * b2 + J23 * b3;
+    G3 = J31 * b1 + J32 * b2 + J33 * b3;
+    a.pre[4][ip] = iJ * (F1 - G1); a.pre[9][ip] = iJ * (F2 - G2); a.pre[14][ip] = iJ * (F3 - G3);
+}
+
+// one conserved variable: k2 and the register update of stage op
+__device__ __forceinline__ void rk_update(const RkArgs &a, long ip, int q, double k2) {
+    double *r1 = a.r1.q[q], *rk = a.rk.q[q], *r2 = a.r2.q[q];
+    switch (a.op) {
+        case kTvd1: rk[ip] = r1[ip] + a.c * k2; break;                               // TVDRK3.hpp:62-80
+        case kTvd2: rk[ip] = a.a * r1[ip] + a.b * rk[ip] + a.c * k2; break;          // :82-102
+        case kTvd3: r2[ip] = a.a * r1[ip] + a.b * rk[ip] + a.c * k2; break;          // :104-124
+        case kRk4First: r2[ip] = r1[ip] + a.a * k2; rk[ip] = r1[ip] + a.b * k2; break;  // RK4.hpp:53-70
+        case kRk4Mid: r2[ip] += a.a * k2; rk[ip] = r1[ip] + a.b * k2; break;            // :72-106
+        case kRk4Last: r2[ip] += a.a * k2; break;                                       // :108-118
+        case kLsFirst: rk[ip] = r1[ip] + a.b * k2; r1[ip] = k2; break;                  // KenRK4.hpp:58-74
+        case kLsMid: { double v = k2 + a.a * r1[ip]; r1[ip] = v; rk[ip] += a.b * v; break; }  // :76-127
+        case kLsLast: { double v = k2 + a.a * r1[ip]; r1[ip] = v; r2[ip] = rk[ip] + a.b * v; break; }  // :128-145
+    }
+}
+
+// solveContinuity .. solveEnergy (src/CurvilinearCSolver_Core.cpp:790-947) fused
+// with updateConservedData.  Momentum increments accumulate into the previous
+// k2 before scaling (:839-840,872-873,904-905).
+__global__ __launch_bounds__(kBlock) void k_rhs_rk(RkArgs a, long n) {
+    const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= n) return;
+    const double J = a.J[ip];
+    const double dtJ = a.sc->dt * J;
+    const double z = (0.0 + 0.0) / J;
+    // continuity
+    {
+        const double k2 = dtJ * (-a.div[0][ip] + z);
+        a.k2.q[0][ip] = k2;
+        rk_update(a, ip, 0, k2);
+    }
+#pragma unroll
+    for (int q = 1; q < 4; ++q) {
+        double *kk = a.k2.q[q];
+        double k2 = kk[ip] + (-a.div[q][ip] + z);
+        k2 = k2 * dtJ;
+        kk[ip] = k2;
+        rk_update(a, ip, q, k2);
+    }
+    {
+        double k2 = -a.div[4][ip] + z;
+        k2 = k2 * dtJ;
+        a.k2.q[4][ip] = k2;
+        rk_update(a, ip, 4, k2);
+    }
+}
+
+// Vreman eddy viscosity, src/VremanSGS.hpp:43-92 (abeta as the reference sums it)
+__global__ __launch_bounds__(kBlock) void k_vreman(const double *dU0, const double *dU1,
+                                                   const double *dU2, const double *dV0,
+                                                   const double *dV1, const double *dV2,
+                                                   const double *dW0, const double *dW1,
+                                                   const double *dW2, Metrics m,
+                                                   const double *__restrict__ rho, double *mu_sgs,
+                                                   double dx, double dy, double dz, long n) {
+    const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= n) return;
+    const double J11 = m.J11[ip], J12 = m.J12[ip], J13 = m.J13[ip];
+    const double J21 = m.J21[ip], J22 = m.J22[ip], J23 = m.J23[ip];
+    const double J31 = m.J31[ip], J32 = m.J32[ip], J33 = m.J33[ip];
+    const double u1 = dU0[ip], u2 = dU1[ip], u3 = dU2[ip];
+    const double v1 = dV0[ip], v2 = dV1[ip], v3 = dV2[ip];
+    const double w1 = dW0[ip], w2 = dW1[ip], w3 = dW2[ip];
+    const double d1v1 = J11 * u1 + J21 * u2 + J31 * u3;
+    const double d2v1 = J12 * u1 + J22 * u2 + J32 * u3;
+    const double d3v1 = J13 * u1 + J23 * u2 + J33 * u3;
+    const double d1v2 = J11 * v1 + J21 * v2 + J31 * v3;
+    const double d2v2 = J12 * v1 + J22 * v2 + J32 * v3;
+    const double d3v2 = J13 * v1 + J23 * v2 + J33 * v3;
+    const double d1v3 = J11 * w1 + J21 * w2 + J31 * w3;
+    const double d2v3 = J12 * w1 + J22 * w2 + J32 * w3;
+    const double d3v3 = J13 * w1 + J23 * w2 + J33 * w3;
+    const double vol = dx * dy * dz / m.J[ip];
+    const double del = pow(vol, 1.0 / 3.0);
+    const double d1 = del * del, d2 = d1, d3 = d1;
+    const double b11 = d1 * d1v1 * d1v1 + d2 * d2v1 * d2v1 + d3 * d3v1 * d3v1;
+    const double b12 = d1 * d1v1 * d1v2 + d2 * d2v1 * d2v2 + d3 * d3v1 * d3v2;
+    const double b13 = d1 * d1v1 * d1v3 + d2 * d2v1 * d2v3 + d3 * d3v1 * d3v3;
+    const double b22 = d1 * d1v2 * d1v2 + d2 * d2v2 * d2v2 + d3 * d3v2 * d3v2;
+    const double b23 = d1 * d1v2 * d1v3 + d2 * d2v2 * d2v3 + d3 * d3v2 * d3v3;
+    const double b33 = d1 * d1v3 * d1v3 + d2 * d2v3 * d2v3 + d3 * d3v3 * d3v3;
+    const double abeta = d1v1 * d1v1 + d1v2 * d1v2 + d1v3 * d1v3 + d2v1 * d2v1 + d2v2 * d2v2 +
+                         d2v3 * d2v3 + d3v1 * d3v1 + d2v3 * d2v3 + d3v3 * d3v3;
+    double bbeta = b11 * b22 - (b12 * b12) + b11 * b33 - (b13 * b13) + b22 * b33 - (b23 * b23);
+    const double eps = 1e-12;
+    if (bbeta < eps) bbeta = eps;
+    const double ss = sqrt(bbeta / (abeta + eps));
+    const double c = 2.5 * 0.17 * 0.17;
+    const double ms = c * rho[ip] * ss;
+    const double cap = 0.05 * d1 * sqrt(2.0 * abeta);
+    mu_sgs[ip] = ms < cap ? ms : cap;
+}
+
+__global__ void k_fill(double *p, double v, long n) {
+    const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip < n) p[ip] = v;
+}
+
+// src/AbstractSGS.cpp:15-131: each term divided by the line length, summed in
+// index order, then broadcast back along the line
+__global__ void k_xi_average(const double *phi, double *phiF, int dir, int Nx, int Ny, int Nz) {
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nl = dir == 0 ? Nx : dir == 1 ? Ny : Nz;
+    const long nlines = (long)Nx * Ny * Nz / nl;
+    if (q >= nlines) return;
+    long base, stride;
+    if (dir == 0) { base = q * Nx; stride = 1; }
+    else if (dir == 1) { base = (q / Nx) * (long)Nx * Ny + q % Nx; stride = Nx; }
+    else { base = q; stride = (long)Nx * Ny; }
+    double acc = 0.0;
+    for (int i = 0; i < nl; ++i) acc += phi[base + i * stride] / (double)nl;
+    for (int i = 0; i < nl; ++i) phiF[base + i * stride] = acc;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_partial(const double *phi, double *partial, long n) {
+    __shared__ double red[kBlock];
+    double v = 0.0;
+    for (long ip = (long)blockIdx.x * blockDim.x + threadIdx.x; ip < n; ip += (long)gridDim.x * blockDim.x)
+        v += phi[ip];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_mean_fill(const double *partial, int nb, double *phiF, long n) {
+    __shared__ double red[kBlock];
+    __shared__ double mean;
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) v += partial[i];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) mean = red[0] / (double)n;
+    __syncthreads();
+    for (long ip = (long)blockIdx.x * blockDim.x + threadIdx.x; ip < n; ip += (long)gridDim.x * blockDim.x)
+        phiF[ip] = mean;
+}
+
+}  // namespace
+
+hipError_t launch_init_state(const double *rho0, const double *U0, const double *V0,
+                             const double *W0, const double *p0, Cons r1, Prim pr, Metrics m,
+                             Gas g, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_init_state, grid_for(n), dim3(kBlock), 0, st, rho0, U0, V0, W0, p0, r1, pr,
+                       m, g, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_eos(Cons in, Prim pr, Metrics m, Gas g, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_eos, grid_for(n), dim3(kBlock), 0, st, in, pr, m, g, n);
+    return hipGetLastError();
+}
+
+int cfl_blocks(long n) {
+    long b = (n + kBlock - 1) / kBlock;
+    return (int)(b < 2048 ? b : 2048);
+}
+
+hipError_t launch_cfl(Prim pr, Metrics m, double dx, double dy, double dz, double *partial, long n,
+                      hipStream_t st) {
+    hipLaunchKernelGGL(k_cfl, dim3(cfl_blocks(n)), dim3(kBlock), 0, st, pr, m, dx, dy, dz, partial, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_cfl_finish(const double *partial, int nblocks, StepScalars *sc, int const_cfl,
+                             double cfl_target, int first_step, double time0, hipStream_t st) {
+    hipLaunchKernelGGL(k_cfl_finish, dim3(1), dim3(kBlock), 0, st, partial, nblocks, sc, const_cfl,
+                       cfl_target, first_step, time0);
+    return hipGetLastError();
+}
+
+hipError_t launch_flux(const FluxArgs &a, Gas g, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_flux, grid_for(n), dim3(kBlock), 0, st, a, g, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_rhs_rk(const RkArgs &a, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_rhs_rk, grid_for(n), dim3(kBlock), 0, st, a, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_vreman(const double *const dU[3], const double *const dV[3],
+                         const double *const dW[3], Metrics m, const double *rho, double *mu_sgs,
+                         double dx, double dy, double dz, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_vreman, grid_for(n), dim3(kBlock), 0, st, dU[0], dU[1], dU[2], dV[0], dV[1],
+                       dV[2], dW[0], dW[1], dW[2], m, rho, mu_sgs, dx, dy, dz, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(double *p, double v, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_fill, grid_for(n), dim3(kBlock), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_xi_average(const double *phi, double *phiF, int dir, int Nx, int Ny, int Nz,
+                             hipStream_t st) {
+    const int nl = dir == 0 ? Nx : dir == 1 ? Ny : Nz;
+    const long nlines = (long)Nx * Ny * Nz / nl;
+    hipLaunchKernelGGL(k_xi_average, grid_for(nlines), dim3(kBlock), 0, st, phi, phiF, dir, Nx, Ny, Nz);
+    return hipGetLastError();
+}
+
+hipError_t launch_global_mean(const double *phi, double *phiF, double *partial, long n,
+                              hipStream_t st) {
+    const int nb = cfl_blocks(n);
+    hipLaunchKernelGGL(k_sum_partial, dim3(nb), dim3(kBlock), 0, st, phi, partial, n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // one block reduces the partials; the fill is grid-strided over the same single block
+    hipLaunchKernelGGL(k_mean_fill, dim3(1), dim3(kBlock), 0, st, partial, nb, phiF, n);
+    return hipGetLastError();
+}
+
+}  // namespace mc3d

@@ -1,0 +1,135 @@
+"""Decaying compressible HIT workload: options deck, mesh metrics and initial field.
+
+The initial velocity follows the Rogallo construction of the reference's HIT
+generator (tests/hit/generate_init_hit.f90:246-363): random phases phi1, phi2,
+phi3 per wave vector, a = sqrt(E/(2 pi k^2)) e^{i phi1} cos phi3,
+b = sqrt(E/(2 pi k^2)) e^{i phi2} sin phi3, the solenoidal projection onto the
+plane normal to k, odd-ball (Nyquist) modes and the mean removed, inverse FFT.
+The spectrum is E(k) = (k/k0)^4 exp(-2 (k/k0)^2).  Phases come from numpy's
+PCG64 generator with an explicit seed (the Fortran LCG shuffle is not
+reproduced).  The field is scaled so that the turbulent Mach number
+M_t = sqrt(<u.u>)/<c> equals the requested value (the definition of
+tests/hit/Samtaney_D4/MPI_3DCompact.cpp:455,485-486,535-537), with rho = 1,
+p = 1/gamma (so T = 1 and c = 1).
+
+The mesh is the HIT identity map x = 2 pi xi (tests/hit/Samtaney_D4/
+AlgebraicSingleBlockMesh.hpp:151-156), whose conservative metrics are
+J = (2 pi)^-3, J11 = J22 = J33 = 1/(2 pi) and zero off-diagonal terms.
+"""
+import numpy as np
+
+GAMMA = 1.4
+
+
+def hit_velocity(N, k0=4.0, Mt=0.3, seed=0):
+    """Solenoidal random velocity (U, V, W), each of shape (N, N, N) indexed [k][j][i]."""
+    rng = np.random.default_rng(seed)
+    kf = np.fft.fftfreq(N, d=1.0 / N)          # integer wavenumbers, FFT order
+    kr = np.fft.rfftfreq(N, d=1.0 / N)
+    kz, ky, kx = np.meshgrid(kf, kf, kr, indexing="ij")
+    k = np.sqrt(kx ** 2 + ky ** 2 + kz ** 2)
+    kxy = np.sqrt(kx ** 2 + ky ** 2)
+    shape = k.shape
+    ph1 = 2 * np.pi * (rng.random(shape) - 0.5)
+    ph2 = 2 * np.pi * (rng.random(shape) - 0.5)
+    ph3 = 2 * np.pi * (rng.random(shape) - 0.5)
+    kmin = 1e-10
+    E = (k / k0) ** 4 * np.exp(-2.0 * (k / k0) ** 2)
+    amp = np.where(k > kmin, np.sqrt(E / (2 * np.pi * np.maximum(k, kmin) ** 2)), 0.0)
+    a = amp * np.exp(1j * ph1) * np.cos(ph3)
+    b = amp * np.exp(1j * ph2) * np.sin(ph3)
+    safe_k = np.maximum(k, kmin)
+    safe_kxy = np.maximum(kxy, kmin)
+    on_axis = kxy <= kmin
+    ux = np.where(on_axis, (a + b) / np.sqrt(2.0), (a * k * ky + b * kx * kz) / (safe_k * safe_kxy))
+    uy = np.where(on_axis, (b - a) / np.sqrt(2.0), (b * ky * kz - a * k * kx) / (safe_k * safe_kxy))
+    uz = -b * kxy / safe_k
+    out = []
+    for uk in (ux, uy, uz):
+        uk = np.where(k > kmin, uk, 0.0)
+        # remove odd-ball wave numbers (the Nyquist planes) and the mean
+        uk[N // 2, :, :] = 0.0
+        uk[:, N // 2, :] = 0.0
+        uk[:, :, -1] = 0.0
+        uk[0, 0, 0] = 0.0
+        out.append(np.fft.irfftn(uk, s=(N, N, N), axes=(0, 1, 2)))
+    U, V, W = out
+    urms = np.sqrt(np.mean(U ** 2 + V ** 2 + W ** 2))
+    scale = Mt * 1.0 / urms  # mean speed of sound is 1 at rho = 1, p = 1/gamma
+    return U * scale, V * scale, W * scale
+
+
+def hit_initial_condition(N, k0=4.0, Mt=0.3, seed=0):
+    U, V, W = hit_velocity(N, k0, Mt, seed)
+    rho = np.ones((N, N, N))
+    p = np.full((N, N, N), 1.0 / GAMMA)
+    return dict(rho=rho, U=U, V=V, W=W, p=p)
+
+
+def identity_metrics(N):
+    """J, J11..J33 of x = 2 pi xi (10 arrays, shape (N, N, N))."""
+    L = 2 * np.pi
+    one = np.ones((N, N, N))
+    zero = np.zeros((N, N, N))
+    J = one / L ** 3
+    d = one / L
+    return [J, d, zero, zero, zero, d, zero, zero, zero, d]
+
+
+def options_text(N, fd="PADE6", filt="COMPACT10", alpha_f=0.49, rk="TVDRK3", cfl=0.5,
+                 mu_ref=1.0 / 1600.0, les="NOMODEL", les_avg="NONE", max_steps=100000,
+                 filter_step=1, substep_filtering=False, dsm_filter=1, use_taukk=True,
+                 coeff_range=False, ts_type="CONST_CFL", dt=None):
+    """MPI_3DCompact.in text for a periodic HIT box of N^3 points."""
+    fds = fd if isinstance(fd, (list, tuple)) else (fd, fd, fd)
+    lines = [
+        "[RESTART]", "FROMRESTART = FALSE", "ONLYGRIDFROMRESTART = FALSE",
+        "SPONGEAVGFROMRESTART = FALSE",
+        "[TIMESTEPPING]", f"TSTYPE = {ts_type}",
+        (f"CFL = {cfl!r}" if ts_type == "CONST_CFL" else f"DT = {dt!r}"),
+        "MAXTIME = 1000000.0", f"MAXTIMESTEP = {max_steps}", f"FILTERSTEP = {filter_step}",
+        f"SUBSTEPFILTERING = {'TRUE' if substep_filtering else 'FALSE'}",
+        "CHECKSTEP = 1000000", "DUMPSTEP = 1000000",
+        "[STATS]", "STATSAVGTYPE = NONE", "VELOCITYSTATS = FALSE", "THERMOSTATS = FALSE",
+        "[BC]", "BCXTYPE = PERIODIC_SOLVE", "BCYTYPE = PERIODIC_SOLVE", "BCZTYPE = PERIODIC_SOLVE",
+        "BCX0 = PERIODIC", "BCX1 = PERIODIC", "BCY0 = PERIODIC", "BCY1 = PERIODIC",
+        "BCZ0 = PERIODIC", "BCZ1 = PERIODIC",
+    ]
+    L = repr(2 * np.pi)
+    for a in "XYZ":
+        for b in "XYZ":
+            lines.append(f"PERIODICDISP{a}_{b} = {L if a == b else 0}")
+    lines += [
+        "[DOMAIN]", f"NX = {N}", f"NY = {N}", f"NZ = {N}",
+        "[PENCILDECOMP]", "PROW = 0", "PCOL = 0",
+        "[SOLVER]", f"ALPHAF = {alpha_f!r}", f"MU_REF = {mu_ref!r}", "USETIMING = FALSE",
+        f"XFDTYPE = {fds[0]}", f"YFDTYPE = {fds[1]}", f"ZFDTYPE = {fds[2]}",
+        f"FILTERTYPE = {filt}", f"RKTYPE = {rk}",
+        "[LES]", f"LESMODEL = {les}", f"LESAVERAGING = {les_avg}", f"DSMTESTFILTER = {dsm_filter}",
+        f"DSMUSETAUKK = {'TRUE' if use_taukk else 'FALSE'}",
+        f"DSMCOEFFRANGE = {'TRUE' if coeff_range else 'FALSE'}",
+    ]
+    return "\n".join(lines) + "\n"
+
+
+_FD = {"CD2": 0, "PADE6": 1, "PENTA10": 2}
+_FILT = {"COMPACT8": 0, "COMPACT10": 1}
+_RK = {"TVDRK3": 0, "RK4": 1, "KENRK4": 2, "LSLDDRK4": 3}
+_LES = {"NOMODEL": 0, "NONE": 0, "VREMAN": 1, "DSM": 2}
+_AVG = {"NONE": 0, "XI1_AVG": 1, "XI2_AVG": 2, "XI3_AVG": 3, "LOCAL": 4, "GLOBAL": 5}
+
+
+def oracle_config(N, fd="PADE6", filt="COMPACT10", alpha_f=0.49, rk="TVDRK3", cfl=0.5,
+                  mu_ref=1.0 / 1600.0, les="NOMODEL", les_avg="NONE", filter_step=1,
+                  substep_filtering=False, dsm_filter=1, use_taukk=True, coeff_range=False,
+                  ts_type="CONST_CFL", dt=None, **_):
+    """The same case as options_text(), as the field dict of the CPU oracle's config."""
+    fds = fd if isinstance(fd, (list, tuple)) else (fd, fd, fd)
+    return dict(Nx=N, Ny=N, Nz=N, bc=(0, 0, 0), fd=tuple(_FD[f] for f in fds),
+                filter=_FILT[filt], alphaF=alpha_f, rk=_RK[rk], les=_LES[les],
+                les_avg=_AVG[les_avg], dsm_test_filter=dsm_filter, dsm_use_taukk=int(use_taukk),
+                dsm_coeff_range=int(coeff_range), mu_ref=mu_ref,
+                ts_type=1 if ts_type == "CONST_CFL" else 0, cfl=cfl if cfl else 0.0,
+                dt=dt if dt else 0.0, filter_step=filter_step,
+                substep_filtering=int(substep_filtering), dx=1.0 / N, dy=1.0 / N, dz=1.0 / N,
+                timestep0=0, time0=0.0)

@@ -1,0 +1,99 @@
+"""Python handle over the whole-solver C ABI (mc3d_solver_*)."""
+import ctypes
+
+import numpy as np
+
+from ._lib import check, lib
+
+_VP = ctypes.c_void_p
+_D = ctypes.POINTER(ctypes.c_double)
+
+_SIGS = [
+    ("mc3d_solver_create", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_VP)]),
+    ("mc3d_solver_set_metrics", ctypes.c_int, [_VP, ctypes.POINTER(_D)]),
+    ("mc3d_solver_set_initial", ctypes.c_int, [_VP, _D, _D, _D, _D, _D]),
+    ("mc3d_solver_step", ctypes.c_int, [_VP, ctypes.c_int]),
+    ("mc3d_solver_run", ctypes.c_int, [_VP]),
+    ("mc3d_solver_get_field", ctypes.c_int, [_VP, ctypes.c_char_p, _D]),
+    ("mc3d_solver_field_device", ctypes.c_int, [_VP, ctypes.c_char_p, ctypes.POINTER(_VP)]),
+    ("mc3d_solver_scalars", ctypes.c_int, [_VP, _D, _D, ctypes.POINTER(ctypes.c_int)]),
+    ("mc3d_solver_stream", ctypes.c_int, [_VP, ctypes.POINTER(_VP)]),
+    ("mc3d_solver_sync", ctypes.c_int, [_VP]),
+    ("mc3d_solver_destroy", None, [_VP]),
+]
+
+
+def _bind():
+    L = lib()
+    if not getattr(L, "_solver_bound", False):
+        for name, res, args in _SIGS:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        L._solver_bound = True
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(_D)
+
+
+class Solver:
+    """GPU solver configured by options-file text; fields are (Nz, Ny, Nx) arrays."""
+
+    def __init__(self, options_text, shape):
+        L = _bind()
+        h = _VP()
+        check(L.mc3d_solver_create(options_text.encode(), ctypes.byref(h)))
+        self._h = h
+        self._L = L
+        self.shape = tuple(shape)
+
+    def set_metrics(self, metrics):
+        ms = [np.ascontiguousarray(m, dtype=np.float64) for m in metrics]
+        arr = (_D * 10)(*[_p(m) for m in ms])
+        check(self._L.mc3d_solver_set_metrics(self._h, arr))
+
+    def set_initial(self, ic):
+        a = [np.ascontiguousarray(ic[k], dtype=np.float64) for k in ("rho", "U", "V", "W", "p")]
+        check(self._L.mc3d_solver_set_initial(self._h, *[_p(x) for x in a]))
+
+    def step(self, n=1):
+        check(self._L.mc3d_solver_step(self._h, n))
+
+    def run(self):
+        check(self._L.mc3d_solver_run(self._h))
+
+    def field(self, name):
+        out = np.empty(self.shape)
+        check(self._L.mc3d_solver_get_field(self._h, name.encode(), _p(out)))
+        return out
+
+    def device_ptr(self, name):
+        p = _VP()
+        check(self._L.mc3d_solver_field_device(self._h, name.encode(), ctypes.byref(p)))
+        return p.value
+
+    def scalars(self):
+        t, dt, ts = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        check(self._L.mc3d_solver_scalars(self._h, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(ts)))
+        return t.value, dt.value, ts.value
+
+    def stream(self):
+        p = _VP()
+        check(self._L.mc3d_solver_stream(self._h, ctypes.byref(p)))
+        return p.value
+
+    def sync(self):
+        check(self._L.mc3d_solver_sync(self._h))
+
+    def close(self):
+        if self._h:
+            self._L.mc3d_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -554,15 +554,23 @@ int orc_deriv_field(int scheme, int dir, int bc, int Nx, int Ny, int Nz, double 
     Geo g{Nx, Ny, Nz};
     const int n = g.n_line(dir);
     const long s = g.stride(dir);
-    std::vector<double> li(n), lo(n);
-    for (int q = 0; q < g.n_lines(dir); ++q) {
-        const long b = g.base(dir, q);
-        for (int i = 0; i < n; ++i) li[i] = in[b + i * s];
-        int rc = orc_deriv_line(scheme, bc, n, dd, li.data(), lo.data(), nullptr);
-        if (rc) return rc;
-        for (int i = 0; i < n; ++i) out[b + i * s] = lo[i];
+    int err = 0;
+#pragma omp parallel
+    {
+        std::vector<double> li(n), lo(n);
+#pragma omp for schedule(static)
+        for (int q = 0; q < g.n_lines(dir); ++q) {
+            const long b = g.base(dir, q);
+            for (int i = 0; i < n; ++i) li[i] = in[b + i * s];
+            int rc = orc_deriv_line(scheme, bc, n, dd, li.data(), lo.data(), nullptr);
+            if (rc) {
+#pragma omp atomic write
+                err = rc;
+            }
+            for (int i = 0; i < n; ++i) out[b + i * s] = lo[i];
+        }
     }
-    return 0;
+    return err;
 }
 
 int orc_deriv_field_tpb(int scheme, int dir, int Nx, int Ny, int Nz, double dd, const double *in,
@@ -592,28 +600,36 @@ int orc_filter_field(int kind, double alphaF, int dir, const int bc3[3], int Nx,
     Geo g{Nx, Ny, Nz};
     const int n = g.n_line(dir);
     const long s = g.stride(dir);
-    std::vector<double> li(n), lo(n);
-    for (int q = 0; q < g.n_lines(dir); ++q) {
-        const long b = g.base(dir, q);
-        // coordinates of the line in the two other directions
-        int ii = 0, jj = 0, kk = 0;
-        if (dir == 0) { jj = q % Ny; kk = q / Ny; }
-        if (dir == 1) { ii = q % Nx; kk = q / Nx; }
-        if (dir == 2) { ii = q % Nx; jj = q / Nx; }
-        bool pass = false;
-        if (dir != 0 && bc3[0] == ORC_DIRICHLET && (ii == 0 || ii == Nx - 1)) pass = true;
-        if (dir != 1 && bc3[1] == ORC_DIRICHLET && (jj == 0 || jj == Ny - 1)) pass = true;
-        if (dir != 2 && bc3[2] == ORC_DIRICHLET && (kk == 0 || kk == Nz - 1)) pass = true;
-        for (int i = 0; i < n; ++i) li[i] = in[b + i * s];
-        if (pass) {
-            lo = li;
-        } else {
-            int rc = orc_filter_line(kind, bc3[dir], n, alphaF, li.data(), lo.data());
-            if (rc) return rc;
+    int err = 0;
+#pragma omp parallel
+    {
+        std::vector<double> li(n), lo(n);
+#pragma omp for schedule(static)
+        for (int q = 0; q < g.n_lines(dir); ++q) {
+            const long b = g.base(dir, q);
+            // coordinates of the line in the two other directions
+            int ii = 0, jj = 0, kk = 0;
+            if (dir == 0) { jj = q % Ny; kk = q / Ny; }
+            if (dir == 1) { ii = q % Nx; kk = q / Nx; }
+            if (dir == 2) { ii = q % Nx; jj = q / Nx; }
+            bool pass = false;
+            if (dir != 0 && bc3[0] == ORC_DIRICHLET && (ii == 0 || ii == Nx - 1)) pass = true;
+            if (dir != 1 && bc3[1] == ORC_DIRICHLET && (jj == 0 || jj == Ny - 1)) pass = true;
+            if (dir != 2 && bc3[2] == ORC_DIRICHLET && (kk == 0 || kk == Nz - 1)) pass = true;
+            for (int i = 0; i < n; ++i) li[i] = in[b + i * s];
+            if (pass) {
+                lo = li;
+            } else {
+                int rc = orc_filter_line(kind, bc3[dir], n, alphaF, li.data(), lo.data());
+                if (rc) {
+#pragma omp atomic write
+                    err = rc;
+                }
+            }
+            for (int i = 0; i < n; ++i) out[b + i * s] = lo[i];
         }
-        for (int i = 0; i < n; ++i) out[b + i * s] = lo[i];
     }
-    return 0;
+    return err;
 }
 
 }  // extern "C"

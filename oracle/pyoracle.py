@@ -156,3 +156,112 @@ def invert4(m):
     out = np.empty(16)
     lib().orc_invert4(_p(m), _p(out))
     return out.reshape(4, 4)
+
+
+# ---------------------------------------------------------------------------
+# solver step oracle (orc_solver.h)
+# ---------------------------------------------------------------------------
+TVDRK3, RK4, KENRK4, LSLDDRK4 = 0, 1, 2, 3
+NOMODEL, VREMAN, DSM = 0, 1, 2
+AVG_NONE, AVG_XI1, AVG_XI2, AVG_XI3, AVG_LOCAL, AVG_GLOBAL = 0, 1, 2, 3, 4, 5
+CONST_DT, CONST_CFL = 0, 1
+
+FIELD_IDS = {name: i for i, name in enumerate(
+    ["rho", "rhoU", "rhoV", "rhoW", "rhoE", "U", "V", "W", "p", "T", "mu", "sos",
+     "mu_sgs", "taukk", "k_sgs", "C", "CI", "Prt",
+     "rhok", "rhoUk", "rhoVk", "rhoWk", "rhoEk", "rhok2", "rhoUk2", "rhoVk2", "rhoWk2", "rhoEk2"])}
+
+
+class OrcConfig(ctypes.Structure):
+    _fields_ = [("Nx", ctypes.c_int), ("Ny", ctypes.c_int), ("Nz", ctypes.c_int),
+                ("bc", ctypes.c_int * 3), ("fd", ctypes.c_int * 3), ("filter", ctypes.c_int),
+                ("alphaF", ctypes.c_double), ("rk", ctypes.c_int), ("les", ctypes.c_int),
+                ("les_avg", ctypes.c_int), ("dsm_test_filter", ctypes.c_int),
+                ("dsm_use_taukk", ctypes.c_int), ("dsm_coeff_range", ctypes.c_int),
+                ("mu_ref", ctypes.c_double), ("ts_type", ctypes.c_int), ("cfl", ctypes.c_double),
+                ("dt", ctypes.c_double), ("filter_step", ctypes.c_int),
+                ("substep_filtering", ctypes.c_int), ("dx", ctypes.c_double),
+                ("dy", ctypes.c_double), ("dz", ctypes.c_double), ("timestep0", ctypes.c_int),
+                ("time0", ctypes.c_double)]
+
+
+def _solver_lib():
+    L = lib()
+    if not hasattr(L, "_solver_bound"):
+        L.orc_solver_create.restype = ctypes.c_void_p
+        L.orc_solver_create.argtypes = [ctypes.POINTER(OrcConfig), ctypes.POINTER(_D), _D, _D, _D,
+                                        _D, _D]
+        L.orc_solver_steps.restype = ctypes.c_int
+        L.orc_solver_steps.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_solver_field.restype = _D
+        L.orc_solver_field.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        for nm in ("orc_solver_time", "orc_solver_dt"):
+            getattr(L, nm).restype = ctypes.c_double
+            getattr(L, nm).argtypes = [ctypes.c_void_p]
+        L.orc_solver_timestep.restype = ctypes.c_int
+        L.orc_solver_timestep.argtypes = [ctypes.c_void_p]
+        L.orc_solver_destroy.restype = None
+        L.orc_solver_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_solver_rhs.restype = ctypes.c_int
+        L.orc_solver_rhs.argtypes = [ctypes.c_void_p, ctypes.POINTER(_D)]
+        L.orc_rk_amplification.restype = ctypes.c_double
+        L.orc_rk_amplification.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        L._solver_bound = True
+    return L
+
+
+class Solver:
+    """CPU oracle of the solver.  cfg: dict of OrcConfig fields; metrics: list of
+    10 arrays (J, J11..J33); ic: dict rho, U, V, W, p.  Arrays shaped (Nz,Ny,Nx)."""
+
+    def __init__(self, cfg, metrics, ic):
+        L = _solver_lib()
+        c = OrcConfig()
+        for k, v in cfg.items():
+            if k in ("bc", "fd"):
+                getattr(c, k)[:] = list(v)
+            else:
+                setattr(c, k, v)
+        self.shape = (cfg["Nz"], cfg["Ny"], cfg["Nx"])
+        self._keep = [_c(m) for m in metrics] + [_c(ic[k]) for k in ("rho", "U", "V", "W", "p")]
+        mp = (_D * 10)(*[_p(m) for m in self._keep[:10]])
+        r, u, v, w, p = self._keep[10:]
+        self._h = L.orc_solver_create(ctypes.byref(c), mp, _p(r), _p(u), _p(v), _p(w), _p(p))
+        self._L = L
+
+    def steps(self, n):
+        self._L.orc_solver_steps(self._h, n)
+
+    def field(self, name):
+        ptr = self._L.orc_solver_field(self._h, FIELD_IDS[name])
+        n = self.shape[0] * self.shape[1] * self.shape[2]
+        return np.ctypeslib.as_array(ptr, shape=(n,)).reshape(self.shape).copy()
+
+    def rhs(self):
+        n = self.shape[0] * self.shape[1] * self.shape[2]
+        outs = [np.empty(n) for _ in range(5)]
+        arr = (_D * 5)(*[_p(o) for o in outs])
+        self._L.orc_solver_rhs(self._h, arr)
+        return [o.reshape(self.shape) for o in outs]
+
+    @property
+    def time(self):
+        return self._L.orc_solver_time(self._h)
+
+    @property
+    def dt(self):
+        return self._L.orc_solver_dt(self._h)
+
+    @property
+    def timestep(self):
+        return self._L.orc_solver_timestep(self._h)
+
+    def __del__(self):
+        try:
+            self._L.orc_solver_destroy(self._h)
+        except Exception:
+            pass
+
+
+def rk_amplification(rk, z, dual_step=1):
+    return _solver_lib().orc_rk_amplification(rk, dual_step, z)
