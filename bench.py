@@ -1,0 +1,143 @@
+#!/usr/bin/env python3
+"""Throughput of the compact-difference LES time step on one MI355X.
+
+Workload: decaying compressible HIT on a 256^3 periodic grid (identity
+curvilinear map), FP64, PENTA10 derivatives, COMPACT8 filter (alpha_f 0.49),
+KENRK4 time stepping at CFL 0.5, VREMAN SGS with LOCAL averaging,
+mu_ref = 1/5000; seeded synthetic initial field (Rogallo spectrum,
+M_t = 0.3).  A "step" is one full time step (5 RK stages + filter + CFL).
+
+value = grid-point-steps per second (Mpt-steps/s), inputs resident in HBM.
+roofline = the kernel with the largest share of step time, its algorithmic
+bytes per launch over its mean launch duration (HIP events on the solver
+stream during the timed steps) against 8 TB/s.
+cpu_baseline = the CPU oracle (a plain C++ restatement of the reference
+algorithm, OpenMP over lines) timed on this host for one step of the same
+configuration at 128^3 (per-point cost does not depend on the grid size).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("Mpt-steps/s at 256³ FP64 vs CPU; learned vs DSM LES speedup at "
+          "≤2% KE error")
+HBM_PEAK_GBS = 8000.0
+
+CASE = dict(fd="PENTA10", filt="COMPACT8", alpha_f=0.49, rk="KENRK4", cfl=0.5,
+            les="VREMAN", les_avg="LOCAL", mu_ref=1.0 / 5000.0)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n_cpu):
+    """Oracle step rate on the host (test infrastructure, used only as the checker/baseline)."""
+    from mpi_3dcompact_amd import hit
+    from oracle import pyoracle as P
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    ic = hit.hit_initial_condition(n_cpu, k0=4.0, Mt=0.3, seed=0)
+    s = P.Solver(hit.oracle_config(n_cpu, **CASE), hit.identity_metrics(n_cpu), ic)
+    t0 = time.perf_counter()
+    s.steps(1)
+    dt = time.perf_counter() - t0
+    return {"value": n_cpu ** 3 / dt / 1e6, "unit": "Mpt-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"1 KENRK4 step of the same PENTA10/COMPACT8/VREMAN case at {n_cpu}^3 "
+                      f"({dt:.1f} s), OpenMP over independent lines"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256, help="grid points per direction")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-n", type=int, default=128)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU slab decomposition is a design, not a build)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from mpi_3dcompact_amd import hit
+    from mpi_3dcompact_amd.solver import Solver
+
+    N = args.n
+    log(f"[bench] generating {N}^3 HIT initial field")
+    ic = hit.hit_initial_condition(N, k0=4.0, Mt=0.3, seed=0)
+    met = hit.identity_metrics(N)
+    g = Solver(hit.options_text(N, **CASE), (N, N, N))
+    g.set_metrics(met)
+    g.set_initial(ic)
+    del met
+    log(f"[bench] warmup {args.warmup} steps")
+    g.step(args.warmup)
+    g.sync()
+    g.timing(True)
+    log(f"[bench] timing {args.steps} steps")
+    t0 = time.perf_counter()
+    g.step(args.steps)
+    g.sync()
+    wall = time.perf_counter() - t0
+    rep = g.timing_report()
+    g.timing(False)
+    ms_per_step = wall * 1e3 / args.steps
+    value = N ** 3 * args.steps / wall / 1e6
+    kern_ms = sum(r["ms"] for r in rep)
+    dom = max(rep, key=lambda r: r["ms"])
+    avg_ms = dom["ms"] / dom["count"]
+    bytes_per_launch = dom["bytes"] / dom["count"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    for r in sorted(rep, key=lambda r: -r["ms"]):
+        gbs = r["bytes"] / (r["ms"] * 1e-3) / 1e9 if r["ms"] > 0 else 0.0
+        log(f"[bench] {r['kernel']:28s} launches {r['count']:5d}  {r['ms'] / args.steps:8.2f} ms/step "
+            f"{100 * r['ms'] / kern_ms:5.1f}%  {gbs:7.0f} GB/s")
+    t_final, dt_final, ts = g.scalars()
+    rho = g.field("rho")
+    if not np.isfinite(rho).all():
+        raise SystemExit("non-finite density after the timed steps")
+    g.close()
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] CPU oracle baseline at {args.cpu_n}^3")
+        cpu = cpu_baseline(args.cpu_n)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mpt-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"full plain LES, decaying HIT {N}^3 periodic, PENTA10 + COMPACT8 "
+                               f"(alpha_f 0.49) + KENRK4 (CFL 0.5) + VREMAN LOCAL, mu 1/5000",
+                   "grid": [N, N, N], "points": N ** 3, "rk_stages_per_step": 5},
+        "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "bytes_per_launch": bytes_per_launch,
+                     "avg_launch_ms": round(avg_ms, 4),
+                     "share_of_step": round(dom["ms"] / kern_ms, 3)},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

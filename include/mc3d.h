@@ -128,6 +128,15 @@ int mc3d_solver_stream(mc3d_solver *s, void **stream);
 int mc3d_solver_sync(mc3d_solver *s);
 void mc3d_solver_destroy(mc3d_solver *s);
 
+/* Per-kernel timing of the step with HIP events on the solver stream (the
+ * counterpart of SOLVER.USETIMING, src/CurvilinearCSolver_Core.cpp:6,182-185).
+ * enable != 0 starts recording (and clears previous totals); the report is a
+ * JSON array of {"kernel", "count", "ms", "bytes"} objects, bytes being the
+ * algorithmic bytes (distinct full fields read + written) of those launches.
+ * Returns MC3D_EINVAL if buf is too small (the required size goes to *needed). */
+int mc3d_solver_timing(mc3d_solver *s, int enable);
+int mc3d_solver_timing_report(mc3d_solver *s, char *buf, int buflen, int *needed);
+
 /* Diagnostic (no counterpart in the reference): builds, on the host only,
  * the partition tables an operator would upload, for verification of the
  * factorisation without a GPU.  cls 0 = derivative (kind = scheme), 1 = filter

@@ -16,6 +16,7 @@ struct mc3d_lineop {
     int dims[3] = {0, 0, 0};
     void *dmem = nullptr;
     int device = 0;
+    std::string kname;
 };
 
 namespace {
@@ -87,6 +88,8 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
         delete op;
         return mc3d::fail(MC3D_EINVAL, "line too long for the LDS tile");
     }
+    op->kname = "lineop_kernel<" + std::to_string(p.W) + ", " + std::to_string(dir) + ", " +
+                std::to_string(bc == MC3D_PERIODIC ? 0 : 1) + ">";
     rc = upload(op);
     if (rc) {
         if (op->dmem) (void)hipFree(op->dmem);
@@ -209,8 +212,10 @@ int mc3d_plan_tables(int cls, int kind, int bc, int n, double dd, double alphaF,
 
 }  // extern "C"
 
-// internal accessor used by the solver
+// internal accessors used by the solver
 namespace mc3d {
+const char *lineop_kernel_name(const mc3d_lineop *op) { return op->kname.c_str(); }
+
 int lineop_launch_accumulate(const mc3d_lineop *op, const double *const *in, double *const *out,
                              int nfields, int accumulate, void *stream) {
     const int mode = op->bc == MC3D_PERIODIC ? kModePlainPeriodic : kModePlainDirichlet;

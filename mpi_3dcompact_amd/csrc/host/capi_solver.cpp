@@ -1,6 +1,7 @@
 // C ABI of the whole solver (include/mc3d.h).  Exceptions never cross it.
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
@@ -128,6 +129,38 @@ int mc3d_solver_stream(mc3d_solver *s, void **stream) {
 int mc3d_solver_sync(mc3d_solver *s) {
     if (!s) return mc3d::fail(MC3D_EINVAL, "null solver");
     return guarded([&] { mc3d::hip_check(hipStreamSynchronize(s->stream), "synchronize"); });
+}
+
+int mc3d_solver_timing(mc3d_solver *s, int enable) {
+    if (!s) return mc3d::fail(MC3D_EINVAL, "null solver");
+    return guarded([&] {
+        s->cs->timer.reset();
+        s->cs->timer.enable(enable != 0);
+    });
+}
+
+int mc3d_solver_timing_report(mc3d_solver *s, char *buf, int buflen, int *needed) {
+    if (!s) return mc3d::fail(MC3D_EINVAL, "null solver");
+    std::string j;
+    int rc = guarded([&] {
+        s->cs->timer.collect();
+        j = "[";
+        bool first = true;
+        char tmp[512];
+        for (const auto &kv : s->cs->timer.stats()) {
+            std::snprintf(tmp, sizeof(tmp), "%s{\"kernel\": \"%s\", \"count\": %ld, \"ms\": %.6f, \"bytes\": %.0f}",
+                          first ? "" : ", ", kv.first.c_str(), kv.second.count, kv.second.ms,
+                          kv.second.bytes);
+            j += tmp;
+            first = false;
+        }
+        j += "]";
+    });
+    if (rc) return rc;
+    if (needed) *needed = (int)j.size() + 1;
+    if (!buf || buflen < (int)j.size() + 1) return mc3d::fail(MC3D_EINVAL, "report buffer too small");
+    std::memcpy(buf, j.c_str(), j.size() + 1);
+    return MC3D_OK;
 }
 
 void mc3d_solver_destroy(mc3d_solver *s) {

@@ -74,7 +74,11 @@ double LineDerivative::calcNeumann(const double *f) {
 
 void LineDerivative::calc1stDerivFields(int n, const double *const *in, double *const *out,
                                         bool accumulate) {
+    int t = -1;
+    if (timer && timer->enabled())
+        t = timer->begin(lineop_kernel_name(op_), (double)n * (accumulate ? 24.0 : 16.0) * Nx * Ny * Nz);
     mc3d_check(lineop_launch_accumulate(op_, in, out, n, accumulate ? 1 : 0, st_), "derivative");
+    if (timer) timer->end(t);
 }
 
 LineFilter::LineFilter(int kind, double alphaF, const Domain &d, const int bc3[3],
@@ -93,7 +97,10 @@ LineFilter::~LineFilter() { mc3d_lineop_destroy(op_); }
 void LineFilter::filterField(const double *in, double *out) { filterFields(1, &in, &out); }
 
 void LineFilter::filterFields(int n, const double *const *in, double *const *out) {
+    int t = -1;
+    if (timer && timer->enabled()) t = timer->begin(lineop_kernel_name(op_), (double)n * 16.0 * Nx * Ny * Nz);
     mc3d_check(lineop_launch_accumulate(op_, in, out, n, 0, st_), "filter");
+    if (timer) timer->end(t);
 }
 
 // ---------------------------------------------------------------------------
@@ -136,14 +143,17 @@ VremanSGS::VremanSGS(CurvilinearCSolver *c) {
     filtX.reset(new Compact10Filter(-0.49, c->dom, bc3, AbstractDerivatives::DIRX, c->st));
     filtY.reset(new Compact10Filter(-0.49, c->dom, bc3, AbstractDerivatives::DIRY, c->st));
     filtZ.reset(new Compact10Filter(-0.49, c->dom, bc3, AbstractDerivatives::DIRZ, c->st));
+    for (auto *f : {filtX.get(), filtY.get(), filtZ.get()}) static_cast<LineFilter *>(f)->timer = &c->timer;
     mu_sgs = c->alloc_field();
 }
 
 void VremanSGS::getSGSViscosity(const double *const dU[3], const double *const dV[3],
                                 const double *const dW[3], const double *rho) {
-    hip_check(launch_vreman(dU, dV, dW, cs->msh, rho, mu_sgs, cs->dom.dx, cs->dom.dy, cs->dom.dz,
-                            cs->N, cs->st),
-              "vreman");
+    cs->timed("k_vreman", 21.0, [&] {
+        hip_check(launch_vreman(dU, dV, dW, cs->msh, rho, mu_sgs, cs->dom.dx, cs->dom.dy, cs->dom.dz,
+                                cs->N, cs->st),
+                  "vreman");
+    });
     doAveraging(mu_sgs, mu_sgs);
 }
 
@@ -243,7 +253,7 @@ void LSLDDRK4::updateConservedData() {
 // ---------------------------------------------------------------------------
 // src/CurvilinearCSolver.hpp:120-245
 CurvilinearCSolver::CurvilinearCSolver(const Options &o, hipStream_t s)
-    : opt(o), dom(o), st(s) {
+    : opt(o), dom(o), st(s), timer(s) {
     N = dom.gN;
     timeStep = opt.timeStep;
     // src/IdealGas.hpp:35-48
@@ -302,6 +312,8 @@ CurvilinearCSolver::CurvilinearCSolver(const Options &o, hipStream_t s)
         if (opt.filterType == Options::COMPACT8) fv[d]->reset(new Compact8Filter(opt.alphaF, dom, bc3, dir, st));
         else fv[d]->reset(new Compact10Filter(opt.alphaF, dom, bc3, dir, st));
     }
+    for (auto *d : {derivX.get(), derivY.get(), derivZ.get()}) static_cast<LineDerivative *>(d)->timer = &timer;
+    for (auto *f : {filtX.get(), filtY.get(), filtZ.get()}) static_cast<LineFilter *>(f)->timer = &timer;
     // src/CurvilinearCSolver.hpp:232-242
     if (opt.lesModel == Options::VREMAN) les.reset(new VremanSGS(this));
     else if (opt.lesModel == Options::DSM)
@@ -356,7 +368,7 @@ void CurvilinearCSolver::setInitialConditions_host(const double *rho0, const dou
 
 // calcDtFromCFL, src/CurvilinearCSolver_Core.cpp:189-237
 void CurvilinearCSolver::preStep() {
-    hip_check(launch_cfl(pr, msh, dom.dx, dom.dy, dom.dz, cfl_partial, N, st), "cfl");
+    timed("k_cfl", 7.0, [&] { hip_check(launch_cfl(pr, msh, dom.dx, dom.dy, dom.dz, cfl_partial, N, st), "cfl"); });
     const bool first = (timeStep == opt.timeStep);
     hip_check(launch_cfl_finish(cfl_partial, cfl_blocks(N), sc, opt.TSType == Options::CONST_CFL,
                                 opt.CFL, first ? 1 : 0, opt.time, st),
@@ -387,7 +399,8 @@ void CurvilinearCSolver::preSubStep() {
     for (int e = 0; e < 15; ++e) fa.pre[e] = pre[e];
     fa.les = (opt.lesModel == Options::NOMODEL) ? 0 : (opt.lesModel == Options::DSM ? 2 : 1);
     fa.Pr_t = Pr_t;
-    hip_check(launch_flux(fa, ig, N, st), "flux");
+    const double flux_fields = 35.0 + 15.0 + (fa.les ? 1.0 : 0.0) + (fa.les == 2 ? 1.0 : 0.0);
+    timed("k_flux", flux_fields, [&] { hip_check(launch_flux(fa, ig, N, st), "flux"); });
     // divergence: xi of the _1 fluxes, then eta of _2 and zeta of _3 added in
     // (src/CurvilinearCSolver_Core.cpp:763-773 with the sums of :810,839,...)
     const double *p1[5] = {pre[0], pre[1], pre[2], pre[3], pre[4]};
@@ -406,7 +419,9 @@ void CurvilinearCSolver::stage_update(int op, double a, double b, double c) {
     ra.sc = sc;
     ra.op = op;
     ra.a = a; ra.b = b; ra.c = c;
-    hip_check(launch_rhs_rk(ra, N, st), "stage update");
+    // fields: div (5) + J + previous momentum k2 (3) read, k2 (5) written, plus the registers
+    static const double reg[9] = {10, 15, 15, 15, 20, 10, 15, 20, 20};
+    timed("k_rhs_rk", 14.0 + reg[op], [&] { hip_check(launch_rhs_rk(ra, N, st), "stage update"); });
 }
 
 // src/CurvilinearCSolver.hpp:346-353
@@ -440,7 +455,7 @@ void CurvilinearCSolver::filterConservedData() {
 }
 
 void CurvilinearCSolver::updateNonConservedData() {
-    hip_check(launch_eos(rkLast ? r1 : rk_, pr, msh, ig, N, st), "eos");
+    timed("k_eos", 24.0, [&] { hip_check(launch_eos(rkLast ? r1 : rk_, pr, msh, ig, N, st), "eos"); });
 }
 
 // checkEnd without the kill file: src/CurvilinearCSolver_Core.cpp:1798-1819

@@ -16,6 +16,7 @@
 #include "../pointwise.h"
 #include "mc3d.h"
 #include "options.hpp"
+#include "timer.hpp"
 
 namespace mc3d {
 
@@ -76,6 +77,7 @@ class LineDerivative : public AbstractDerivatives {
                                const double *Np3) override;
     double calcNeumann(const double *f) override;
     void calc1stDerivFields(int n, const double *const *in, double *const *out, bool accumulate) override;
+    KernelTimer *timer = nullptr;
 
   protected:
     int scheme_;
@@ -118,6 +120,7 @@ class LineFilter : public AbstractFilter {
     ~LineFilter() override;
     void filterField(const double *in, double *out) override;
     void filterFields(int n, const double *const *in, double *const *out) override;
+    KernelTimer *timer = nullptr;
 
   protected:
     mc3d_lineop *op_ = nullptr;
@@ -259,6 +262,14 @@ class CurvilinearCSolver {
     double *tmp[10] = {};
     double *cfl_partial = nullptr;
     StepScalars *sc = nullptr;
+    KernelTimer timer;
+    // times one pointwise launch under its kernel name with the given
+    // number of full fields read + written
+    template <class F> void timed(const char *key, double fields, F &&launch) {
+        int t = timer.begin(key, fields * 8.0 * (double)N);
+        launch();
+        timer.end(t);
+    }
 
     double *alloc_field();
     void set_metrics_host(const double *const m[10]);

@@ -10,4 +10,8 @@ namespace mc3d {
 int lineop_launch_accumulate(const mc3d_lineop *op, const double *const *in, double *const *out,
                              int nfields, int accumulate, void *stream);
 
+// Symbol-style name of the kernel instantiation an operator launches,
+// e.g. "lineop_kernel<2, 1, 0>" (half bandwidth, direction, boundary mode).
+const char *lineop_kernel_name(const mc3d_lineop *op);
+
 }  // namespace mc3d

@@ -20,6 +20,9 @@ _SIGS = [
     ("mc3d_solver_stream", ctypes.c_int, [_VP, ctypes.POINTER(_VP)]),
     ("mc3d_solver_sync", ctypes.c_int, [_VP]),
     ("mc3d_solver_destroy", None, [_VP]),
+    ("mc3d_solver_timing", ctypes.c_int, [_VP, ctypes.c_int]),
+    ("mc3d_solver_timing_report", ctypes.c_int, [_VP, ctypes.c_char_p, ctypes.c_int,
+                                                 ctypes.POINTER(ctypes.c_int)]),
 ]
 
 
@@ -86,6 +89,17 @@ class Solver:
 
     def sync(self):
         check(self._L.mc3d_solver_sync(self._h))
+
+    def timing(self, enable=True):
+        check(self._L.mc3d_solver_timing(self._h, 1 if enable else 0))
+
+    def timing_report(self):
+        import json
+        need = ctypes.c_int(0)
+        self._L.mc3d_solver_timing_report(self._h, None, 0, ctypes.byref(need))
+        buf = ctypes.create_string_buffer(max(need.value, 2))
+        check(self._L.mc3d_solver_timing_report(self._h, buf, len(buf), ctypes.byref(need)))
+        return json.loads(buf.value.decode())
 
     def close(self):
         if self._h:
