@@ -93,6 +93,12 @@ void mc3d_lineop_destroy(mc3d_lineop *op);
  * chunk, lines per workgroup. */
 int mc3d_lineop_info(const mc3d_lineop *op, int *chunks, int *chunk_rows, int *lines_per_block);
 
+/* Diagnostic: name of the kernel instantiation a plain (non-TPB) application
+ * of the operator launches, as profilers show it, e.g.
+ * "lineop_strided<2, 3, 32, true>" (half bandwidth, stencil half width, rows
+ * per chunk, periodic).  NULL for a NULL operator. */
+const char *mc3d_lineop_kernel(const mc3d_lineop *op);
+
 /* ---------------------------------------------------------------------------
  * Whole solver step (CurvilinearCSolver + RK + SGS), configured by the
  * reference's options file text (MPI_3DCompact.in: sections [TIMESTEPPING],
@@ -140,13 +146,15 @@ int mc3d_solver_timing_report(mc3d_solver *s, char *buf, int buflen, int *needed
 /* Diagnostic (no counterpart in the reference): builds, on the host only,
  * the partition tables an operator would upload, for verification of the
  * factorisation without a GPU.  cls 0 = derivative (kind = scheme), 1 = filter
- * (kind = filter kind).  sizes[8] receives {N, W, H, P, NI, nbl, nbr, int_off};
- * each non-NULL table pointer must have room for the sizes documented in
- * mpi_3dcompact_amd/csrc/lineop.h (lu: P*16*5, sp: P*16*2W, G: P*2W*NI,
- * rows: (nbl+nbr)*19, interior: 11, cstart/clen: P). */
-int mc3d_plan_tables(int cls, int kind, int bc, int n, double dd, double alphaF, int sizes[8],
-                     double *lu, double *sp, double *G, double *rows, double *interior,
-                     int *cstart, int *clen);
+ * (kind = filter kind); mm = rows per chunk (16 or 32, 0 = the choice the
+ * operators make).  sizes[10] receives {N, W, H, P, NI, nbl, nbr, int_off, MM,
+ * dense_ok}; each non-NULL table pointer must have room for the sizes
+ * documented in mpi_3dcompact_amd/csrc/lineop.h (lu: P*MM*5, sp: P*MM*2W,
+ * G: P*2W*NI, rows: (nbl+nbr)*19, drows: (nbl+nbr)*17, interior: 11,
+ * cstart/clen: P). */
+int mc3d_plan_tables(int cls, int kind, int bc, int n, double dd, double alphaF, int mm,
+                     int sizes[10], double *lu, double *sp, double *G, double *rows,
+                     double *drows, double *interior, int *cstart, int *clen);
 
 #ifdef __cplusplus
 }

@@ -33,9 +33,10 @@ SIGNATURES = [
                                          ctypes.c_int, _VP]),
     ("mc3d_lineop_destroy", None, [_VP]),
     ("mc3d_lineop_info", ctypes.c_int, [_VP, _I, _I, _I]),
+    ("mc3d_lineop_kernel", ctypes.c_char_p, [_VP]),
     ("mc3d_plan_tables", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                        ctypes.c_double, ctypes.c_double, _I, _D, _D, _D, _D, _D,
-                                        _I, _I]),
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_int, _I, _D, _D,
+                                        _D, _D, _D, _D, _I, _I]),
 ]
 
 _LIB = None

@@ -1,6 +1,7 @@
 // C ABI of the derivative and filter operators (include/mc3d.h).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstring>
 #include <string>
 
@@ -16,15 +17,17 @@ struct mc3d_lineop {
     int dims[3] = {0, 0, 0};
     void *dmem = nullptr;
     int device = 0;
-    std::string kname;
+    bool fast = false;      // plain applications use the fast kernels
+    std::string kname;      // kernel of plain applications
 };
 
 namespace {
 
 int upload(mc3d_lineop *op) {
     const mc3d::Plan &p = op->plan;
-    const size_t nr = p.rows.size(), nl = p.lu.size(), ns = p.sp.size(), ng = p.G.size();
-    const size_t bytes = sizeof(double) * (nr + nl + ns + ng) + sizeof(int) * 2 * p.P + 256;
+    const size_t nr = p.rows.size(), nd = p.drows.size(), nl = p.lu.size(), ns = p.sp.size(),
+                 ng = p.G.size();
+    const size_t bytes = sizeof(double) * (nr + nd + nl + ns + ng + 2) + sizeof(int) * 2 * p.P + 256;
     if (hipGetDevice(&op->device) != hipSuccess) return mc3d::fail(MC3D_EHIP, "hipGetDevice failed");
     if (hipMalloc(&op->dmem, bytes) != hipSuccess) return mc3d::fail(MC3D_ENOMEM, "hipMalloc of operator tables failed");
     char *base = static_cast<char *>(op->dmem);
@@ -38,12 +41,13 @@ int upload(mc3d_lineop *op) {
     double dummy = 0.0;
     mc3d::OpDev &d = op->dev;
     d.rows = static_cast<const double *>(put(nr ? p.rows.data() : &dummy, sizeof(double) * (nr ? nr : 1)));
+    d.drows = static_cast<const double *>(put(nd ? p.drows.data() : &dummy, sizeof(double) * (nd ? nd : 1)));
     d.lu = static_cast<const double *>(put(p.lu.data(), sizeof(double) * nl));
     d.sp = static_cast<const double *>(put(p.sp.data(), sizeof(double) * ns));
     d.G = static_cast<const double *>(put(p.G.data(), sizeof(double) * ng));
     d.cstart = static_cast<const int *>(put(p.cstart.data(), sizeof(int) * p.P));
     d.clen = static_cast<const int *>(put(p.clen.data(), sizeof(int) * p.P));
-    if (!d.rows || !d.lu || !d.sp || !d.G || !d.cstart || !d.clen)
+    if (!d.rows || !d.drows || !d.lu || !d.sp || !d.G || !d.cstart || !d.clen)
         return mc3d::fail(MC3D_EHIP, "upload of operator tables failed");
     return MC3D_OK;
 }
@@ -59,37 +63,59 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
     if (cls == mc3d::kDeriv && !(dd > 0.0)) return mc3d::fail(MC3D_EINVAL, "dd must be positive");
     const long npts = (long)dims[0] * dims[1] * dims[2];
     if (npts > (long)2147483647) return mc3d::fail(MC3D_EINVAL, "field larger than 2^31 points");
-    mc3d::PlanRequest rq{cls, kind, bc, dims[dir], dd, alphaF};
+    int dev = 0, lds_limit = 65536;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+        return mc3d::fail(MC3D_EHIP, "cannot query the device");
     auto *op = new mc3d_lineop();
     std::string err;
-    int rc = mc3d::build_plan(rq, op->plan, err);
-    if (rc) {
-        delete op;
-        return mc3d::fail(rc, err);
+    int rc = 0;
+    // 32-row chunks only pay off in the fast kernels; otherwise plan 16-row chunks
+    for (int mm : {0, 16}) {
+        mc3d::PlanRequest rq{cls, kind, bc, dims[dir], dd, alphaF, mm};
+        op->plan = mc3d::Plan();
+        rc = mc3d::build_plan(rq, op->plan, err);
+        if (rc) {
+            delete op;
+            return mc3d::fail(rc, err);
+        }
+        const mc3d::Plan &p = op->plan;
+        mc3d::OpDev &d = op->dev;
+        d = mc3d::OpDev{};
+        d.N = p.N; d.dir = dir; d.W = p.W; d.H = p.H; d.P = p.P; d.L = p.L; d.NI = p.NI;
+        d.ext = p.ext; d.MM = p.MM; d.periodic = p.periodic; d.nbl = p.nbl; d.nbr = p.nbr;
+        d.int_off = p.int_off; d.int_w = (int)p.int_c.size();
+        for (int k = 0; k < d.int_w; ++k) d.int_c[k] = p.int_c[k];
+        d.dd = p.dd; d.divide = p.divide ? 1 : 0;
+        // 1/dd is exact when dd is a power of two; then r * (1/dd) == r / dd bitwise
+        int ex = 0;
+        d.inv_dd = (p.divide && std::frexp(p.dd, &ex) == 0.5) ? 1.0 / p.dd : 0.0;
+        d.passA = d.passB = 0;
+        if (cls == mc3d::kFilter && bc3) {
+            const int da = dir == 0 ? 1 : 0, db = dir == 2 ? 1 : 2;
+            d.passA = bc3[da] == MC3D_DIRICHLET;
+            d.passB = bc3[db] == MC3D_DIRICHLET;
+            d.NA = dims[da];
+            d.NB = dims[db];
+        }
+        const bool rows_ok = p.periodic ? (p.nbl + p.nbr == 0) : p.dense_ok;
+        op->fast = p.uniform && rows_ok && mc3d::lineop_fast_supported(d, (size_t)lds_limit);
+        if (op->fast || p.MM == 16) break;
     }
-    op->cls = cls; op->kind = kind; op->dir = dir; op->bc = bc;
-    std::memcpy(op->dims, dims, sizeof(op->dims));
     const mc3d::Plan &p = op->plan;
-    mc3d::OpDev &d = op->dev;
-    d.N = p.N; d.dir = dir; d.W = p.W; d.H = p.H; d.P = p.P; d.L = p.L; d.NI = p.NI;
-    d.ext = p.ext; d.periodic = p.periodic; d.nbl = p.nbl; d.nbr = p.nbr;
-    d.int_off = p.int_off; d.int_w = (int)p.int_c.size();
-    for (int k = 0; k < d.int_w; ++k) d.int_c[k] = p.int_c[k];
-    d.dd = p.dd; d.divide = p.divide ? 1 : 0;
-    d.passA = d.passB = 0;
-    if (cls == mc3d::kFilter && bc3) {
-        const int da = dir == 0 ? 1 : 0, db = dir == 2 ? 1 : 2;
-        d.passA = bc3[da] == MC3D_DIRICHLET;
-        d.passB = bc3[db] == MC3D_DIRICHLET;
-        d.NA = dims[da];
-        d.NB = dims[db];
-    }
-    if (mc3d::lineop_lds_bytes(d) > 65536) {
+    if (mc3d::lineop_lds_bytes(op->dev) > 65536) {
         delete op;
         return mc3d::fail(MC3D_EINVAL, "line too long for the LDS tile");
     }
-    op->kname = "lineop_kernel<" + std::to_string(p.W) + ", " + std::to_string(dir) + ", " +
-                std::to_string(bc == MC3D_PERIODIC ? 0 : 1) + ">";
+    op->cls = cls; op->kind = kind; op->dir = dir; op->bc = bc;
+    std::memcpy(op->dims, dims, sizeof(op->dims));
+    const std::string per = p.periodic ? "true" : "false";
+    if (op->fast)
+        op->kname = std::string(dir == 0 ? "lineop_contig<" : "lineop_strided<") + std::to_string(p.W) +
+                    ", " + std::to_string(p.H) + ", " + std::to_string(p.MM) + ", " + per + ">";
+    else
+        op->kname = "lineop_kernel<" + std::to_string(p.W) + ", " + std::to_string(dir) + ", " +
+                    std::to_string(bc == MC3D_PERIODIC ? 0 : 1) + ", " + std::to_string(p.MM) + ">";
     rc = upload(op);
     if (rc) {
         if (op->dmem) (void)hipFree(op->dmem);
@@ -115,7 +141,9 @@ int apply(const mc3d_lineop *op, const double *const *in, double *const *out, in
     a.Nx = op->dims[0]; a.Ny = op->dims[1]; a.Nz = op->dims[2];
     a.nlines = (int)((long)a.Nx * a.Ny * a.Nz / op->dims[op->dir]);
     a.accumulate = accumulate;
-    hipError_t e = mc3d::lineop_launch(op->dev, a, nfields, mode, static_cast<hipStream_t>(stream));
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    hipError_t e = (op->fast && mode != mc3d::kModeTPB) ? mc3d::lineop_launch_fast(op->dev, a, nfields, st)
+                                                         : mc3d::lineop_launch(op->dev, a, nfields, mode, st);
     if (e != hipSuccess) return mc3d::fail(MC3D_EHIP, std::string("line operator launch: ") + hipGetErrorString(e));
     return MC3D_OK;
 }
@@ -182,25 +210,28 @@ int mc3d_lineop_info(const mc3d_lineop *op, int *chunks, int *chunk_rows, int *l
     if (!op) return mc3d::fail(MC3D_EINVAL, "null operator");
     if (chunks) *chunks = op->plan.P;
     if (chunk_rows) *chunk_rows = op->plan.clen.empty() ? 0 : op->plan.clen[0];
-    if (lines_per_block) *lines_per_block = op->plan.L;
+    if (lines_per_block) *lines_per_block = op->fast ? 64 : op->plan.L;
     return MC3D_OK;
 }
 
-int mc3d_plan_tables(int cls, int kind, int bc, int n, double dd, double alphaF, int sizes[8],
-                     double *lu, double *sp, double *G, double *rows, double *interior,
-                     int *cstart, int *clen) {
+const char *mc3d_lineop_kernel(const mc3d_lineop *op) { return op ? op->kname.c_str() : nullptr; }
+
+int mc3d_plan_tables(int cls, int kind, int bc, int n, double dd, double alphaF, int mm,
+                     int sizes[10], double *lu, double *sp, double *G, double *rows,
+                     double *drows, double *interior, int *cstart, int *clen) {
     if (!sizes) return mc3d::fail(MC3D_EINVAL, "null sizes");
-    mc3d::PlanRequest rq{cls, kind, bc, n, dd, alphaF};
+    mc3d::PlanRequest rq{cls, kind, bc, n, dd, alphaF, mm};
     mc3d::Plan p;
     std::string err;
     int rc = mc3d::build_plan(rq, p, err);
     if (rc) return mc3d::fail(rc, err);
-    const int v[8] = {p.N, p.W, p.H, p.P, p.NI, p.nbl, p.nbr, p.int_off};
+    const int v[10] = {p.N, p.W, p.H, p.P, p.NI, p.nbl, p.nbr, p.int_off, p.MM, p.dense_ok ? 1 : 0};
     std::memcpy(sizes, v, sizeof(v));
     if (lu) std::memcpy(lu, p.lu.data(), sizeof(double) * p.lu.size());
     if (sp) std::memcpy(sp, p.sp.data(), sizeof(double) * p.sp.size());
     if (G) std::memcpy(G, p.G.data(), sizeof(double) * p.G.size());
     if (rows && !p.rows.empty()) std::memcpy(rows, p.rows.data(), sizeof(double) * p.rows.size());
+    if (drows && !p.drows.empty()) std::memcpy(drows, p.drows.data(), sizeof(double) * p.drows.size());
     if (interior) {
         for (int k = 0; k < 2 * mc3d::kHMax + 1; ++k)
             interior[k] = k < (int)p.int_c.size() ? p.int_c[k] : 0.0;

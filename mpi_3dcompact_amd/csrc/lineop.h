@@ -21,12 +21,13 @@
 namespace mc3d {
 
 constexpr int kMaxFields = 16;
-constexpr int kMMax = 16;      // rows per chunk held in registers
+constexpr int kMMax = 32;      // largest chunk (rows held in registers by one lane)
 constexpr int kHMax = 5;       // max interior stencil half width
 constexpr int kSpecRows = 5;   // max special rows at each end
 constexpr int kSpecW = 16;     // max taps of a special row
 constexpr int kRowRec = 3 + kSpecW;  // {offset, step, width, w[16]}
 constexpr int kLuRec = 5;      // {l1, l2, u1, u2, invd}
+constexpr int kDRowRec = 17;   // dense special row {order, w[16]}
 
 enum OpClass { kDeriv = 0, kFilter = 1 };
 
@@ -40,6 +41,7 @@ struct OpDev {
     int L;          // lines per workgroup
     int NI;         // interface unknowns per line = 2*W*P
     int ext;        // LDS row stride in doubles (>= N + 2H, odd)
+    int MM;         // table row stride: rows per chunk capacity (16 or 32)
     int periodic;   // halo by wrap
     int nbl, nbr;   // special rows at the left / right end
     int int_off, int_w;        // interior stencil: first offset, taps
@@ -49,8 +51,10 @@ struct OpDev {
     // pass-through of lines on Dirichlet faces of the other directions
     int passA, NA, passB, NB;
     const double *rows;   // (nbl+nbr) * kRowRec, taps in summation order
-    const double *lu;     // P * kMMax * kLuRec
-    const double *sp;     // P * kMMax * 2W
+    const double *drows;  // (nbl+nbr) * kDRowRec dense special rows (fast kernels)
+    double inv_dd;        // 1/dd when that reciprocal is exact (dd a power of two), else 0
+    const double *lu;     // P * MM * kLuRec
+    const double *sp;     // P * MM * 2W
     const double *G;      // P * 2W * NI
     const int *cstart;    // P
     const int *clen;      // P

@@ -1,0 +1,334 @@
+// Fast gfx950 line-operator kernels for evenly chunked lines (N = P * MM).
+//
+// Strided directions (y, z): a wave is 64 consecutive lines (consecutive x)
+// and one chunk of MM rows, so every global load and store is a 512-byte
+// contiguous run and the chunk index is wave-uniform: the factor tables are
+// read with scalar loads.  Each lane keeps its chunk plus the stencil halo in
+// registers; LDS is only used for the interface exchange.
+//
+// Contiguous direction (x): the workgroup stages 64 lines into LDS in a
+// transposed [point][line] layout (row stride 65 doubles), so the coalesced
+// fill and the per-lane chunk reads are both free of bank conflicts; a wave
+// is again 64 lines x one chunk.
+//
+// Per chunk: explicit stencil (taps in the reference's summation order, no
+// FMA contraction, so the explicit stage is bitwise that of the reference),
+// banded LU solve of the chunk block, then the interface correction
+// x = y - S * (G * yhat) with the 2W-per-chunk interface values exchanged
+// through LDS.
+#include <hip/hip_runtime.h>
+
+#include "lineop.h"
+#include "lineop_launch.h"
+
+namespace mc3d {
+
+namespace {
+
+constexpr int kContigLines = 64;
+
+// explicit stage of the MM rows of one chunk from the window w[0..MM+2H)
+// (w[j] = phi[cs - H + j])
+template <int H, int MM, bool PER>
+__device__ __forceinline__ void chunk_rhs(const OpDev &op, const double *w, double *y, bool first,
+                                          bool last) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int i = 0; i < MM; ++i) {
+        double r = 0.0;
+        bool special = false;
+        if (!PER) {
+            if (first && i < kSpecRows && i < op.nbl) {
+                const double *d = op.drows + i * kDRowRec;
+                if (d[0] > 0.0) {
+#pragma unroll
+                    for (int t = 0; t < 16; ++t) r = r + d[1 + t] * w[H + t];
+                } else {
+#pragma unroll
+                    for (int t = 15; t >= 0; --t) r = r + d[1 + t] * w[H + t];
+                }
+                special = true;
+            } else if (last && i >= MM - kSpecRows && i >= MM - op.nbr) {
+                const double *d = op.drows + (op.nbl + i - (MM - op.nbr)) * kDRowRec;
+                if (d[0] > 0.0) {
+#pragma unroll
+                    for (int t = 0; t < 16; ++t) r = r + d[1 + t] * w[H + MM - 16 + t];
+                } else {
+#pragma unroll
+                    for (int t = 15; t >= 0; --t) r = r + d[1 + t] * w[H + MM - 16 + t];
+                }
+                special = true;
+            }
+        }
+        if (!special) {
+#pragma unroll
+            for (int k = 0; k < 2 * H + 1; ++k) r = r + op.int_c[k] * w[i + k];
+        }
+        if (op.inv_dd != 0.0) r = r * op.inv_dd;
+        else if (op.divide) r = r / op.dd;
+        y[i] = r;
+    }
+}
+
+// banded LU solve of one chunk block with the factor rows lu[MM][kLuRec]
+template <int W, int MM>
+__device__ __forceinline__ void chunk_solve(const double *lu, double *y) {
+#pragma unroll
+    for (int i = 1; i < MM; ++i) {
+        double z = y[i] - lu[i * kLuRec + 0] * y[i - 1];
+        if (W == 2 && i >= 2) z -= lu[i * kLuRec + 1] * y[i - 2];
+        y[i] = z;
+    }
+#pragma unroll
+    for (int i = MM - 1; i >= 0; --i) {
+        double x = y[i];
+        if (i + 1 < MM) x -= lu[i * kLuRec + 2] * y[i + 1];
+        if (W == 2 && i + 2 < MM) x -= lu[i * kLuRec + 3] * y[i + 2];
+        y[i] = x * lu[i * kLuRec + 4];
+    }
+}
+
+__device__ __forceinline__ bool pass_line(const OpDev &op, int dir, int q, int Nx, int Ny) {
+    if (!(op.passA | op.passB)) return false;
+    int ia, ib;
+    if (dir == 0) { ia = q % Ny; ib = q / Ny; }
+    else { ia = q % Nx; ib = q / Nx; }
+    return (op.passA && (ia == 0 || ia == op.NA - 1)) || (op.passB && (ib == 0 || ib == op.NB - 1));
+}
+
+template <int W, int H, int MM, bool PER>
+__global__ __launch_bounds__(1024) void lineop_strided(OpDev op, LaunchArgs a) {
+    extern __shared__ double red[];
+    const int l = threadIdx.x;
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int P = op.P, N = op.N, NI = op.NI;
+    const int q = blockIdx.x * 64 + l;
+    const bool live = q < a.nlines;
+    const int qq = live ? q : 0;
+    const double *__restrict__ in = a.in[blockIdx.y];
+    double *__restrict__ out = a.out[blockIdx.y];
+    long base, ps;
+    if (op.dir == 1) {
+        base = (long)(qq / a.Nx) * a.Nx * a.Ny + qq % a.Nx;
+        ps = a.Nx;
+    } else {
+        base = qq;
+        ps = (long)a.Nx * a.Ny;
+    }
+    const int cs = c * MM;
+    double w[MM + 2 * H];
+#pragma unroll
+    for (int j = 0; j < MM + 2 * H; ++j) {
+        int p = cs - H + j;
+        if (PER) {
+            p = p < 0 ? p + N : (p >= N ? p - N : p);
+            w[j] = in[base + p * ps];
+        } else {
+            w[j] = (p >= 0 && p < N) ? in[base + p * ps] : 0.0;
+        }
+    }
+    double y[MM];
+    chunk_rhs<H, MM, PER>(op, w, y, c == 0, c == P - 1);
+    if (W > 0) {
+        chunk_solve<W, MM>(op.lu + (PER ? 0 : (size_t)c * MM * kLuRec), y);
+        double *myred = red + l * (NI + 1);
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            myred[c * 2 * W + k] = y[k];
+            myred[c * 2 * W + W + k] = y[MM - W + k];
+        }
+    }
+    // every lane has read its window (in may equal out) and published its interface
+    __syncthreads();
+    if (W > 0) {
+        const double *myred = red + l * (NI + 1);
+        const double *G = op.G + (size_t)c * 2 * W * NI;
+        double s[2 * W > 0 ? 2 * W : 1];
+#pragma unroll
+        for (int j = 0; j < 2 * W; ++j) s[j] = 0.0;
+        for (int u = 0; u < NI; ++u) {
+            const double v = myred[u];
+#pragma unroll
+            for (int j = 0; j < 2 * W; ++j) s[j] += G[j * NI + u] * v;
+        }
+        const double *sp = op.sp + (PER ? 0 : (size_t)c * MM * 2 * W);
+#pragma unroll
+        for (int i = 0; i < MM; ++i) {
+            double x = y[i];
+#pragma unroll
+            for (int j = 0; j < 2 * W; ++j) x -= sp[i * 2 * W + j] * s[j];
+            y[i] = x;
+        }
+    }
+    if (!live) return;
+    const bool pass = pass_line(op, op.dir, q, a.Nx, a.Ny);
+#pragma unroll
+    for (int i = 0; i < MM; ++i) {
+        const long g = base + (long)(cs + i) * ps;
+        const double v = pass ? in[g] : y[i];
+        if (a.accumulate) out[g] += v;
+        else out[g] = v;
+    }
+}
+
+template <int W, int H, int MM, bool PER>
+__global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
+    extern __shared__ double smem[];
+    constexpr int L = kContigLines;
+    constexpr int LS = L + 1;  // row stride of the transposed tile
+    const int N = op.N, P = op.P, NI = op.NI;
+    const int span = N + 2 * H;
+    double *tile = smem;                      // [span][LS]
+    double *red = smem + (size_t)span * LS;   // [L][NI+1]
+    const int tid = threadIdx.y * L + threadIdx.x;
+    const int nthr = L * P;
+    const int q0 = blockIdx.x * L;
+    const double *__restrict__ in = a.in[blockIdx.y];
+    double *__restrict__ out = a.out[blockIdx.y];
+    const int wave = tid >> 6, lane = tid & 63, nwaves = (nthr + 63) >> 6;
+    // fill: one line per wave iteration, lanes along the line
+    for (int ll = wave; ll < L; ll += nwaves) {
+        const int q = q0 + ll;
+        if (q >= a.nlines) continue;
+        const double *row = in + (long)q * N;
+        for (int e = lane; e < span; e += 64) {
+            int p = e - H;
+            p = p < 0 ? p + N : (p >= N ? p - N : p);
+            if (PER || (e >= H && e < H + N)) tile[e * LS + ll] = row[p];
+        }
+    }
+    __syncthreads();
+    const int l = threadIdx.x;
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int q = q0 + l;
+    const bool live = q < a.nlines;
+    const int cs = c * MM;
+    double w[MM + 2 * H];
+#pragma unroll
+    for (int j = 0; j < MM + 2 * H; ++j) {
+        const int e = cs + j;  // ext index of phi[cs - H + j]
+        w[j] = (PER || (e >= H && e < H + N)) ? tile[e * LS + l] : 0.0;
+    }
+    double y[MM];
+    chunk_rhs<H, MM, PER>(op, w, y, c == 0, c == P - 1);
+    if (W > 0) {
+        chunk_solve<W, MM>(op.lu + (PER ? 0 : (size_t)c * MM * kLuRec), y);
+        double *myred = red + l * (NI + 1);
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            myred[c * 2 * W + k] = y[k];
+            myred[c * 2 * W + W + k] = y[MM - W + k];
+        }
+    }
+    __syncthreads();
+    if (W > 0) {
+        const double *myred = red + l * (NI + 1);
+        double s[2 * W > 0 ? 2 * W : 1];
+#pragma unroll
+        for (int j = 0; j < 2 * W; ++j) s[j] = 0.0;
+        if (PER) {
+            // periodic chunks are identical up to rotation: G_c[j][u] = G_0[j][u - 2Wc]
+            const int rot = c * 2 * W;
+            for (int v = 0; v < NI; ++v) {
+                int u = v + rot;
+                u = u >= NI ? u - NI : u;
+                const double yv = myred[u];
+#pragma unroll
+                for (int j = 0; j < 2 * W; ++j) s[j] += op.G[j * NI + v] * yv;
+            }
+        } else {
+            const double *G = op.G + (size_t)c * 2 * W * NI;
+            for (int u = 0; u < NI; ++u) {
+                const double yv = myred[u];
+#pragma unroll
+                for (int j = 0; j < 2 * W; ++j) s[j] += G[j * NI + u] * yv;
+            }
+        }
+        const double *sp = op.sp + (PER ? 0 : (size_t)c * MM * 2 * W);
+#pragma unroll
+        for (int i = 0; i < MM; ++i) {
+            double x = y[i];
+#pragma unroll
+            for (int j = 0; j < 2 * W; ++j) x -= sp[i * 2 * W + j] * s[j];
+            y[i] = x;
+        }
+    }
+    const bool pass = live && pass_line(op, 0, q, a.Nx, a.Ny);
+    if (live && !pass) {
+#pragma unroll
+        for (int i = 0; i < MM; ++i) tile[(H + cs + i) * LS + l] = y[i];
+    }
+    __syncthreads();
+    for (int ll = wave; ll < L; ll += nwaves) {
+        const int qq = q0 + ll;
+        if (qq >= a.nlines) continue;
+        double *row = out + (long)qq * N;
+        for (int p = lane; p < N; p += 64) {
+            const double v = tile[(H + p) * LS + ll];
+            if (a.accumulate) row[p] += v;
+            else row[p] = v;
+        }
+    }
+}
+
+template <int W, int H, int MM, bool PER>
+hipError_t launch_fast(const OpDev &op, const LaunchArgs &a, int nfields, hipStream_t st) {
+    if (op.dir == 0) {
+        const int L = kContigLines;
+        dim3 grid((a.nlines + L - 1) / L, nfields), block(L, op.P);
+        const size_t lds = sizeof(double) * ((size_t)(op.N + 2 * H) * (L + 1) + (size_t)L * (op.NI + 1));
+        hipLaunchKernelGGL((lineop_contig<W, H, MM, PER>), grid, block, lds, st, op, a);
+    } else {
+        dim3 grid((a.nlines + 63) / 64, nfields), block(64, op.P);
+        const size_t lds = sizeof(double) * (size_t)64 * (op.NI + 1);
+        hipLaunchKernelGGL((lineop_strided<W, H, MM, PER>), grid, block, lds, st, op, a);
+    }
+    return hipGetLastError();
+}
+
+template <int W, int H, int MM>
+hipError_t launch_fast_bc(const OpDev &op, const LaunchArgs &a, int nfields, hipStream_t st) {
+    return op.periodic ? launch_fast<W, H, MM, true>(op, a, nfields, st)
+                       : launch_fast<W, H, MM, false>(op, a, nfields, st);
+}
+
+template <int W, int H>
+hipError_t launch_fast_mm(const OpDev &op, const LaunchArgs &a, int nfields, hipStream_t st) {
+    return op.MM == 32 ? launch_fast_bc<W, H, 32>(op, a, nfields, st)
+                       : launch_fast_bc<W, H, 16>(op, a, nfields, st);
+}
+
+}  // namespace
+
+size_t lineop_fast_lds_bytes(const OpDev &op) {
+    if (op.dir == 0)
+        return sizeof(double) * ((size_t)(op.N + 2 * op.H) * (kContigLines + 1) +
+                                 (size_t)kContigLines * (op.NI + 1));
+    return sizeof(double) * (size_t)64 * (op.NI + 1);
+}
+
+bool lineop_fast_supported(const OpDev &op, size_t lds_limit) {
+    const int w = op.W, h = op.H;
+    const bool pair = (w == 0 && (h == 1 || h == 3)) || (w == 1 && (h == 2 || h == 4 || h == 5)) ||
+                      (w == 2 && (h == 3 || h == 5));
+    const int threads = 64 * op.P;
+    // Dirichlet interior rows must not reach past the line ends
+    const bool ends = op.periodic || (op.nbl >= h && op.nbr >= h && op.nbl <= kSpecRows && op.nbr <= kSpecRows);
+    return pair && ends && (op.MM == 16 || op.MM == 32) && op.N == op.P * op.MM && threads <= 1024 &&
+           lineop_fast_lds_bytes(op) <= lds_limit;
+}
+
+hipError_t lineop_launch_fast(const OpDev &op, const LaunchArgs &a, int nfields, hipStream_t st) {
+    switch (op.W * 8 + op.H) {
+        case 0 * 8 + 1: return launch_fast_mm<0, 1>(op, a, nfields, st);
+        case 0 * 8 + 3: return launch_fast_mm<0, 3>(op, a, nfields, st);
+        case 1 * 8 + 2: return launch_fast_mm<1, 2>(op, a, nfields, st);
+        case 1 * 8 + 4: return launch_fast_mm<1, 4>(op, a, nfields, st);
+        case 1 * 8 + 5: return launch_fast_mm<1, 5>(op, a, nfields, st);
+        case 2 * 8 + 3: return launch_fast_mm<2, 3>(op, a, nfields, st);
+        case 2 * 8 + 5: return launch_fast_mm<2, 5>(op, a, nfields, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace mc3d

@@ -5,7 +5,7 @@
 // x-lines consecutive lanes walk along the line, for y/z-lines consecutive
 // lanes walk across neighbouring lines (consecutive x), so every global access
 // is unit-stride whatever the direction.  Phase 2 gives each line P lanes
-// (one per chunk of <= kMMax rows, rows held in registers): explicit stencil,
+// (one per chunk of <= MM rows, rows held in registers): explicit stencil,
 // banded LU solve of the chunk block, exchange of 2W interface values per
 // chunk through LDS, dense interface correction.  Phase 3 writes the tile back
 // through LDS with the same coalesced pattern.
@@ -52,7 +52,7 @@ __device__ __forceinline__ double rhs_row(const OpDev &op, const double *trow, i
     return r;
 }
 
-template <int W, int DIR, int MODE>
+template <int W, int DIR, int MODE, int MM>
 __global__ __launch_bounds__(256) void lineop_kernel(OpDev op, LaunchArgs a) {
     extern __shared__ double smem[];
     const int L = op.L, P = op.P, N = op.N, H = op.H, EXT = op.ext;
@@ -104,23 +104,23 @@ __global__ __launch_bounds__(256) void lineop_kernel(OpDev op, LaunchArgs a) {
     const int cm = op.clen[cc];
     const double *trow = tile + l * EXT + H;
 
-    double y[kMMax];
+    double y[MM];
 #pragma unroll
-    for (int i = 0; i < kMMax; ++i) y[i] = (live && i < cm) ? rhs_row(op, trow, cs + i) : 0.0;
+    for (int i = 0; i < MM; ++i) y[i] = (live && i < cm) ? rhs_row(op, trow, cs + i) : 0.0;
 
     if (W > 0) {
-        const double *lu = op.lu + (size_t)cc * kMMax * kLuRec;
+        const double *lu = op.lu + (size_t)cc * MM * kLuRec;
 #pragma unroll
-        for (int i = 1; i < kMMax; ++i) {
+        for (int i = 1; i < MM; ++i) {
             double z = y[i] - lu[i * kLuRec + 0] * y[i - 1];
             if (W == 2 && i >= 2) z -= lu[i * kLuRec + 1] * y[i - 2];
             y[i] = z;
         }
 #pragma unroll
-        for (int i = kMMax - 1; i >= 0; --i) {
+        for (int i = MM - 1; i >= 0; --i) {
             double x = y[i];
-            if (i + 1 < kMMax) x -= lu[i * kLuRec + 2] * y[i + 1];
-            if (W == 2 && i + 2 < kMMax) x -= lu[i * kLuRec + 3] * y[i + 2];
+            if (i + 1 < MM) x -= lu[i * kLuRec + 2] * y[i + 1];
+            if (W == 2 && i + 2 < MM) x -= lu[i * kLuRec + 3] * y[i + 2];
             y[i] = x * lu[i * kLuRec + 4];
         }
         // interface values
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(256) void lineop_kernel(OpDev op, LaunchArgs a) {
                 double v = 0.0;
                 const int want = cm - W + k;
 #pragma unroll
-                for (int i = 0; i < kMMax; ++i) v = (i == want) ? y[i] : v;
+                for (int i = 0; i < MM; ++i) v = (i == want) ? y[i] : v;
                 myred[c * 2 * W + W + k] = v;
             }
         }
@@ -146,9 +146,9 @@ __global__ __launch_bounds__(256) void lineop_kernel(OpDev op, LaunchArgs a) {
             for (int u = 0; u < NI; ++u) acc += G[j * NI + u] * myred[u];
             s[j] = acc;
         }
-        const double *sp = op.sp + (size_t)cc * kMMax * 2 * W;
+        const double *sp = op.sp + (size_t)cc * MM * 2 * W;
 #pragma unroll
-        for (int i = 0; i < kMMax; ++i) {
+        for (int i = 0; i < MM; ++i) {
             double x = y[i];
 #pragma unroll
             for (int j = 0; j < 2 * W; ++j) x -= sp[i * 2 * W + j] * s[j];
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(256) void lineop_kernel(OpDev op, LaunchArgs a) {
     }
     if (live && !pass) {
 #pragma unroll
-        for (int i = 0; i < kMMax; ++i)
+        for (int i = 0; i < MM; ++i)
             if (i < cm) tile[l * EXT + H + cs + i] = y[i];
     }
     __syncthreads();
@@ -189,29 +189,36 @@ __global__ __launch_bounds__(256) void lineop_kernel(OpDev op, LaunchArgs a) {
     }
 }
 
-template <int W, int DIR>
+template <int W, int DIR, int MM>
 static hipError_t launch_mode(const OpDev &op, const LaunchArgs &a, int mode, dim3 grid, dim3 block,
                               size_t lds, hipStream_t st) {
     switch (mode) {
         case kModePlainPeriodic:
-            hipLaunchKernelGGL((lineop_kernel<W, DIR, kModePlainPeriodic>), grid, block, lds, st, op, a);
+            hipLaunchKernelGGL((lineop_kernel<W, DIR, kModePlainPeriodic, MM>), grid, block, lds, st, op, a);
             break;
         case kModePlainDirichlet:
-            hipLaunchKernelGGL((lineop_kernel<W, DIR, kModePlainDirichlet>), grid, block, lds, st, op, a);
+            hipLaunchKernelGGL((lineop_kernel<W, DIR, kModePlainDirichlet, MM>), grid, block, lds, st, op, a);
             break;
         default:
-            hipLaunchKernelGGL((lineop_kernel<W, DIR, kModeTPB>), grid, block, lds, st, op, a);
+            hipLaunchKernelGGL((lineop_kernel<W, DIR, kModeTPB, MM>), grid, block, lds, st, op, a);
             break;
     }
     return hipGetLastError();
 }
 
+template <int W, int DIR>
+static hipError_t launch_mm(const OpDev &op, const LaunchArgs &a, int mode, dim3 grid, dim3 block,
+                            size_t lds, hipStream_t st) {
+    if (op.MM == 32) return launch_mode<W, DIR, 32>(op, a, mode, grid, block, lds, st);
+    return launch_mode<W, DIR, 16>(op, a, mode, grid, block, lds, st);
+}
+
 template <int W>
 static hipError_t launch_dir(const OpDev &op, const LaunchArgs &a, int mode, dim3 grid, dim3 block,
                              size_t lds, hipStream_t st) {
-    if (op.dir == 0) return launch_mode<W, 0>(op, a, mode, grid, block, lds, st);
-    if (op.dir == 1) return launch_mode<W, 1>(op, a, mode, grid, block, lds, st);
-    return launch_mode<W, 2>(op, a, mode, grid, block, lds, st);
+    if (op.dir == 0) return launch_mm<W, 0>(op, a, mode, grid, block, lds, st);
+    if (op.dir == 1) return launch_mm<W, 1>(op, a, mode, grid, block, lds, st);
+    return launch_mm<W, 2>(op, a, mode, grid, block, lds, st);
 }
 
 size_t lineop_lds_bytes(const OpDev &op) {

@@ -18,8 +18,15 @@ struct LaunchArgs {
     int accumulate;  // out += result instead of out = result
 };
 
+// LDS-tiled kernel, any chunking, all modes (TPB included)
 size_t lineop_lds_bytes(const OpDev &op);
 hipError_t lineop_launch(const OpDev &op, const LaunchArgs &a, int nfields, int mode,
                          hipStream_t st);
+
+// fast kernels: evenly chunked lines (N = P * MM), plain periodic (no special
+// rows) or Dirichlet with dense special rows; no TPB
+size_t lineop_fast_lds_bytes(const OpDev &op);
+bool lineop_fast_supported(const OpDev &op, size_t lds_limit);
+hipError_t lineop_launch_fast(const OpDev &op, const LaunchArgs &a, int nfields, hipStream_t st);
 
 }  // namespace mc3d

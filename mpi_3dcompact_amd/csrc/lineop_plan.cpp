@@ -413,19 +413,29 @@ int build_plan(const PlanRequest &rq, Plan &plan, std::string &err) {
         else { err = "unknown filter kind"; return -1; }
     }
     const int W = s.W;
-    // chunking
-    int P = (n + kMMax - 1) / kMMax;
+    // chunking: 32-row chunks when the line splits evenly into at least two,
+    // else 16-row chunks (the last one possibly shorter)
+    int mm = rq.mm;
+    if (mm == 0) mm = (n % 32 == 0 && n / 32 >= 2) ? 32 : 16;
+    if (mm != 16 && mm != 32) { err = "chunk rows must be 16 or 32"; return -1; }
+    int P = (n + mm - 1) / mm;
     if (per && W > 0 && P < 2) P = 2;
     int M = (n + P - 1) / P;
     int last = n - (P - 1) * M;
-    if (last < std::max(2 * W, 1) || M > kMMax) { err = "cannot partition line"; return -1; }
+    if (last < std::max(2 * W, 1) || M > mm) { err = "cannot partition line"; return -1; }
     plan.N = n; plan.W = W; plan.H = s.H; plan.P = P; plan.periodic = per;
-    int L = 1;
-    while (L * 2 * P <= 256) L *= 2;
-    plan.L = L;
+    plan.MM = mm;
+    plan.uniform = (M == mm && last == mm);
     plan.NI = 2 * W * P;
     plan.ext = n + 2 * s.H;
     if ((plan.ext & 1) == 0) plan.ext += 1;
+    // lines per workgroup of the LDS-tiled kernel: at most 256 lanes and 64 KiB
+    auto tile_bytes = [&](int l) {
+        return sizeof(double) * ((size_t)l * plan.ext + (size_t)l * std::max(plan.NI, 1));
+    };
+    int L = 1;
+    while (L * 2 * P <= 256 && tile_bytes(2 * L) <= 65536) L *= 2;
+    plan.L = L;
     plan.divide = s.divide;
     plan.dd = s.divide ? rq.dd : 1.0;
     plan.int_off = s.int_off;
@@ -442,17 +452,31 @@ int build_plan(const PlanRequest &rq, Plan &plan, std::string &err) {
     };
     for (int r = 0; r < plan.nbl; ++r) put_row(r, s.left[r]);
     for (int r = 0; r < plan.nbr; ++r) put_row(plan.nbl + r, s.right[r]);
+    // dense form: absolute tap positions must fall inside the 16 points at the end
+    plan.drows.assign((size_t)(plan.nbl + plan.nbr) * kDRowRec, 0.0);
+    plan.dense_ok = n >= 16;
+    auto put_dense = [&](int idx, const Row &r, int row, int origin) {
+        double *d = &plan.drows[(size_t)idx * kDRowRec];
+        d[0] = r.step > 0 ? 1.0 : -1.0;
+        for (size_t t = 0; t < r.c.size(); ++t) {
+            const int pos = row + r.off + (int)t * r.step - origin;
+            if (pos < 0 || pos >= 16 || (r.step != 1 && r.step != -1)) { plan.dense_ok = false; return; }
+            d[1 + pos] = r.c[t];
+        }
+    };
+    for (int r = 0; r < plan.nbl; ++r) put_dense(r, s.left[r], r, 0);
+    for (int r = 0; r < plan.nbr; ++r) put_dense(plan.nbl + r, s.right[r], n - plan.nbr + r, n - 16);
     plan.cstart.resize(P);
     plan.clen.resize(P);
     for (int c = 0; c < P; ++c) {
         plan.cstart[c] = c * M;
         plan.clen[c] = (c == P - 1) ? last : M;
     }
-    plan.lu.assign((size_t)P * kMMax * kLuRec, 0.0);
-    plan.sp.assign((size_t)P * kMMax * std::max(2 * W, 1), 0.0);
+    plan.lu.assign((size_t)P * mm * kLuRec, 0.0);
+    plan.sp.assign((size_t)P * mm * std::max(2 * W, 1), 0.0);
     plan.G.assign((size_t)P * std::max(2 * W, 1) * std::max(plan.NI, 1), 0.0);
     for (int c = 0; c < P; ++c)
-        for (int i = 0; i < kMMax; ++i) plan.lu[((size_t)c * kMMax + i) * kLuRec + 4] = 1.0;
+        for (int i = 0; i < mm; ++i) plan.lu[((size_t)c * mm + i) * kLuRec + 4] = 1.0;
     if (W == 0) return 0;
 
     const int NI = plan.NI, W2 = 2 * W;
@@ -472,7 +496,7 @@ int build_plan(const PlanRequest &rq, Plan &plan, std::string &err) {
             return -4;
         }
         for (int i = 0; i < cm; ++i) {
-            double *e = &plan.lu[((size_t)c * kMMax + i) * kLuRec];
+            double *e = &plan.lu[((size_t)c * mm + i) * kLuRec];
             e[0] = (double)l1[i]; e[1] = (double)l2[i];
             e[2] = (double)u1[i]; e[3] = (double)u2[i];
             e[4] = (double)(1.0L / dd[i]);
@@ -506,7 +530,7 @@ int build_plan(const PlanRequest &rq, Plan &plan, std::string &err) {
             lu_solve(W, l1, l2, u1, u2, dd, col);
             for (int i = 0; i < cm; ++i) {
                 S[c][(size_t)i * W2 + k] = col[i];
-                plan.sp[((size_t)c * kMMax + i) * W2 + k] = (double)col[i];
+                plan.sp[((size_t)c * mm + i) * W2 + k] = (double)col[i];
             }
         }
     }
@@ -536,6 +560,14 @@ int build_plan(const PlanRequest &rq, Plan &plan, std::string &err) {
             for (int u = 0; u < NI; ++u)
                 plan.G[((size_t)c * W2 + j) * NI + u] = (double)R(v, u);
         }
+    // evenly chunked periodic lines: the interface system is block circulant, so
+    // chunk c's rows are chunk 0's rotated by c chunks; store them exactly so
+    // (kernels may read chunk 0's rows with a rotated index)
+    if (per && plan.uniform)
+        for (int c = 1; c < P; ++c)
+            for (int j = 0; j < W2; ++j)
+                for (int u = 0; u < NI; ++u)
+                    plan.G[((size_t)c * W2 + j) * NI + u] = plan.G[(size_t)j * NI + (u - c * W2 + NI) % NI];
     return 0;
 }
 

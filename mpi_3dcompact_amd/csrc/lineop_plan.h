@@ -15,16 +15,23 @@ struct PlanRequest {
     int N;
     double dd;      // derivative spacing
     double alphaF;  // filter coefficient
+    int mm = 0;     // rows per chunk (16 or 32); 0 chooses
 };
 
 struct Plan {
     int N = 0, W = 0, H = 0, P = 0, L = 0, NI = 0, ext = 0;
+    int MM = 16;         // table row stride (rows per chunk capacity)
+    bool uniform = false;  // every chunk has MM rows
+    bool dense_ok = false;  // special rows fit the 16-wide dense form
     bool periodic = false, divide = false;
     double dd = 1.0;
     int int_off = 0;
     std::vector<double> int_c;
     int nbl = 0, nbr = 0;
     std::vector<double> rows, lu, sp, G;
+    // special rows as dense 16-wide rows: {order (+1 increasing, -1 decreasing), w[16]};
+    // left rows over phi[0..15], right rows over phi[N-16..N-1]
+    std::vector<double> drows;
     std::vector<int> cstart, clen;
 };
 

@@ -59,6 +59,10 @@ class LineOp:
         check(lib().mc3d_lineop_info(self._h, ctypes.byref(p), ctypes.byref(m), ctypes.byref(l)))
         return p.value, m.value, l.value
 
+    def kernel(self):
+        """Kernel instantiation a plain application launches (profiler name)."""
+        return lib().mc3d_lineop_kernel(self._h).decode()
+
     def apply(self, ins, outs, stream=None):
         ins = list(ins) if isinstance(ins, (list, tuple)) else [ins]
         outs = list(outs) if isinstance(outs, (list, tuple)) else [outs]
@@ -90,27 +94,31 @@ class LineOp:
             pass
 
 
-def plan_tables(cls, kind, bc, n, dd=1.0, alpha_f=0.0):
+def plan_tables(cls, kind, bc, n, dd=1.0, alpha_f=0.0, mm=0):
     """Host-built partition tables of an operator (diagnostic; no GPU needed)."""
-    sizes = (ctypes.c_int * 8)()
-    check(lib().mc3d_plan_tables(cls, kind, bc, n, dd, alpha_f, sizes, None, None, None, None,
-                                 None, None, None))
-    N, W, H, P, NI, nbl, nbr, int_off = list(sizes)
+    sizes = (ctypes.c_int * 10)()
+    check(lib().mc3d_plan_tables(cls, kind, bc, n, dd, alpha_f, mm, sizes, None, None, None, None,
+                                 None, None, None, None))
+    N, W, H, P, NI, nbl, nbr, int_off, MM, dense_ok = list(sizes)
     W2 = max(2 * W, 1)
-    lu = np.zeros(P * 16 * 5)
-    sp = np.zeros(P * 16 * W2)
+    nrow = max(nbl + nbr, 1)
+    lu = np.zeros(P * MM * 5)
+    sp = np.zeros(P * MM * W2)
     G = np.zeros(P * W2 * max(NI, 1))
-    rows = np.zeros(max(nbl + nbr, 1) * 19)
+    rows = np.zeros(nrow * 19)
+    drows = np.zeros(nrow * 17)
     interior = np.zeros(11)
     cstart = np.zeros(P, dtype=np.int32)
     clen = np.zeros(P, dtype=np.int32)
     D = ctypes.POINTER(ctypes.c_double)
     I = ctypes.POINTER(ctypes.c_int)
-    check(lib().mc3d_plan_tables(cls, kind, bc, n, dd, alpha_f, sizes,
+    check(lib().mc3d_plan_tables(cls, kind, bc, n, dd, alpha_f, mm, sizes,
                                  lu.ctypes.data_as(D), sp.ctypes.data_as(D), G.ctypes.data_as(D),
-                                 rows.ctypes.data_as(D), interior.ctypes.data_as(D),
+                                 rows.ctypes.data_as(D), drows.ctypes.data_as(D),
+                                 interior.ctypes.data_as(D),
                                  cstart.ctypes.data_as(I), clen.ctypes.data_as(I)))
-    return dict(N=N, W=W, H=H, P=P, NI=NI, nbl=nbl, nbr=nbr, int_off=int_off,
-                lu=lu.reshape(P, 16, 5), sp=sp.reshape(P, 16, W2), G=G.reshape(P, W2, max(NI, 1)),
-                rows=rows.reshape(-1, 19), interior=interior, cstart=cstart, clen=clen,
-                periodic=(bc == MC3D_PERIODIC))
+    return dict(N=N, W=W, H=H, P=P, NI=NI, nbl=nbl, nbr=nbr, int_off=int_off, MM=MM,
+                dense_ok=bool(dense_ok),
+                lu=lu.reshape(P, MM, 5), sp=sp.reshape(P, MM, W2), G=G.reshape(P, W2, max(NI, 1)),
+                rows=rows.reshape(-1, 19), drows=drows.reshape(-1, 17), interior=interior,
+                cstart=cstart, clen=clen, periodic=(bc == MC3D_PERIODIC))

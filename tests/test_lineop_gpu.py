@@ -111,6 +111,39 @@ def test_batched_fields_and_inplace_filter(torch_cuda):
         assert relerr(o.cpu().numpy(), P.filter_field(P.COMPACT8, 0.49, 2, [0, 0, 0], f)) <= TOL
 
 
+@pytest.mark.parametrize("kind,af", [(P.COMPACT8, 0.49), (P.COMMN5PADE, 0.0)])
+@pytest.mark.parametrize("bc3", [(0, 0, 0), (1, 1, 1)])
+def test_inplace_filter_fast_kernels(torch_cuda, kind, af, bc3):
+    """In-place filtering through the fast kernels (evenly chunked lines in all
+    three directions), with Dirichlet pass-through lines when bc3 asks."""
+    from mpi_3dcompact_amd.ops import LineOp
+    torch = torch_cuda
+    shape = (48, 32, 64)
+    dims = (64, 32, 48)
+    fs = [field(shape, 30 + i) for i in range(3)]
+    for d in range(3):
+        fl = LineOp.filter(kind, af, d, bc3, dims)
+        kname = fl.kernel()
+        _say(f"in-place filter kind={kind} bc3={bc3} dir={d} kernel={kname}")
+        assert kname.startswith("lineop_contig" if d == 0 else "lineop_strided"), kname
+        ts = [torch.from_numpy(f).cuda() for f in fs]
+        fl.apply(ts, ts)
+        torch.cuda.synchronize()
+        for f, t in zip(fs, ts):
+            assert relerr(t.cpu().numpy(), P.filter_field(kind, af, d, list(bc3), f)) <= TOL
+
+
+def test_kernel_choice(torch_cuda):
+    """Evenly chunked lines take the fast kernels; ragged lines, CD2 with its
+    periodic wrap rows and TPB take the LDS-tiled kernel."""
+    from mpi_3dcompact_amd.ops import LineOp
+    dims = (64, 17, 48)
+    assert LineOp.deriv(P.PENTA10, 0, P.PERIODIC, dims, 1 / 64).kernel() == "lineop_contig<2, 3, 32, true>"
+    assert LineOp.deriv(P.PENTA10, 2, P.DIRICHLET, dims, 1 / 47).kernel() == "lineop_strided<2, 3, 16, false>"
+    assert LineOp.deriv(P.PADE6, 1, P.PERIODIC, dims, 1 / 17).kernel().startswith("lineop_kernel<")
+    assert LineOp.deriv(P.CD2, 0, P.PERIODIC, dims, 1 / 64).kernel().startswith("lineop_kernel<")
+
+
 @pytest.mark.parametrize("scheme", [P.PADE6, P.PENTA10, P.CD2])
 def test_tpb_wrap_values(torch_cuda, scheme):
     from mpi_3dcompact_amd.ops import LineOp

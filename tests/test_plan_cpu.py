@@ -1,9 +1,11 @@
 """CPU check of the partitioned line-solve plan (host logic of libmc3d.so).
 
-mc3d_plan_tables returns the tables the GPU kernel consumes (chunk LU factors,
-spikes, interface-correction rows, stencils).  This test replays the kernel's
-arithmetic on them in numpy, line by line, and compares with the oracle, so a
-wrong factorisation or coupling is caught before any GPU run.  It also checks
+mc3d_plan_tables returns the tables the GPU kernels consume (chunk LU factors,
+spikes, interface-correction rows, stencils, dense end rows).  This test replays
+the arithmetic of both kernel families (the LDS-tiled kernel for any chunking
+and the fast kernels for evenly chunked lines) on them in numpy, line by line,
+and compares with the oracle, so a wrong factorisation, coupling or table
+layout is caught before any GPU run.  It also checks
 that the C ABI library loads and exports every symbol include/mc3d.h declares.
 """
 import os
@@ -44,7 +46,8 @@ def emulate(t, f):
     ys = []
     for c in range(Pn):
         cs, cm = int(t["cstart"][c]), int(t["clen"][c])
-        y = np.zeros(16)
+        MM = t["MM"]
+        y = np.zeros(MM)
         for i in range(cm):
             p = cs + i
             if p < nbl or p >= N - nbr:
@@ -60,15 +63,15 @@ def emulate(t, f):
             y[i] = r / t["dd"]
         if W > 0:
             lu = t["lu"][c]
-            for i in range(1, 16):
+            for i in range(1, MM):
                 y[i] -= lu[i, 0] * y[i - 1]
                 if W == 2 and i >= 2:
                     y[i] -= lu[i, 1] * y[i - 2]
-            for i in range(15, -1, -1):
+            for i in range(MM - 1, -1, -1):
                 x = y[i]
-                if i + 1 < 16:
+                if i + 1 < MM:
                     x -= lu[i, 2] * y[i + 1]
-                if W == 2 and i + 2 < 16:
+                if W == 2 and i + 2 < MM:
                     x -= lu[i, 3] * y[i + 2]
                 y[i] = x * lu[i, 4]
         ys.append(y)
@@ -89,10 +92,88 @@ def emulate(t, f):
     return out
 
 
-def tables(cls, kind, bc, n, dd=1.0, af=0.0):
+def _lu_solve(lu, y, W):
+    MM = len(y)
+    for i in range(1, MM):
+        y[i] -= lu[i, 0] * y[i - 1]
+        if W == 2 and i >= 2:
+            y[i] -= lu[i, 1] * y[i - 2]
+    for i in range(MM - 1, -1, -1):
+        x = y[i]
+        if i + 1 < MM:
+            x -= lu[i, 2] * y[i + 1]
+        if W == 2 and i + 2 < MM:
+            x -= lu[i, 3] * y[i + 2]
+        y[i] = x * lu[i, 4]
+
+
+def _dense_row(d, seg):
+    """Dense special row d = {order, w[16]} over the 16 end points seg."""
+    r = 0.0
+    order = range(16) if d[0] > 0 else range(15, -1, -1)
+    for k in order:
+        r = r + d[1 + k] * seg[k]
+    return r
+
+
+def fast_eligible(t):
+    rows_ok = (t["nbl"] + t["nbr"] == 0) if t["periodic"] else (
+        t["dense_ok"] and t["H"] <= t["nbl"] <= 5 and t["H"] <= t["nbr"] <= 5)
+    return t["N"] == t["P"] * t["MM"] and rows_ok
+
+
+def emulate_fast(t, f):
+    """Replay the fast kernels (evenly chunked lines) on one line f: register
+    window per chunk, dense end rows, multiplication by an exact 1/dd, chunk 0
+    factors for every periodic chunk and chunk 0's interface rows rotated."""
+    N, W, Pn, H, MM, NI = t["N"], t["W"], t["P"], t["H"], t["MM"], t["NI"]
+    nbl, nbr, per = t["nbl"], t["nbr"], t["periodic"]
+    dd = t["dd"]
+    m, ex = np.frexp(dd)
+    inv = 1.0 / dd if (t["divide"] and m == 0.5) else 0.0
+    ys = []
+    for c in range(Pn):
+        cs = c * MM
+        idx = np.arange(cs - H, cs + MM + H)
+        if per:
+            w = f[idx % N]
+        else:
+            w = np.where((idx >= 0) & (idx < N), f[np.clip(idx, 0, N - 1)], 0.0)
+        y = np.zeros(MM)
+        for i in range(MM):
+            if not per and c == 0 and i < nbl:
+                r = _dense_row(t["drows"][i], w[H:H + 16])
+            elif not per and c == Pn - 1 and i >= MM - nbr:
+                r = _dense_row(t["drows"][nbl + i - (MM - nbr)], w[H + MM - 16:H + MM])
+            else:
+                r = 0.0
+                for k in range(2 * H + 1):
+                    r = r + t["interior"][k] * w[i + k]
+            if inv != 0.0:
+                r = r * inv
+            elif t["divide"]:
+                r = r / dd
+            y[i] = r
+        if W > 0:
+            _lu_solve(t["lu"][0 if per else c], y, W)
+        ys.append(y)
+    if W > 0:
+        yhat = np.concatenate([np.concatenate([y[:W], y[MM - W:]]) for y in ys])
+        for c in range(Pn):
+            if per:
+                G = np.roll(t["G"][0], c * 2 * W, axis=1)
+            else:
+                G = t["G"][c]
+            s = G @ yhat
+            ys[c] = ys[c] - t["sp"][0 if per else c] @ s
+    return np.concatenate(ys)
+
+
+def tables(cls, kind, bc, n, dd=1.0, af=0.0, mm=0):
     from mpi_3dcompact_amd.ops import plan_tables
-    t = plan_tables(cls, kind, bc, n, dd, af)
+    t = plan_tables(cls, kind, bc, n, dd, af, mm)
     t["dd"] = dd if cls == 0 else 1.0
+    t["divide"] = cls == 0
     return t
 
 
@@ -103,29 +184,76 @@ FILTS = [(P.COMPACT8, 0.49), (P.COMPACT10, 0.49), (P.COMPACT10, -0.49), (P.COMMN
          (P.COMMN5EXP, 0.0)]
 
 
+def _check(t, ref_fn, n):
+    for _ in range(3):
+        f = RNG.normal(size=n)
+        ref = ref_fn(f)
+        tol = 1e-12 * max(1.0, np.max(np.abs(ref)))
+        assert np.max(np.abs(emulate(t, f) - ref)) <= tol
+        if fast_eligible(t):
+            assert np.max(np.abs(emulate_fast(t, f) - ref)) <= tol
+
+
+@pytest.mark.parametrize("mm", [0, 16])
 @pytest.mark.parametrize("bc", [P.PERIODIC, P.DIRICHLET])
 @pytest.mark.parametrize("n", [11, 17, 32, 48, 64, 96, 256])
 @pytest.mark.parametrize("scheme", DERIVS)
-def test_deriv_plan_matches_oracle(scheme, n, bc):
+def test_deriv_plan_matches_oracle(scheme, n, bc, mm):
     dd = 1.0 / n if bc == P.PERIODIC else 1.0 / (n - 1)
-    t = tables(0, scheme, bc, n, dd)
-    for _ in range(3):
-        f = RNG.normal(size=n)
-        ref = P.deriv_line(scheme, bc, f, dd)
-        got = emulate(t, f)
-        assert np.max(np.abs(got - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
+    t = tables(0, scheme, bc, n, dd, mm=mm)
+    _check(t, lambda f: P.deriv_line(scheme, bc, f, dd), n)
 
 
+@pytest.mark.parametrize("mm", [0, 16])
 @pytest.mark.parametrize("bc", [P.PERIODIC, P.DIRICHLET])
 @pytest.mark.parametrize("n", [11, 17, 32, 48, 64, 96, 256])
 @pytest.mark.parametrize("kind,af", FILTS)
-def test_filter_plan_matches_oracle(kind, af, n, bc):
-    t = tables(1, kind, bc, n, 1.0, af)
-    for _ in range(3):
-        f = RNG.normal(size=n)
-        ref = P.filter_line(kind, bc, f, af)
-        got = emulate(t, f)
-        assert np.max(np.abs(got - ref)) <= 1e-12 * max(1.0, np.max(np.abs(ref)))
+def test_filter_plan_matches_oracle(kind, af, n, bc, mm):
+    t = tables(1, kind, bc, n, 1.0, af, mm=mm)
+    _check(t, lambda f: P.filter_line(kind, bc, f, af), n)
+
+
+def test_chunking_choice():
+    """32-row chunks when a line splits evenly into at least two of them."""
+    assert tables(0, P.PENTA10, P.PERIODIC, 256, 1 / 256)["MM"] == 32
+    assert tables(0, P.PENTA10, P.PERIODIC, 256, 1 / 256)["P"] == 8
+    assert tables(0, P.PENTA10, P.PERIODIC, 48, 1 / 48)["MM"] == 16
+    assert tables(0, P.PENTA10, P.PERIODIC, 32, 1 / 32)["MM"] == 16
+    assert tables(0, P.PADE6, P.DIRICHLET, 17, 1 / 16)["MM"] == 16
+
+
+@pytest.mark.parametrize("kind", [P.PADE6, P.PENTA10, P.CD2])
+@pytest.mark.parametrize("n", [32, 64, 256])
+def test_dense_rows_match_row_records_bitwise(kind, n):
+    """The dense end rows sum the same taps in the same order as the row
+    records (zero taps add exact zeros), so both kernel families produce the
+    same bits for the explicit stage at the line ends."""
+    t = tables(0, kind, P.DIRICHLET, n, 1.0 / (n - 1))
+    assert t["dense_ok"]
+    f = RNG.normal(size=n)
+    rows, nbl, nbr = t["rows"], t["nbl"], t["nbr"]
+    for ridx in range(nbl + nbr):
+        p = ridx if ridx < nbl else n - nbr + (ridx - nbl)
+        off, step, wd = int(rows[ridx, 0]), int(rows[ridx, 1]), int(rows[ridx, 2])
+        r = 0.0
+        for k in range(wd):
+            r = r + rows[ridx, 3 + k] * f[p + off + k * step]
+        seg = f[:16] if ridx < nbl else f[n - 16:]
+        assert _dense_row(t["drows"][ridx], seg) == r
+
+
+@pytest.mark.parametrize("kind,cls,af", [(P.PENTA10, 0, 0.0), (P.PADE6, 0, 0.0), (P.COMPACT8, 1, 0.49),
+                                         (P.COMMN5PADE, 1, 0.0)])
+def test_periodic_uniform_chunks_share_factors(kind, cls, af):
+    """Evenly chunked periodic lines: every chunk has chunk 0's LU factors and
+    spikes, and its interface rows are chunk 0's rotated (the fast kernels
+    rely on both)."""
+    t = tables(cls, kind, P.PERIODIC, 256, 1 / 256, af)
+    W2 = 2 * t["W"]
+    for c in range(1, t["P"]):
+        assert np.array_equal(t["lu"][c], t["lu"][0])
+        assert np.array_equal(t["sp"][c], t["sp"][0])
+        assert np.array_equal(t["G"][c], np.roll(t["G"][0], c * W2, axis=1))
 
 
 def test_plan_rejects_short_lines():
