@@ -13,7 +13,10 @@ bytes per launch over its mean launch duration (HIP events on the solver
 stream during the timed steps) against 8 TB/s.
 cpu_baseline = the CPU oracle (a plain C++ restatement of the reference
 algorithm, OpenMP over lines) timed on this host for one step of the same
-configuration at 128^3 (per-point cost does not depend on the grid size).
+configuration (256^3 by default).
+effective_bandwidth = the design-independent dataflow of the reference for
+this configuration (9592 B per point-step: every reference operator's distinct
+full-size FP64 inputs and outputs, KenRK4 + Vreman LOCAL) times the step rate.
 """
 import argparse
 import json
@@ -27,6 +30,10 @@ sys.path.insert(0, ROOT)
 METRIC = ("Mpt-steps/s at 256³ FP64 vs CPU; learned vs DSM LES speedup at "
           "≤2% KE error")
 HBM_PEAK_GBS = 8000.0
+# reference dataflow per point-step of KENRK4 + VREMAN LOCAL: stage subtotal
+# 1512 B x 5 stages, KenRK4 register updates 960 B, filter + CFL 312 B, Vreman
+# LOCAL 152 B x 5 stages
+B_REF_PER_PT_STEP = 9592.0
 
 CASE = dict(fd="PENTA10", filt="COMPACT8", alpha_f=0.49, rk="KENRK4", cfl=0.5,
             les="VREMAN", les_avg="LOCAL", mu_ref=1.0 / 5000.0)
@@ -59,7 +66,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256, help="grid points per direction")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-n", type=int, default=128)
+    ap.add_argument("--cpu-n", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -133,6 +140,10 @@ def main():
                      "avg_launch_ms": round(avg_ms, 4),
                      "share_of_step": round(dom["ms"] / kern_ms, 3)},
         "cpu_baseline": cpu,
+        "effective_bandwidth": {"bytes_per_pt_step": B_REF_PER_PT_STEP,
+                                "achieved": round(B_REF_PER_PT_STEP * value * 1e6 / 1e9, 1),
+                                "unit": "GB/s",
+                                "frac": round(B_REF_PER_PT_STEP * value * 1e6 / 1e9 / HBM_PEAK_GBS, 4)},
     }
     if cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 1)

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "../dsm.h"
 #include "../lineop_api.h"
 
 namespace mc3d {
@@ -155,6 +156,157 @@ void VremanSGS::getSGSViscosity(const double *const dU[3], const double *const d
                   "vreman");
     });
     doAveraging(mu_sgs, mu_sgs);
+}
+
+// ---------------------------------------------------------------------------
+// src/DSMSGS.hpp:34-111
+DSMSGS::DSMSGS(CurvilinearCSolver *c) {
+    cs = c;
+    isDSM = true;
+    musgsAvgType = c->opt.musgsAvgType;
+    useTaukk = c->opt.useTaukk;
+    coeffRangeFlag = c->opt.dumpCoeffRange;
+    const int bc3[3] = {c->opt.bcXType == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET,
+                        c->opt.bcYType == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET,
+                        c->opt.bcZType == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET};
+    std::unique_ptr<AbstractFilter> *fv[3] = {&filtX, &filtY, &filtZ};
+    for (int d = 0; d < 3; ++d) {
+        auto dir = (AbstractDerivatives::Direct)d;
+        // src/DSMSGS.hpp:57-67: 1 = Pade-type, 2 = explicit
+        if (c->opt.testFilterType == 1) fv[d]->reset(new CommN5PadeTestFilter(c->dom, bc3, dir, c->st));
+        else fv[d]->reset(new CommN5ExpTestFilter(c->dom, bc3, dir, c->st));
+        static_cast<LineFilter *>(fv[d]->get())->timer = &c->timer;
+    }
+    mu_sgs = c->alloc_field();
+    taukk = c->alloc_field();
+    k_sgs = c->alloc_field();
+    if (coeffRangeFlag) { C = c->alloc_field(); CI = c->alloc_field(); Prt = c->alloc_field(); }
+    for (auto &f : S) f = c->alloc_field();
+    Smag = c->alloc_field(); rho_hat = c->alloc_field(); Smag_hat = c->alloc_field();
+    rhoU_hat = c->alloc_field(); rhoV_hat = c->alloc_field(); rhoW_hat = c->alloc_field();
+    for (int k = 1; k <= 18; ++k) work[k] = c->alloc_field();
+    for (auto &f : gT) f = c->alloc_field();
+    for (auto &f : ftmp_) f = c->alloc_field();
+}
+
+void DSMSGS::filterQuantity(int n, const double *const *in, double *const *out) {
+    for (int b = 0; b < n; b += 16) {
+        const int m = n - b < 16 ? n - b : 16;
+        const double *tc[16];
+        for (int k = 0; k < m; ++k) tc[k] = ftmp_[k];
+        filtY->filterFields(m, in + b, ftmp_);
+        filtZ->filterFields(m, tc, ftmp_);
+        filtX->filterFields(m, tc, out + b);
+    }
+}
+
+void DSMSGS::average(int n, const double *const *in, double *const *out) {
+    if (musgsAvgType == Options::LOCAL) filterQuantity(n, in, out);
+    else
+        for (int k = 0; k < n; ++k) doAveraging(in[k], out[k]);
+}
+
+// src/DSMSGS.hpp:124-343
+void DSMSGS::calcMuSGSTaukk(const double *const dU[3], const double *const dV[3],
+                            const double *const dW[3], const double *rho, const double *rhoU,
+                            const double *rhoV, const double *rhoW) {
+    const long N = cs->N;
+    hipStream_t st = cs->st;
+    double **w = work;
+    DsmStrainArgs sa{};
+    for (int d = 0; d < 3; ++d) { sa.dU[d] = dU[d]; sa.dV[d] = dV[d]; sa.dW[d] = dW[d]; }
+    sa.m = cs->msh;
+    sa.rho = rho;
+    for (int k = 0; k < 6; ++k) { sa.S[k] = S[k]; sa.M2[k] = w[1 + k]; }
+    sa.Smag = Smag;
+    cs->timed("k_dsm_strain", 32.0, [&] { hip_check(launch_dsm_strain(sa, N, st), "dsm strain"); });
+    {   // M_ij (work7..12), rho_hat, S_ij hat (work1..6), Smag_hat
+        const double *in[14] = {w[1], w[2], w[3], w[4], w[5], w[6], rho, S[0], S[1], S[2], S[3], S[4], S[5], Smag};
+        double *out[14] = {w[7], w[8], w[9], w[10], w[11], w[12], rho_hat, w[1], w[2], w[3], w[4], w[5], w[6], Smag_hat};
+        filterQuantity(14, in, out);
+    }
+    DsmMLArgs ma{};
+    for (int k = 0; k < 6; ++k) { ma.M[k] = w[7 + k]; ma.S_hat[k] = w[1 + k]; ma.Lt[k] = w[13 + k]; }
+    ma.rho_hat = rho_hat; ma.Smag_hat = Smag_hat;
+    ma.rho = rho; ma.rhoU = rhoU; ma.rhoV = rhoV; ma.rhoW = rhoW;
+    cs->timed("k_dsm_ml", 30.0, [&] { hip_check(launch_dsm_ml(ma, N, st), "dsm M/L"); });
+    {   // L_ij (work1..6), momentum hats
+        const double *in[9] = {w[13], w[14], w[15], w[16], w[17], w[18], rhoU, rhoV, rhoW};
+        double *out[9] = {w[1], w[2], w[3], w[4], w[5], w[6], rhoU_hat, rhoV_hat, rhoW_hat};
+        filterQuantity(9, in, out);
+    }
+    DsmLArgs la{};
+    for (int k = 0; k < 6; ++k) la.L[k] = w[1 + k];
+    la.rhoU_hat = rhoU_hat; la.rhoV_hat = rhoV_hat; la.rhoW_hat = rhoW_hat; la.rho_hat = rho_hat;
+    la.rho = rho; la.Smag = Smag;
+    la.CIdenom = useTaukk ? w[13] : nullptr;
+    cs->timed("k_dsm_l", useTaukk ? 19.0 : 16.0, [&] { hip_check(launch_dsm_l(la, N, st), "dsm L"); });
+    if (useTaukk) {
+        const double *in[1] = {w[13]};
+        double *out[1] = {w[14]};
+        filterQuantity(1, in, out);  // alpha_hat
+    }
+    DsmContractArgs ca{};
+    for (int k = 0; k < 6; ++k) { ca.L[k] = w[1 + k]; ca.M[k] = w[7 + k]; }
+    ca.rho_hat = rho_hat; ca.Smag_hat = Smag_hat;
+    ca.alpha_hat = useTaukk ? w[14] : nullptr;
+    ca.CIdenom = w[13];
+    ca.Lkk = w[15]; ca.LijMij = w[17]; ca.MijMij = w[18];
+    cs->timed("k_dsm_contract", useTaukk ? 19.0 : 15.0,
+              [&] { hip_check(launch_dsm_contract(ca, N, st), "dsm contractions"); });
+    if (useTaukk) {
+        const double *in[4] = {w[15], w[13], w[17], w[18]};
+        double *out[4] = {w[7], w[8], w[9], w[10]};
+        average(4, in, out);
+    } else {
+        const double *in[2] = {w[17], w[18]};
+        double *out[2] = {w[9], w[10]};
+        average(2, in, out);
+    }
+    DsmMuArgs ua{};
+    ua.rho = rho; ua.Smag = Smag; ua.LijMij_avg = w[9]; ua.MijMij_avg = w[10];
+    ua.Lkk_avg = w[7]; ua.CIdenom_avg = w[8]; ua.J = cs->msh.J;
+    ua.mu_sgs = mu_sgs; ua.taukk = taukk; ua.C = C; ua.CI = CI;
+    ua.use_taukk = useTaukk ? 1 : 0;
+    ua.dx = cs->dom.dx; ua.dy = cs->dom.dy; ua.dz = cs->dom.dz;
+    cs->timed("k_dsm_mu", 6.0 + (useTaukk ? 2.0 : 0.0) + (C ? 3.0 : 0.0),
+              [&] { hip_check(launch_dsm_mu(ua, N, st), "dsm mu"); });
+}
+
+// src/DSMSGS.hpp:345-444
+void DSMSGS::calcKSGS(const double *const dT[3], const double *rho, const double *rhoU,
+                      const double *rhoV, const double *rhoW, const double *T) {
+    const long N = cs->N;
+    hipStream_t st = cs->st;
+    double **w = work;
+    DsmNKArgs na{};
+    for (int d = 0; d < 3; ++d) { na.dT[d] = dT[d]; na.N[d] = w[1 + d]; na.K[d] = w[7 + d]; na.gT[d] = gT[d]; }
+    na.m = cs->msh;
+    na.rho = rho; na.Smag = Smag; na.rhoU = rhoU; na.rhoV = rhoV; na.rhoW = rhoW; na.T = T;
+    na.rhoT = w[16];
+    cs->timed("k_dsm_nk", 28.0, [&] { hip_check(launch_dsm_nk(na, N, st), "dsm N/K"); });
+    {   // NRHS (work4..6), KRHS (work10..12), rhoT_hat (work17), gradT hats (work13..15)
+        const double *in[10] = {w[1], w[2], w[3], w[7], w[8], w[9], w[16], gT[0], gT[1], gT[2]};
+        double *out[10] = {w[4], w[5], w[6], w[10], w[11], w[12], w[17], w[13], w[14], w[15]};
+        filterQuantity(10, in, out);
+    }
+    DsmNNArgs nn{};
+    nn.rho_hat = rho_hat; nn.Smag_hat = Smag_hat;
+    for (int d = 0; d < 3; ++d) { nn.g_hat[d] = w[13 + d]; nn.NRHS[d] = w[4 + d]; nn.KRHS[d] = w[10 + d]; }
+    nn.rhoU_hat = rhoU_hat; nn.rhoV_hat = rhoV_hat; nn.rhoW_hat = rhoW_hat; nn.rhoT_hat = w[17];
+    nn.rho = rho;
+    nn.NiNi = w[4]; nn.NiKi = w[5];
+    cs->timed("k_dsm_nn", 18.0, [&] { hip_check(launch_dsm_nn(nn, N, st), "dsm NiNi/NiKi"); });
+    {
+        const double *in[2] = {w[4], w[5]};
+        double *out[2] = {w[6], w[10]};
+        average(2, in, out);
+    }
+    DsmKArgs ka{};
+    ka.NiNi_avg = w[6]; ka.NiKi_avg = w[10]; ka.rho = rho; ka.Smag = Smag; ka.C = C; ka.J = cs->msh.J;
+    ka.k_sgs = k_sgs; ka.Prt = Prt;
+    ka.cp = cs->ig.cp; ka.dx = cs->dom.dx; ka.dy = cs->dom.dy; ka.dz = cs->dom.dz;
+    cs->timed("k_dsm_ksgs", Prt ? 8.0 : 5.0, [&] { hip_check(launch_dsm_ksgs(ka, N, st), "dsm k_sgs"); });
 }
 
 // ---------------------------------------------------------------------------
@@ -316,8 +468,7 @@ CurvilinearCSolver::CurvilinearCSolver(const Options &o, hipStream_t s)
     for (auto *f : {filtX.get(), filtY.get(), filtZ.get()}) static_cast<LineFilter *>(f)->timer = &timer;
     // src/CurvilinearCSolver.hpp:232-242
     if (opt.lesModel == Options::VREMAN) les.reset(new VremanSGS(this));
-    else if (opt.lesModel == Options::DSM)
-        throw Error(MC3D_ECONFIG, "LESMODEL DSM is not available in this build");
+    else if (opt.lesModel == Options::DSM) les.reset(new DSMSGS(this));
     // src/MPI_3DCompact.cpp:164-176
     switch (opt.rkType) {
         case Options::TVDRK3: rk.reset(new TVDRK3(this)); break;
@@ -382,12 +533,30 @@ void CurvilinearCSolver::preSubStep() {
     AbstractDerivatives *dv[3] = {derivX.get(), derivY.get(), derivZ.get()};
     // gradients of U, V, W and T per direction (T does not depend on the SGS
     // model here; the DSM branch, which modifies T first, is not built)
-    for (int d = 0; d < 3; ++d) {
-        const double *in[4] = {pr.U, pr.V, pr.W, pr.T};
-        double *out[4] = {dU[d], dV[d], dW[d], dT[d]};
-        dv[d]->calc1stDerivFields(4, in, out, false);
+    if (les && les->isDSM) {
+        // DSM changes p, T, mu and sos before the temperature gradient (:568-594)
+        for (int d = 0; d < 3; ++d) {
+            const double *in[3] = {pr.U, pr.V, pr.W};
+            double *out[3] = {dU[d], dV[d], dW[d]};
+            dv[d]->calc1stDerivFields(3, in, out, false);
+        }
+        les->calcMuSGSTaukk(dU, dV, dW, s.q[0], s.q[1], s.q[2], s.q[3]);
+        timed("k_dsm_taukk_eos", 7.0,
+              [&] { hip_check(launch_dsm_taukk_eos(s.q[0], les->taukk, pr, ig, N, st), "dsm tau_kk"); });
+        for (int d = 0; d < 3; ++d) {
+            const double *in[1] = {pr.T};
+            double *out[1] = {dT[d]};
+            dv[d]->calc1stDerivFields(1, in, out, false);
+        }
+        les->calcKSGS(dT, s.q[0], s.q[1], s.q[2], s.q[3], pr.T);
+    } else {
+        for (int d = 0; d < 3; ++d) {
+            const double *in[4] = {pr.U, pr.V, pr.W, pr.T};
+            double *out[4] = {dU[d], dV[d], dW[d], dT[d]};
+            dv[d]->calc1stDerivFields(4, in, out, false);
+        }
+        if (les) les->getSGSViscosity(dU, dV, dW, s.q[0]);
     }
-    if (les) les->getSGSViscosity(dU, dV, dW, s.q[0]);
     FluxArgs fa{};
     for (int d = 0; d < 3; ++d) { fa.dU[d] = dU[d]; fa.dV[d] = dV[d]; fa.dW[d] = dW[d]; fa.dT[d] = dT[d]; }
     fa.m = msh;
@@ -395,11 +564,11 @@ void CurvilinearCSolver::preSubStep() {
     fa.mu_sgs = les ? les->mu_sgs : nullptr;
     fa.k_sgs = les ? les->k_sgs : nullptr;
     fa.rho = s.q[0]; fa.rhoU = s.q[1]; fa.rhoV = s.q[2]; fa.rhoW = s.q[3]; fa.rhoE = s.q[4];
-    fa.Ucurv = pr.Ucurv; fa.Vcurv = pr.Vcurv; fa.Wcurv = pr.Wcurv;
     for (int e = 0; e < 15; ++e) fa.pre[e] = pre[e];
     fa.les = (opt.lesModel == Options::NOMODEL) ? 0 : (opt.lesModel == Options::DSM ? 2 : 1);
     fa.Pr_t = Pr_t;
-    const double flux_fields = 35.0 + 15.0 + (fa.les ? 1.0 : 0.0) + (fa.les == 2 ? 1.0 : 0.0);
+    // read: 12 derivatives, 10 metrics, U V W p mu, 5 conserved (+ mu_sgs, k_sgs); written: 15
+    const double flux_fields = 32.0 + 15.0 + (fa.les ? 1.0 : 0.0) + (fa.les == 2 ? 1.0 : 0.0);
     timed("k_flux", flux_fields, [&] { hip_check(launch_flux(fa, ig, N, st), "flux"); });
     // divergence: xi of the _1 fluxes, then eta of _2 and zeta of _3 added in
     // (src/CurvilinearCSolver_Core.cpp:763-773 with the sums of :810,839,...)
@@ -419,9 +588,18 @@ void CurvilinearCSolver::stage_update(int op, double a, double b, double c) {
     ra.sc = sc;
     ra.op = op;
     ra.a = a; ra.b = b; ra.c = c;
-    // fields: div (5) + J + previous momentum k2 (3) read, k2 (5) written, plus the registers
+    // the non-conserved update of a stage that writes rk and is not followed by
+    // a filter rides along (its result does not depend on where it is computed)
+    const bool writes_rk = op != kTvd3 && op != kRk4Last && op != kLsLast;
+    ra.eos = (writes_rk && !rkLast && !opt.subStepFiltering) ? 1 : 0;
+    ra.pr = pr;
+    ra.g = ig;
+    eos_done_ = ra.eos != 0;
+    // fields: div (5) + J + previous momentum k2 (3) read, k2 (5) written, plus the
+    // registers, plus U V W p T mu sos written when the update rides along
     static const double reg[9] = {10, 15, 15, 15, 20, 10, 15, 20, 20};
-    timed("k_rhs_rk", 14.0 + reg[op], [&] { hip_check(launch_rhs_rk(ra, N, st), "stage update"); });
+    timed("k_rhs_rk", 14.0 + reg[op] + (ra.eos ? 7.0 : 0.0),
+          [&] { hip_check(launch_rhs_rk(ra, N, st), "stage update"); });
 }
 
 // src/CurvilinearCSolver.hpp:346-353
@@ -455,7 +633,12 @@ void CurvilinearCSolver::filterConservedData() {
 }
 
 void CurvilinearCSolver::updateNonConservedData() {
-    timed("k_eos", 24.0, [&] { hip_check(launch_eos(rkLast ? r1 : rk_, pr, msh, ig, N, st), "eos"); });
+    if (eos_done_) {  // already done by the stage kernel
+        eos_done_ = false;
+        return;
+    }
+    // read: 5 conserved; written: U V W p T mu sos
+    timed("k_eos", 12.0, [&] { hip_check(launch_eos(rkLast ? r1 : rk_, pr, ig, N, st), "eos"); });
 }
 
 // checkEnd without the kill file: src/CurvilinearCSolver_Core.cpp:1798-1819
@@ -484,12 +667,16 @@ double *CurvilinearCSolver::field_by_name(const std::string &n) {
     if (n == "T") return pr.T;
     if (n == "mu") return pr.mu;
     if (n == "sos") return pr.sos;
-    if (n == "Ucurv") return pr.Ucurv;
-    if (n == "Vcurv") return pr.Vcurv;
-    if (n == "Wcurv") return pr.Wcurv;
+    if (n == "Ucurv" || n == "Vcurv" || n == "Wcurv") {
+        hip_check(launch_contravariant(pr, msh, N, st), "contravariant velocities");
+        return n == "Ucurv" ? pr.Ucurv : n == "Vcurv" ? pr.Vcurv : pr.Wcurv;
+    }
     if (n == "mu_sgs") return les ? les->mu_sgs : nullptr;
     if (n == "k_sgs") return les ? les->k_sgs : nullptr;
     if (n == "taukk") return les ? les->taukk : nullptr;
+    if (n == "C") return les ? les->C : nullptr;
+    if (n == "CI") return les ? les->CI : nullptr;
+    if (n == "Prt") return les ? les->Prt : nullptr;
     static const char *met[10] = {"J", "J11", "J12", "J13", "J21", "J22", "J23", "J31", "J32", "J33"};
     for (int k = 0; k < 10; ++k)
         if (n == met[k]) return Jm[k];

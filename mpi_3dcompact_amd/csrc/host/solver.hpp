@@ -171,6 +171,7 @@ class AbstractSGS {
                           const double *rhoV, const double *rhoW, const double *T) = 0;
     // src/AbstractSGS.cpp:13-164
     void doAveraging(const double *phi, double *phiF);
+    bool isDSM = false;
 };
 
 class VremanSGS : public AbstractSGS {
@@ -182,6 +183,35 @@ class VremanSGS : public AbstractSGS {
                         const double *, const double *, const double *, const double *) override {}
     void calcKSGS(const double *const *, const double *, const double *, const double *,
                   const double *, const double *) override {}
+};
+
+// Dynamic Smagorinsky model, src/DSMSGS.hpp:13-451.  The test filter is
+// CommN5PadeTestFilter (DSMTESTFILTER 1) or CommN5ExpTestFilter (2); every
+// filterQuantity / doAveraging of the reference (Y, then Z, then X) is issued
+// as batched line-operator launches over all the fields it filters.
+class DSMSGS : public AbstractSGS {
+  public:
+    explicit DSMSGS(CurvilinearCSolver *cs);
+    void getSGSViscosity(const double *const *, const double *const *, const double *const *,
+                         const double *) override {}
+    void calcMuSGSTaukk(const double *const dU[3], const double *const dV[3],
+                        const double *const dW[3], const double *rho, const double *rhoU,
+                        const double *rhoV, const double *rhoW) override;
+    void calcKSGS(const double *const dT[3], const double *rho, const double *rhoU,
+                  const double *rhoV, const double *rhoW, const double *T) override;
+
+    bool useTaukk = true;
+    double *S[6] = {}, *Smag = nullptr, *rho_hat = nullptr, *Smag_hat = nullptr;
+    double *rhoU_hat = nullptr, *rhoV_hat = nullptr, *rhoW_hat = nullptr;
+    double *work[19] = {};  // work[1..18] as the reference numbers them
+    double *gT[3] = {};     // Cartesian temperature gradient (filtered in calcKSGS)
+
+  private:
+    double *ftmp_[16] = {};
+    // filterQuantity of n fields: Y into temporaries, Z in place, X into out
+    void filterQuantity(int n, const double *const *in, double *const *out);
+    // doAveraging of n fields (batched for LOCAL)
+    void average(int n, const double *const *in, double *const *out);
 };
 
 // ---------------------------------------------------------------------------
@@ -292,6 +322,7 @@ class CurvilinearCSolver {
     double *field_by_name(const std::string &name);
 
   private:
+    bool eos_done_ = false;  // the last stage kernel also updated the non-conserved data
     void *slab_ = nullptr;
     size_t slab_bytes_ = 0, slab_used_ = 0;
     std::vector<void *> extra_;

@@ -6,10 +6,11 @@
 // read with scalar loads.  Each lane keeps its chunk plus the stencil halo in
 // registers; LDS is only used for the interface exchange.
 //
-// Contiguous direction (x): the workgroup stages 64 lines into LDS in a
-// transposed [point][line] layout (row stride 65 doubles), so the coalesced
-// fill and the per-lane chunk reads are both free of bank conflicts; a wave
-// is again 64 lines x one chunk.
+// Contiguous direction (x): the workgroup stages 32 (periodic) or 64 lines
+// into LDS in a transposed [point][line] layout (row stride lines + 1) with
+// halo rows, so the coalesced fill and the per-lane chunk reads are both free
+// of bank conflicts; each thread issues batches of loads before its LDS
+// writes.
 //
 // Per chunk: explicit stencil (taps in the reference's summation order, no
 // FMA contraction, so the explicit stage is bitwise that of the reference),
@@ -24,8 +25,6 @@
 namespace mc3d {
 
 namespace {
-
-constexpr int kContigLines = 64;
 
 // explicit stage of the MM rows of one chunk from the window w[0..MM+2H)
 // (w[j] = phi[cs - H + j])
@@ -171,44 +170,79 @@ __global__ __launch_bounds__(1024) void lineop_strided(OpDev op, LaunchArgs a) {
     }
 }
 
+// lines per workgroup of the contiguous kernel: 32 for periodic lines (every
+// table read is chunk independent, two workgroups fit one CU), 64 otherwise
+// (a wave is then 64 lines x one chunk, so the end-row branches are uniform)
+template <bool PER>
+constexpr int contig_lines() { return PER ? 32 : 64; }
+
+constexpr int kBatch = 8;  // loads in flight per thread while staging (MM is a multiple)
+
+// (line, point) of element tid + k * nthr of a LINES x N tile, advanced
+// incrementally (nthr = qd * N + rd)
+struct TileWalk {
+    int ll, p;
+    __device__ __forceinline__ void next(int qd, int rd, int N) {
+        ll += qd;
+        p += rd;
+        if (p >= N) { p -= N; ++ll; }
+    }
+};
+
 template <int W, int H, int MM, bool PER>
 __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
     extern __shared__ double smem[];
-    constexpr int L = kContigLines;
+    constexpr int L = contig_lines<PER>();
     constexpr int LS = L + 1;  // row stride of the transposed tile
     const int N = op.N, P = op.P, NI = op.NI;
-    const int span = N + 2 * H;
-    double *tile = smem;                      // [span][LS]
-    double *red = smem + (size_t)span * LS;   // [L][NI+1]
+    double *tile = smem;                            // [N + 2H][LS], row e holds phi[e - H]
+    double *red = smem + (size_t)(N + 2 * H) * LS;  // [L][NI+1]
     const int tid = threadIdx.y * L + threadIdx.x;
-    const int nthr = L * P;
+    const int nthr = L * P;                         // L * N = nthr * MM elements
+    const int qd = nthr / N, rd = nthr - qd * N;
     const int q0 = blockIdx.x * L;
     const double *__restrict__ in = a.in[blockIdx.y];
     double *__restrict__ out = a.out[blockIdx.y];
-    const int wave = tid >> 6, lane = tid & 63, nwaves = (nthr + 63) >> 6;
-    // fill: one line per wave iteration, lanes along the line
-    for (int ll = wave; ll < L; ll += nwaves) {
-        const int q = q0 + ll;
-        if (q >= a.nlines) continue;
-        const double *row = in + (long)q * N;
-        for (int e = lane; e < span; e += 64) {
-            int p = e - H;
-            p = p < 0 ? p + N : (p >= N ? p - N : p);
-            if (PER || (e >= H && e < H + N)) tile[e * LS + ll] = row[p];
+    // fill: every thread issues batches of kBatch coalesced loads before the LDS writes
+    {
+        TileWalk t{tid / N, tid % N};
+#pragma unroll 1
+        for (int k0 = 0; k0 < MM; k0 += kBatch) {
+            double v[kBatch];
+            TileWalk u = t;
+#pragma unroll
+            for (int k = 0; k < kBatch; ++k) {
+                const int q = q0 + u.ll;
+                v[k] = q < a.nlines ? in[(long)q * N + u.p] : 0.0;
+                u.next(qd, rd, N);
+            }
+#pragma unroll
+            for (int k = 0; k < kBatch; ++k) {
+                tile[(t.p + H) * LS + t.ll] = v[k];
+                if (PER) {  // halo rows: periodic images
+                    if (t.p < H) tile[(t.p + H + N) * LS + t.ll] = v[k];
+                    if (t.p >= N - H) tile[(t.p + H - N) * LS + t.ll] = v[k];
+                }
+                t.next(qd, rd, N);
+            }
+        }
+        if (!PER) {  // halo rows: zeros (only end rows reach past the ends, with zero weight)
+            for (int e = tid; e < 2 * H * L; e += nthr) {
+                const int r = e / L, ll = e - r * L;
+                tile[(r < H ? r : N + r) * LS + ll] = 0.0;
+            }
         }
     }
     __syncthreads();
     const int l = threadIdx.x;
-    const int c = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int c = PER ? (int)threadIdx.y : __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int q = q0 + l;
     const bool live = q < a.nlines;
     const int cs = c * MM;
     double w[MM + 2 * H];
+    const double *trow = tile + (size_t)cs * LS + l;  // row cs holds phi[cs - H]
 #pragma unroll
-    for (int j = 0; j < MM + 2 * H; ++j) {
-        const int e = cs + j;  // ext index of phi[cs - H + j]
-        w[j] = (PER || (e >= H && e < H + N)) ? tile[e * LS + l] : 0.0;
-    }
+    for (int j = 0; j < MM + 2 * H; ++j) w[j] = trow[j * LS];
     double y[MM];
     chunk_rhs<H, MM, PER>(op, w, y, c == 0, c == P - 1);
     if (W > 0) {
@@ -220,6 +254,7 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
             myred[c * 2 * W + W + k] = y[MM - W + k];
         }
     }
+    // every lane has read its window and published its interface values
     __syncthreads();
     if (W > 0) {
         const double *myred = red + l * (NI + 1);
@@ -256,17 +291,31 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
     const bool pass = live && pass_line(op, 0, q, a.Nx, a.Ny);
     if (live && !pass) {
 #pragma unroll
-        for (int i = 0; i < MM; ++i) tile[(H + cs + i) * LS + l] = y[i];
+        for (int i = 0; i < MM; ++i) tile[(cs + H + i) * LS + l] = y[i];
     }
     __syncthreads();
-    for (int ll = wave; ll < L; ll += nwaves) {
-        const int qq = q0 + ll;
-        if (qq >= a.nlines) continue;
-        double *row = out + (long)qq * N;
-        for (int p = lane; p < N; p += 64) {
-            const double v = tile[(H + p) * LS + ll];
-            if (a.accumulate) row[p] += v;
-            else row[p] = v;
+    // store: batches of kBatch coalesced stores (and loads when accumulating)
+    {
+        TileWalk t{tid / N, tid % N};
+#pragma unroll 1
+        for (int k0 = 0; k0 < MM; k0 += kBatch) {
+            double v[kBatch];
+            TileWalk u = t;
+#pragma unroll
+            for (int k = 0; k < kBatch; ++k) {
+                v[k] = tile[(u.p + H) * LS + u.ll];
+                u.next(qd, rd, N);
+            }
+#pragma unroll
+            for (int k = 0; k < kBatch; ++k) {
+                const int qq = q0 + t.ll;
+                if (qq < a.nlines) {
+                    double *g = out + (long)qq * N + t.p;
+                    if (a.accumulate) *g += v[k];
+                    else *g = v[k];
+                }
+                t.next(qd, rd, N);
+            }
         }
     }
 }
@@ -274,10 +323,9 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
 template <int W, int H, int MM, bool PER>
 hipError_t launch_fast(const OpDev &op, const LaunchArgs &a, int nfields, hipStream_t st) {
     if (op.dir == 0) {
-        const int L = kContigLines;
+        constexpr int L = contig_lines<PER>();
         dim3 grid((a.nlines + L - 1) / L, nfields), block(L, op.P);
-        const size_t lds = sizeof(double) * ((size_t)(op.N + 2 * H) * (L + 1) + (size_t)L * (op.NI + 1));
-        hipLaunchKernelGGL((lineop_contig<W, H, MM, PER>), grid, block, lds, st, op, a);
+        hipLaunchKernelGGL((lineop_contig<W, H, MM, PER>), grid, block, lineop_fast_lds_bytes(op), st, op, a);
     } else {
         dim3 grid((a.nlines + 63) / 64, nfields), block(64, op.P);
         const size_t lds = sizeof(double) * (size_t)64 * (op.NI + 1);
@@ -301,9 +349,10 @@ hipError_t launch_fast_mm(const OpDev &op, const LaunchArgs &a, int nfields, hip
 }  // namespace
 
 size_t lineop_fast_lds_bytes(const OpDev &op) {
-    if (op.dir == 0)
-        return sizeof(double) * ((size_t)(op.N + 2 * op.H) * (kContigLines + 1) +
-                                 (size_t)kContigLines * (op.NI + 1));
+    if (op.dir == 0) {
+        const size_t L = op.periodic ? contig_lines<true>() : contig_lines<false>();
+        return sizeof(double) * ((size_t)(op.N + 2 * op.H) * (L + 1) + L * (op.NI + 1));
+    }
     return sizeof(double) * (size_t)64 * (op.NI + 1);
 }
 

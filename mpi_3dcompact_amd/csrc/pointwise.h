@@ -43,6 +43,9 @@ struct RkArgs {
     const StepScalars *sc;
     int op;
     double a, b, c;        // stage coefficients
+    int eos;               // also update the non-conserved data from the new rk state
+    Prim pr;
+    Gas g;
 };
 
 struct FluxArgs {
@@ -50,7 +53,6 @@ struct FluxArgs {
     Metrics m;
     const double *U, *V, *W, *p, *mu, *mu_sgs, *k_sgs;
     const double *rho, *rhoU, *rhoV, *rhoW, *rhoE;
-    const double *Ucurv, *Vcurv, *Wcurv;
     double *pre[15];
     int les;  // 0 none, 1 Vreman-type (k from mu_sgs/Pr_t), 2 DSM (k_sgs)
     double Pr_t;
@@ -59,7 +61,9 @@ struct FluxArgs {
 hipError_t launch_init_state(const double *rho0, const double *U0, const double *V0,
                              const double *W0, const double *p0, Cons r1, Prim pr, Metrics m,
                              Gas g, long n, hipStream_t st);
-hipError_t launch_eos(Cons in, Prim pr, Metrics m, Gas g, long n, hipStream_t st);
+hipError_t launch_eos(Cons in, Prim pr, Gas g, long n, hipStream_t st);
+// Ucurv, Vcurv, Wcurv from U, V, W and the metrics (on request only)
+hipError_t launch_contravariant(Prim pr, Metrics m, long n, hipStream_t st);
 // CFL: partial maxima into partial[nblocks], then a single-block finish that
 // sets dt (const CFL) or cfl (const dt) and advances time.
 int cfl_blocks(long n);
@@ -76,7 +80,7 @@ hipError_t launch_fill(double *p, double v, long n, hipStream_t st);
 // doAveraging XI1/XI2/XI3: average along direction dir (in place allowed)
 hipError_t launch_xi_average(const double *phi, double *phiF, int dir, int Nx, int Ny, int Nz,
                              hipStream_t st);
-// doAveraging GLOBAL: phiF = mean(phi); partial needs cfl_blocks(n) doubles
+// doAveraging GLOBAL: phiF = mean(phi); partial needs cfl_blocks(n) + 1 doubles
 hipError_t launch_global_mean(const double *phi, double *phiF, double *partial, long n,
                               hipStream_t st);
 

@@ -45,9 +45,6 @@ __global__ void k_init_state(const double *__restrict__ rho0, const double *__re
     r1.q[1][ip] = rho * U;
     r1.q[2][ip] = rho * V;
     r1.q[3][ip] = rho * W;
-    pr.Ucurv[ip] = m.J11[ip] * U + m.J12[ip] * V + m.J13[ip] * W;
-    pr.Vcurv[ip] = m.J21[ip] * U + m.J22[ip] * V + m.J23[ip] * W;
-    pr.Wcurv[ip] = m.J31[ip] * U + m.J32[ip] * V + m.J33[ip] * W;
     r1.q[4][ip] = p / (g.gamma - 1.0) + 0.5 * rho * (U * U + V * V + W * W);
     const double T = solve_T(g, rho, p);
     pr.T[ip] = T;
@@ -55,8 +52,20 @@ __global__ void k_init_state(const double *__restrict__ rho0, const double *__re
     pr.sos[ip] = solve_sos(g, rho, p);
 }
 
+// contravariant velocities, src/CurvilinearCSolver_Core.cpp:1190-1196; the flux
+// kernel forms them itself from the same expression, so they are only
+// materialised on request
+__global__ void k_contravariant(Prim pr, Metrics m, long n) {
+    const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= n) return;
+    const double U = pr.U[ip], V = pr.V[ip], W = pr.W[ip];
+    pr.Ucurv[ip] = m.J11[ip] * U + m.J12[ip] * V + m.J13[ip] * W;
+    pr.Vcurv[ip] = m.J21[ip] * U + m.J22[ip] * V + m.J23[ip] * W;
+    pr.Wcurv[ip] = m.J31[ip] * U + m.J32[ip] * V + m.J33[ip] * W;
+}
+
 // updateNonConservedData, src/CurvilinearCSolver_Core.cpp:1158-1198
-__global__ void k_eos(Cons c, Prim pr, Metrics m, Gas g, long n) {
+__global__ void k_eos(Cons c, Prim pr, Gas g, long n) {
     const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= n) return;
     const double rho = c.q[0][ip];
@@ -68,9 +77,6 @@ __global__ void k_eos(Cons c, Prim pr, Metrics m, Gas g, long n) {
     pr.T[ip] = T;
     pr.mu[ip] = solve_mu(g, T);
     pr.sos[ip] = solve_sos(g, rho, p);
-    pr.Ucurv[ip] = m.J11[ip] * U + m.J12[ip] * V + m.J13[ip] * W;
-    pr.Vcurv[ip] = m.J21[ip] * U + m.J22[ip] * V + m.J23[ip] * W;
-    pr.Wcurv[ip] = m.J31[ip] * U + m.J32[ip] * V + m.J33[ip] * W;
 }
 
 // calcDtFromCFL wave-speed maximum, src/CurvilinearCSolver_Core.cpp:197-207
@@ -166,7 +172,10 @@ __global__ __launch_bounds__(kBlock) void k_flux(FluxArgs a, Gas g, long n) {
     const double b2 = U * T21 + V * T22 + W * T23 + q2;
     const double b3 = U * T31 + V * T32 + W * T33 + q3;
     const double iJ = 1.0 / m.J[ip];
-    const double Uc = a.Ucurv[ip], Vc = a.Vcurv[ip], Wc = a.Wcurv[ip];
+    // contravariant velocities (as updateNonConservedData forms them, :1190-1196)
+    const double Uc = J11 * U + J12 * V + J13 * W;
+    const double Vc = J21 * U + J22 * V + J23 * W;
+    const double Wc = J31 * U + J32 * V + J33 * W;
     const double rho = a.rho[ip], rhoU = a.rhoU[ip], rhoV = a.rhoV[ip], rhoW = a.rhoW[ip];
     const double rhoE = a.rhoE[ip];
     a.pre[0][ip] = iJ * (rho * Uc);
@@ -195,36 +204,42 @@ __global__ __launch_bounds__(kBlock) void k_flux(FluxArgs a, Gas g, long n) {
     a.pre[4][ip] = iJ * (F1 - G1); a.pre[9][ip] = iJ * (F2 - G2); a.pre[14][ip] = iJ * (F3 - G3);
 }
 
-// one conserved variable: k2 and the register update of stage op
-__device__ __forceinline__ void rk_update(const RkArgs &a, long ip, int q, double k2) {
+// one conserved variable: k2 and the register update of stage op; returns the
+// value written to the stage register rk (stages that write it)
+__device__ __forceinline__ double rk_update(const RkArgs &a, long ip, int q, double k2) {
     double *r1 = a.r1.q[q], *rk = a.rk.q[q], *r2 = a.r2.q[q];
+    double n = 0.0;
     switch (a.op) {
-        case kTvd1: rk[ip] = r1[ip] + a.c * k2; break;                               // TVDRK3.hpp:62-80
-        case kTvd2: rk[ip] = a.a * r1[ip] + a.b * rk[ip] + a.c * k2; break;          // :82-102
+        case kTvd1: n = r1[ip] + a.c * k2; rk[ip] = n; break;                        // TVDRK3.hpp:62-80
+        case kTvd2: n = a.a * r1[ip] + a.b * rk[ip] + a.c * k2; rk[ip] = n; break;   // :82-102
         case kTvd3: r2[ip] = a.a * r1[ip] + a.b * rk[ip] + a.c * k2; break;          // :104-124
-        case kRk4First: r2[ip] = r1[ip] + a.a * k2; rk[ip] = r1[ip] + a.b * k2; break;  // RK4.hpp:53-70
-        case kRk4Mid: r2[ip] += a.a * k2; rk[ip] = r1[ip] + a.b * k2; break;            // :72-106
-        case kRk4Last: r2[ip] += a.a * k2; break;                                       // :108-118
-        case kLsFirst: rk[ip] = r1[ip] + a.b * k2; r1[ip] = k2; break;                  // KenRK4.hpp:58-74
-        case kLsMid: { double v = k2 + a.a * r1[ip]; r1[ip] = v; rk[ip] += a.b * v; break; }  // :76-127
+        case kRk4First: r2[ip] = r1[ip] + a.a * k2; n = r1[ip] + a.b * k2; rk[ip] = n; break;  // RK4.hpp:53-70
+        case kRk4Mid: r2[ip] += a.a * k2; n = r1[ip] + a.b * k2; rk[ip] = n; break;            // :72-106
+        case kRk4Last: r2[ip] += a.a * k2; break;                                               // :108-118
+        case kLsFirst: n = r1[ip] + a.b * k2; rk[ip] = n; r1[ip] = k2; break;                   // KenRK4.hpp:58-74
+        case kLsMid: { double v = k2 + a.a * r1[ip]; r1[ip] = v; n = rk[ip] + a.b * v; rk[ip] = n; break; }  // :76-127
         case kLsLast: { double v = k2 + a.a * r1[ip]; r1[ip] = v; r2[ip] = rk[ip] + a.b * v; break; }  // :128-145
     }
+    return n;
 }
 
 // solveContinuity .. solveEnergy (src/CurvilinearCSolver_Core.cpp:790-947) fused
 // with updateConservedData.  Momentum increments accumulate into the previous
-// k2 before scaling (:839-840,872-873,904-905).
+// k2 before scaling (:839-840,872-873,904-905).  With a.eos set (a stage that
+// writes rk and is not followed by a filter) updateNonConservedData of the new
+// stage state follows in the same pass, from the values just computed.
 __global__ __launch_bounds__(kBlock) void k_rhs_rk(RkArgs a, long n) {
     const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= n) return;
     const double J = a.J[ip];
     const double dtJ = a.sc->dt * J;
     const double z = (0.0 + 0.0) / J;
+    double nq[5];
     // continuity
     {
         const double k2 = dtJ * (-a.div[0][ip] + z);
         a.k2.q[0][ip] = k2;
-        rk_update(a, ip, 0, k2);
+        nq[0] = rk_update(a, ip, 0, k2);
     }
 #pragma unroll
     for (int q = 1; q < 4; ++q) {
@@ -232,13 +247,26 @@ __global__ __launch_bounds__(kBlock) void k_rhs_rk(RkArgs a, long n) {
         double k2 = kk[ip] + (-a.div[q][ip] + z);
         k2 = k2 * dtJ;
         kk[ip] = k2;
-        rk_update(a, ip, q, k2);
+        nq[q] = rk_update(a, ip, q, k2);
     }
     {
         double k2 = -a.div[4][ip] + z;
         k2 = k2 * dtJ;
         a.k2.q[4][ip] = k2;
-        rk_update(a, ip, 4, k2);
+        nq[4] = rk_update(a, ip, 4, k2);
+    }
+    if (a.eos) {  // as k_eos
+        const Gas &g = a.g;
+        const Prim &pr = a.pr;
+        const double rho = nq[0];
+        const double U = nq[1] / rho, V = nq[2] / rho, W = nq[3] / rho;
+        const double p = solve_p(g, rho, nq[4], U, V, W);
+        const double T = solve_T(g, rho, p);
+        pr.U[ip] = U; pr.V[ip] = V; pr.W[ip] = W;
+        pr.p[ip] = p;
+        pr.T[ip] = T;
+        pr.mu[ip] = solve_mu(g, T);
+        pr.sos[ip] = solve_sos(g, rho, p);
     }
 }
 
@@ -323,9 +351,9 @@ __global__ __launch_bounds__(kBlock) void k_sum_partial(const double *phi, doubl
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-__global__ __launch_bounds__(kBlock) void k_mean_fill(const double *partial, int nb, double *phiF, long n) {
+// mean of the block partial sums, written to *mean
+__global__ __launch_bounds__(kBlock) void k_mean_finish(const double *partial, int nb, double *mean, long n) {
     __shared__ double red[kBlock];
-    __shared__ double mean;
     double v = 0.0;
     for (int i = threadIdx.x; i < nb; i += blockDim.x) v += partial[i];
     red[threadIdx.x] = v;
@@ -334,10 +362,12 @@ __global__ __launch_bounds__(kBlock) void k_mean_fill(const double *partial, int
         if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) mean = red[0] / (double)n;
-    __syncthreads();
-    for (long ip = (long)blockIdx.x * blockDim.x + threadIdx.x; ip < n; ip += (long)gridDim.x * blockDim.x)
-        phiF[ip] = mean;
+    if (threadIdx.x == 0) *mean = red[0] / (double)n;
+}
+
+__global__ void k_fill_from(double *p, const double *v, long n) {
+    const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip < n) p[ip] = *v;
 }
 
 }  // namespace
@@ -350,8 +380,13 @@ hipError_t launch_init_state(const double *rho0, const double *U0, const double 
     return hipGetLastError();
 }
 
-hipError_t launch_eos(Cons in, Prim pr, Metrics m, Gas g, long n, hipStream_t st) {
-    hipLaunchKernelGGL(k_eos, grid_for(n), dim3(kBlock), 0, st, in, pr, m, g, n);
+hipError_t launch_eos(Cons in, Prim pr, Gas g, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_eos, grid_for(n), dim3(kBlock), 0, st, in, pr, g, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_contravariant(Prim pr, Metrics m, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_contravariant, grid_for(n), dim3(kBlock), 0, st, pr, m, n);
     return hipGetLastError();
 }
 
@@ -410,8 +445,11 @@ hipError_t launch_global_mean(const double *phi, double *phiF, double *partial, 
     hipLaunchKernelGGL(k_sum_partial, dim3(nb), dim3(kBlock), 0, st, phi, partial, n);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    // one block reduces the partials; the fill is grid-strided over the same single block
-    hipLaunchKernelGGL(k_mean_fill, dim3(1), dim3(kBlock), 0, st, partial, nb, phiF, n);
+    // the mean goes to partial[nb] (the buffer holds 2 * 2048 doubles)
+    hipLaunchKernelGGL(k_mean_finish, dim3(1), dim3(kBlock), 0, st, partial, nb, partial + nb, n);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_fill_from, grid_for(n), dim3(kBlock), 0, st, phiF, partial + nb, n);
     return hipGetLastError();
 }
 

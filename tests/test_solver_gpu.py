@@ -84,6 +84,39 @@ def test_scheme_combinations_32(gpu, case):
     check_pair(g, o, extra=extra)
 
 
+# LOCAL averaging (a three-direction test filter) gives large negative eddy
+# viscosities and tau_kk on a coarse field; at 32^3 it stays physical only for
+# a mild field (k0 = 2, M_t = 0.1), which the LOCAL cases use
+DSM_CASES = [
+    dict(dsm_filter=1, use_taukk=True, coeff_range=True, les_avg="LOCAL", k0=2.0, Mt=0.1, mu_ref=1e-3),
+    dict(dsm_filter=2, use_taukk=False, coeff_range=False, les_avg="LOCAL", k0=2.0, Mt=0.1, mu_ref=1e-3),
+    dict(dsm_filter=1, use_taukk=True, coeff_range=False, les_avg="GLOBAL", k0=3.0, Mt=0.3, mu_ref=1e-4),
+    dict(dsm_filter=2, use_taukk=True, coeff_range=True, les_avg="XI1_AVG", k0=3.0, Mt=0.3, mu_ref=1e-4),
+]
+
+
+@pytest.mark.parametrize("case", DSM_CASES,
+                         ids=["pade-taukk-coeffs-local", "explicit-notaukk-local", "pade-global",
+                              "explicit-xi1-coeffs"])
+def test_dsm_32(gpu, case):
+    """Dynamic Smagorinsky model (PENTA10, COMPACT8, KENRK4): the conserved
+    fields and the model outputs (mu_sgs, tau_kk, k_sgs and, when kept, C,
+    C_I, Pr_t) against the oracle after 6 steps.  Without tau_kk the initial
+    temperature is uniform, so k_sgs starts as a ratio of rounding noise and is
+    not compared (its effect, k_sgs times a zero gradient, is nil)."""
+    _say(f"case: 32^3 DSM {case}, 6 steps")
+    case = dict(case)
+    k0, Mt = case.pop("k0"), case.pop("Mt")
+    g, o = run_pair(32, 6, seed=5, k0=k0, Mt=Mt, fd="PENTA10", filt="COMPACT8", rk="KENRK4",
+                    les="DSM", **case)
+    extra = ("mu_sgs", "T", "p")
+    if case["use_taukk"]:
+        extra += ("taukk", "k_sgs")
+    if case["coeff_range"]:
+        extra += ("C", "CI", "Prt") if case["use_taukk"] else ("C",)
+    check_pair(g, o, extra=extra)
+
+
 def test_full_les_256_properties(gpu):
     """Full plain LES at 256^3 (PENTA10, COMPACT8, KENRK4, VREMAN, mu = 1/5000):
     conservation of mass, momentum and total energy and kinetic-energy decay
