@@ -127,6 +127,25 @@ int mc3d_solver_run(mc3d_solver *s);
  * rho2.. rhoE2, U V W p T mu sos Ucurv Vcurv Wcurv mu_sgs k_sgs taukk,
  * J J11 .. J33. */
 int mc3d_solver_get_field(mc3d_solver *s, const char *name, double *host_out);
+/* FROMRESTART ([RESTART] FILENAME): the conserved fields of that SolutionDump
+ * file become the initial state through primitive variables, as
+ * src/MPI_3DCompact.cpp:214-271 does (call after set_metrics, instead of
+ * set_initial).  With FROMRESTART or ONLYGRIDFROMRESTART, mc3d_solver_create
+ * takes the grid (and, for a restart, step and time) from the file header
+ * (src/MPI_3DCompact.cpp:80-113).  The momentum k2 registers are not in the
+ * file, so a restarted run starts them at zero. */
+int mc3d_solver_load_restart(mc3d_solver *s);
+/* Write a SolutionDump of the current state (dumpSolution,
+ * src/CurvilinearCSolver_Core.cpp:1333-1409) with host mesh coordinates
+ * x, y, z (NULL entries are written as zeros).  path NULL: "SolutionDump.<step>"
+ * in the output directory (or the working directory). */
+int mc3d_solver_write_dump(mc3d_solver *s, const char *path, const double *x, const double *y,
+                           const double *z);
+/* Enable the reference's automatic dumps: every DUMPSTEP steps and at the end
+ * of the run (src/CurvilinearCSolver.hpp:367-369, src/CurvilinearCSolver_Core.cpp:1822)
+ * into dir (NULL: working directory), with the given host mesh (copied). */
+int mc3d_solver_set_output(mc3d_solver *s, const char *dir, const double *x, const double *y,
+                           const double *z);
 int mc3d_solver_field_device(mc3d_solver *s, const char *name, double **dev_ptr);
 int mc3d_solver_scalars(mc3d_solver *s, double *time, double *dt, int *timestep);
 /* the solver's HIP stream (as void*), for timing with events */
@@ -142,6 +161,17 @@ void mc3d_solver_destroy(mc3d_solver *s);
  * Returns MC3D_EINVAL if buf is too small (the required size goes to *needed). */
 int mc3d_solver_timing(mc3d_solver *s, int enable);
 int mc3d_solver_timing_report(mc3d_solver *s, char *buf, int buflen, int *needed);
+
+/* ---------------------------------------------------------------------------
+ * SolutionDump files (host only, no GPU): five doubles {timeStep, time, Nx,
+ * Ny, Nz} then x, y, z, rho, rhoU, rhoV, rhoW, rhoE as Nx*Ny*Nz doubles each,
+ * x fastest, native byte order (src/CurvilinearCSolver_Core.cpp:1333-1409;
+ * read back by src/MPI_3DCompact.cpp:80-126,214-271).  fields[8] entries may
+ * be NULL (skipped on read, zeros on write). */
+int mc3d_dump_read_header(const char *path, int *timestep, double *time, int dims[3]);
+int mc3d_dump_read(const char *path, double *const fields[8]);
+int mc3d_dump_write(const char *path, int timestep, double time, const int dims[3],
+                    const double *const fields[8]);
 
 /* Diagnostic (no counterpart in the reference): builds, on the host only,
  * the partition tables an operator would upload, for verification of the

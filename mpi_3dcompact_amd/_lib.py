@@ -34,6 +34,10 @@ SIGNATURES = [
     ("mc3d_lineop_destroy", None, [_VP]),
     ("mc3d_lineop_info", ctypes.c_int, [_VP, _I, _I, _I]),
     ("mc3d_lineop_kernel", ctypes.c_char_p, [_VP]),
+    ("mc3d_dump_read_header", ctypes.c_int, [ctypes.c_char_p, _I, _D, _I]),
+    ("mc3d_dump_read", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_D)]),
+    ("mc3d_dump_write", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_double, _I,
+                                       ctypes.POINTER(_D)]),
     ("mc3d_plan_tables", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_double, ctypes.c_double, ctypes.c_int, _I, _D, _D,
                                         _D, _D, _D, _D, _I, _I]),

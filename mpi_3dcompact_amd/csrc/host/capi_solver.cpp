@@ -7,6 +7,7 @@
 #include <string>
 
 #include "../capi_common.h"
+#include "dump.hpp"
 #include "mc3d.h"
 #include "solver.hpp"
 
@@ -44,6 +45,19 @@ int mc3d_solver_create(const char *options_text, mc3d_solver **out) {
     std::string err;
     int rc = mc3d::Options::parse(options_text, o, err);
     if (rc) return mc3d::fail(rc, "options: " + err);
+    if (o.fromRestart || o.onlyGridFromRestart) {
+        // grid (and, for a restart, step and time) from the file header,
+        // src/MPI_3DCompact.cpp:80-113
+        rc = guarded([&] {
+            const mc3d::DumpHeader h = mc3d::read_dump_header(o.filename);
+            o.Nx = h.Nx; o.Ny = h.Ny; o.Nz = h.Nz;
+            if (o.fromRestart) { o.timeStep = h.timeStep; o.time = h.time; }
+            else { o.timeStep = 0; o.time = 0.0; }
+        });
+        if (rc) return rc;
+        if (o.Nx <= 10 || o.Ny <= 10 || o.Nz <= 10)
+            return mc3d::fail(MC3D_ECONFIG, "restart grid: every dimension must exceed 10");
+    }
     auto *s = new mc3d_solver();
     rc = guarded([&] {
         mc3d::hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "stream creation");
@@ -71,6 +85,60 @@ int mc3d_solver_set_initial(mc3d_solver *s, const double *rho0, const double *U0
     if (!s || !rho0 || !U0 || !V0 || !W0 || !p0) return mc3d::fail(MC3D_EINVAL, "null argument");
     if (!s->metrics) return mc3d::fail(MC3D_EINVAL, "set the metrics before the initial condition");
     return guarded([&] { s->cs->setInitialConditions_host(rho0, U0, V0, W0, p0); s->initial = true; });
+}
+
+int mc3d_solver_load_restart(mc3d_solver *s) {
+    if (!s) return mc3d::fail(MC3D_EINVAL, "null solver");
+    if (!s->metrics) return mc3d::fail(MC3D_EINVAL, "set the metrics before the initial state");
+    if (!s->cs->opt.fromRestart) return mc3d::fail(MC3D_ECONFIG, "RESTART.FROMRESTART is not set");
+    return guarded([&] {
+        s->cs->load_restart_host(s->cs->opt.filename);
+        s->initial = true;
+    });
+}
+
+int mc3d_solver_write_dump(mc3d_solver *s, const char *path, const double *x, const double *y,
+                           const double *z) {
+    if (!s) return mc3d::fail(MC3D_EINVAL, "null solver");
+    return guarded([&] {
+        const double *mesh[3] = {x, y, z};
+        s->cs->dumpSolution(path ? std::string(path) : std::string(), mesh);
+    });
+}
+
+int mc3d_solver_set_output(mc3d_solver *s, const char *dir, const double *x, const double *y,
+                           const double *z) {
+    if (!s) return mc3d::fail(MC3D_EINVAL, "null solver");
+    return guarded([&] {
+        const double *mesh[3] = {x, y, z};
+        s->cs->set_output(dir ? std::string(dir) : std::string(), mesh);
+    });
+}
+
+int mc3d_dump_read_header(const char *path, int *timestep, double *time, int dims[3]) {
+    if (!path) return mc3d::fail(MC3D_EINVAL, "null path");
+    return guarded([&] {
+        const mc3d::DumpHeader h = mc3d::read_dump_header(path);
+        if (timestep) *timestep = h.timeStep;
+        if (time) *time = h.time;
+        if (dims) { dims[0] = h.Nx; dims[1] = h.Ny; dims[2] = h.Nz; }
+    });
+}
+
+int mc3d_dump_read(const char *path, double *const fields[8]) {
+    if (!path || !fields) return mc3d::fail(MC3D_EINVAL, "null argument");
+    return guarded([&] { mc3d::read_dump_fields(path, mc3d::read_dump_header(path), fields); });
+}
+
+int mc3d_dump_write(const char *path, int timestep, double time, const int dims[3],
+                    const double *const fields[8]) {
+    if (!path || !dims || !fields) return mc3d::fail(MC3D_EINVAL, "null argument");
+    if (dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0) return mc3d::fail(MC3D_EINVAL, "dims must be positive");
+    return guarded([&] {
+        mc3d::DumpHeader h;
+        h.timeStep = timestep; h.time = time; h.Nx = dims[0]; h.Ny = dims[1]; h.Nz = dims[2];
+        mc3d::write_dump(path, h, fields);
+    });
 }
 
 int mc3d_solver_step(mc3d_solver *s, int nsteps) {

@@ -198,8 +198,9 @@ int Options::parse(const std::string &text, Options &o, std::string &err) {
     r.opt("LES.DSMUSETAUKK", o.useTaukk);
     r.opt("LES.DSMCOEFFRANGE", o.dumpCoeffRange);
     if (!r.ok) return MC3D_ECONFIG;
-    // src/Options.hpp:241 (checked after the sizes are known)
-    if (o.Nx <= 10 || o.Ny <= 10 || o.Nz <= 10) {
+    // src/Options.hpp:241 (checked after the sizes are known; with a restart
+    // they come from the file header and are checked when it is read)
+    if (!(o.fromRestart || o.onlyGridFromRestart) && (o.Nx <= 10 || o.Ny <= 10 || o.Nz <= 10)) {
         err = "number of grid points in any direction must be greater than 10";
         return MC3D_ECONFIG;
     }

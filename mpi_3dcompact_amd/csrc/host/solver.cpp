@@ -1,10 +1,14 @@
 // Host orchestration of the solver step (see solver.hpp).
 #include "solver.hpp"
 
+#include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 
 #include "../dsm.h"
+#include "dump.hpp"
 #include "../lineop_api.h"
 
 namespace mc3d {
@@ -517,6 +521,60 @@ void CurvilinearCSolver::setInitialConditions_host(const double *rho0, const dou
     hip_check(hipStreamSynchronize(st), "initial state");
 }
 
+// src/MPI_3DCompact.cpp:214-271: rho from the file, U = rhoU / rho (IdealGas
+// solveU), p from rhoE (solvep), then setInitialConditions
+void CurvilinearCSolver::load_restart_host(const std::string &path) {
+    const DumpHeader h = read_dump_header(path);
+    if (h.Nx != dom.gNx || h.Ny != dom.gNy || h.Nz != dom.gNz)
+        throw Error(MC3D_EINVAL, path + ": grid differs from the solver's");
+    std::vector<double> f[5];
+    for (auto &v : f) v.resize((size_t)N);
+    double *fields[kDumpFields] = {nullptr, nullptr, nullptr, f[0].data(), f[1].data(),
+                                   f[2].data(), f[3].data(), f[4].data()};
+    read_dump_fields(path, h, fields);
+    std::vector<double> U((size_t)N), V((size_t)N), W((size_t)N), p((size_t)N);
+    const double g1 = ig.gamma - 1.0;
+    for (long i = 0; i < N; ++i) {
+        const double rho = f[0][i];
+        U[i] = f[1][i] / rho;
+        V[i] = f[2][i] / rho;
+        W[i] = f[3][i] / rho;
+        p[i] = g1 * (f[4][i] - 0.5 * rho * (U[i] * U[i] + V[i] * V[i] + W[i] * W[i]));
+    }
+    setInitialConditions_host(f[0].data(), U.data(), V.data(), W.data(), p.data());
+}
+
+// src/CurvilinearCSolver_Core.cpp:1333-1409
+void CurvilinearCSolver::dumpSolution(const std::string &path, const double *const mesh[3]) {
+    StepScalars h;
+    hip_check(hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, st), "scalar read");
+    std::vector<double> f[5];
+    for (int q = 0; q < 5; ++q) {
+        f[q].resize((size_t)N);
+        hip_check(hipMemcpyAsync(f[q].data(), r1.q[q], sizeof(double) * N, hipMemcpyDeviceToHost, st),
+                  "dump read-back");
+    }
+    hip_check(hipStreamSynchronize(st), "dump read-back");
+    DumpHeader d;
+    d.timeStep = timeStep;
+    d.time = h.time;
+    d.Nx = dom.gNx; d.Ny = dom.gNy; d.Nz = dom.gNz;
+    const double *fields[kDumpFields] = {mesh[0], mesh[1], mesh[2], f[0].data(), f[1].data(),
+                                         f[2].data(), f[3].data(), f[4].data()};
+    std::string name = path;
+    if (name.empty()) name = (out_dir_.empty() ? std::string() : out_dir_ + "/") + dump_name(timeStep);
+    write_dump(name, d, fields);
+}
+
+void CurvilinearCSolver::set_output(const std::string &dir, const double *const mesh[3]) {
+    out_dir_ = dir;
+    for (int k = 0; k < 3; ++k) {
+        if (mesh[k]) mesh_[k].assign(mesh[k], mesh[k] + N);
+        else mesh_[k].clear();
+    }
+    dumps_ = true;
+}
+
 // calcDtFromCFL, src/CurvilinearCSolver_Core.cpp:189-237
 void CurvilinearCSolver::preStep() {
     timed("k_cfl", 7.0, [&] { hip_check(launch_cfl(pr, msh, dom.dx, dom.dy, dom.dz, cfl_partial, N, st), "cfl"); });
@@ -642,7 +700,71 @@ void CurvilinearCSolver::updateNonConservedData() {
 }
 
 // checkEnd without the kill file: src/CurvilinearCSolver_Core.cpp:1798-1819
+// src/Utils.cpp:415-431: the sequential fold std::min(phi, acc) from 1e6 (and
+// std::max from -1e6); for finite data the block-wise min/max gives the same
+void CurvilinearCSolver::getRange(const double *phi, const char *name) {
+    const int nb = 512;  // 2 * nb doubles fit the scratch buffer
+    hip_check(launch_minmax_partial(phi, cfl_partial, nb, N, st), "range");
+    std::vector<double> part(2 * nb);
+    hip_check(hipMemcpyAsync(part.data(), cfl_partial, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, st),
+              "range read-back");
+    hip_check(hipStreamSynchronize(st), "range read-back");
+    double lo = 1000000.0, hi = -1000000.0;
+    for (int b = 0; b < nb; ++b) {
+        lo = std::min(part[2 * b], lo);
+        hi = std::max(part[2 * b + 1], hi);
+    }
+    std::printf("   Range of %s: %g:%g\n", name, lo, hi);
+}
+
+// src/CurvilinearCSolver_Core.cpp:1244-1329
+void CurvilinearCSolver::checkSolution() {
+    if (timeStep % opt.checkStep != 0) return;
+    StepScalars h;
+    hip_check(hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, st), "scalar read");
+    hip_check(hipStreamSynchronize(st), "scalar read");
+    const auto now = std::chrono::steady_clock::now();
+    const double since = std::chrono::duration<double>(now - last_check_).count();
+    last_check_ = now;
+    std::printf("\n-------------------------------------------------\n");
+    std::printf(" Step = %d, time = %g, dt = %g\n", timeStep, h.time, h.dt);
+    std::printf("-------------------------------------------------\n");
+    std::printf("  Time since last timestep = %g\n", since);
+    hip_check(launch_contravariant(pr, msh, N, st), "contravariant velocities");
+    getRange(r1.q[0], "RHO");
+    getRange(pr.U, "U");
+    getRange(pr.V, "V");
+    getRange(pr.W, "W");
+    getRange(pr.Ucurv, "Ucurv");
+    getRange(pr.Vcurv, "Vcurv");
+    getRange(pr.Wcurv, "Wcurv");
+    getRange(pr.p, "P");
+    getRange(pr.T, "T");
+    getRange(pr.mu, "mu");
+    getRange(r1.q[4], "RHOE");
+    getRange(pr.sos, "SOS");
+    if (les) {
+        getRange(les->mu_sgs, "MU SGS");
+        if (les->isDSM) {
+            getRange(les->taukk, "TAUKK ");
+            getRange(les->k_sgs, "K SGS");
+            if (les->C) {
+                getRange(les->C, "LES C");
+                getRange(les->CI, "LES CI");
+                getRange(les->Prt, "LES Prt");
+            }
+        }
+    }
+    std::fflush(stdout);
+}
+
 void CurvilinearCSolver::postStep() {
+    checkSolution();
+    const double *mesh[3] = {mesh_[0].empty() ? nullptr : mesh_[0].data(),
+                             mesh_[1].empty() ? nullptr : mesh_[1].data(),
+                             mesh_[2].empty() ? nullptr : mesh_[2].data()};
+    // src/CurvilinearCSolver.hpp:367-369
+    if (dumps_ && timeStep % opt.dumpStep == 0) dumpSolution(std::string(), mesh);
     if (timeStep >= opt.maxTimeStep) endFlag = true;
     if (!endFlag) {
         StepScalars h;
@@ -650,6 +772,8 @@ void CurvilinearCSolver::postStep() {
         hip_check(hipStreamSynchronize(st), "scalar read");
         if (h.time >= opt.maxTime) endFlag = true;
     }
+    // checkEnd dumps the final state, src/CurvilinearCSolver_Core.cpp:1822
+    if (dumps_ && endFlag) dumpSolution(std::string(), mesh);
 }
 
 double *CurvilinearCSolver::field_by_name(const std::string &n) {

@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -303,6 +304,16 @@ class CurvilinearCSolver {
 
     double *alloc_field();
     void set_metrics_host(const double *const m[10]);
+    // FROMRESTART: conserved fields of a SolutionDump file converted to the
+    // primitive initial state (src/MPI_3DCompact.cpp:214-271), then set up
+    void load_restart_host(const std::string &path);
+    // SolutionDump of the current rho1..rhoE1 with the given mesh
+    // (src/CurvilinearCSolver_Core.cpp:1333-1409); empty path: the
+    // reference's name in the output directory
+    void dumpSolution(const std::string &path, const double *const mesh[3]);
+    // dumps every DUMPSTEP steps and at the end, as the reference's postStep
+    // does, into dir (mesh arrays copied; null entries are written as zeros)
+    void set_output(const std::string &dir, const double *const mesh[3]);
     void setInitialConditions_host(const double *rho0, const double *U0, const double *V0,
                                    const double *W0, const double *p0);
 
@@ -312,7 +323,9 @@ class CurvilinearCSolver {
     void solveEqnSet() {}         // k2 assembly is fused into the RK stage kernel
     void postSubStep() {}         // periodic boundaries: no boundary fix-ups
     void updateData();            // filter (last stage / substeps) + EOS
-    void postStep();              // end conditions
+    void postStep();              // checks, dumps, end conditions
+    void checkSolution();         // field ranges every CHECKSTEP steps
+    void getRange(const double *phi, const char *name);
     void stage_update(int op, double a, double b, double c);
     void filterConservedData();
     void updateNonConservedData();
@@ -322,6 +335,10 @@ class CurvilinearCSolver {
     double *field_by_name(const std::string &name);
 
   private:
+    std::chrono::steady_clock::time_point last_check_ = std::chrono::steady_clock::now();
+    bool dumps_ = false;
+    std::string out_dir_;
+    std::vector<double> mesh_[3];
     bool eos_done_ = false;  // the last stage kernel also updated the non-conserved data
     void *slab_ = nullptr;
     size_t slab_bytes_ = 0, slab_used_ = 0;

@@ -80,6 +80,8 @@ hipError_t launch_fill(double *p, double v, long n, hipStream_t st);
 // doAveraging XI1/XI2/XI3: average along direction dir (in place allowed)
 hipError_t launch_xi_average(const double *phi, double *phiF, int dir, int Nx, int Ny, int Nz,
                              hipStream_t st);
+// per-block {min, max} pairs of phi into partial[2 * nblocks] (getRange)
+hipError_t launch_minmax_partial(const double *phi, double *partial, int nblocks, long n, hipStream_t st);
 // doAveraging GLOBAL: phiF = mean(phi); partial needs cfl_blocks(n) + 1 doubles
 hipError_t launch_global_mean(const double *phi, double *phiF, double *partial, long n,
                               hipStream_t st);

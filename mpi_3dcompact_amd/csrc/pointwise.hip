@@ -351,6 +351,31 @@ __global__ __launch_bounds__(kBlock) void k_sum_partial(const double *phi, doubl
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// per-block minimum and maximum of phi into partial[2b], partial[2b+1]
+__global__ __launch_bounds__(kBlock) void k_minmax_partial(const double *phi, double *partial, long n) {
+    __shared__ double lo[kBlock], hi[kBlock];
+    double a = 1000000.0, b = -1000000.0;  // getRange's fold seeds, src/Utils.cpp:417,421
+    for (long ip = (long)blockIdx.x * blockDim.x + threadIdx.x; ip < n; ip += (long)gridDim.x * blockDim.x) {
+        const double v = phi[ip];
+        a = fmin(v, a);
+        b = fmax(v, b);
+    }
+    lo[threadIdx.x] = a;
+    hi[threadIdx.x] = b;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            lo[threadIdx.x] = fmin(lo[threadIdx.x], lo[threadIdx.x + s]);
+            hi[threadIdx.x] = fmax(hi[threadIdx.x], hi[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = lo[0];
+        partial[2 * blockIdx.x + 1] = hi[0];
+    }
+}
+
 // mean of the block partial sums, written to *mean
 __global__ __launch_bounds__(kBlock) void k_mean_finish(const double *partial, int nb, double *mean, long n) {
     __shared__ double red[kBlock];
@@ -436,6 +461,11 @@ hipError_t launch_xi_average(const double *phi, double *phiF, int dir, int Nx, i
     const int nl = dir == 0 ? Nx : dir == 1 ? Ny : Nz;
     const long nlines = (long)Nx * Ny * Nz / nl;
     hipLaunchKernelGGL(k_xi_average, grid_for(nlines), dim3(kBlock), 0, st, phi, phiF, dir, Nx, Ny, Nz);
+    return hipGetLastError();
+}
+
+hipError_t launch_minmax_partial(const double *phi, double *partial, int nblocks, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_minmax_partial, dim3(nblocks), dim3(kBlock), 0, st, phi, partial, n);
     return hipGetLastError();
 }
 

@@ -79,17 +79,23 @@ def identity_metrics(N):
 def options_text(N, fd="PADE6", filt="COMPACT10", alpha_f=0.49, rk="TVDRK3", cfl=0.5,
                  mu_ref=1.0 / 1600.0, les="NOMODEL", les_avg="NONE", max_steps=100000,
                  filter_step=1, substep_filtering=False, dsm_filter=1, use_taukk=True,
-                 coeff_range=False, ts_type="CONST_CFL", dt=None):
-    """MPI_3DCompact.in text for a periodic HIT box of N^3 points."""
+                 coeff_range=False, ts_type="CONST_CFL", dt=None, restart_file=None,
+                 dump_step=1000000):
+    """MPI_3DCompact.in text for a periodic HIT box of N^3 points (restart_file:
+    start from that SolutionDump, which also sets the grid)."""
     fds = fd if isinstance(fd, (list, tuple)) else (fd, fd, fd)
-    lines = [
-        "[RESTART]", "FROMRESTART = FALSE", "ONLYGRIDFROMRESTART = FALSE",
+    lines = ["[RESTART]"]
+    if restart_file:
+        lines += ["FROMRESTART = TRUE", "ONLYGRIDFROMRESTART = FALSE", f"FILENAME = {restart_file}"]
+    else:
+        lines += ["FROMRESTART = FALSE", "ONLYGRIDFROMRESTART = FALSE"]
+    lines += [
         "SPONGEAVGFROMRESTART = FALSE",
         "[TIMESTEPPING]", f"TSTYPE = {ts_type}",
         (f"CFL = {cfl!r}" if ts_type == "CONST_CFL" else f"DT = {dt!r}"),
         "MAXTIME = 1000000.0", f"MAXTIMESTEP = {max_steps}", f"FILTERSTEP = {filter_step}",
         f"SUBSTEPFILTERING = {'TRUE' if substep_filtering else 'FALSE'}",
-        "CHECKSTEP = 1000000", "DUMPSTEP = 1000000",
+        "CHECKSTEP = 1000000", f"DUMPSTEP = {dump_step}",
         "[STATS]", "STATSAVGTYPE = NONE", "VELOCITYSTATS = FALSE", "THERMOSTATS = FALSE",
         "[BC]", "BCXTYPE = PERIODIC_SOLVE", "BCYTYPE = PERIODIC_SOLVE", "BCZTYPE = PERIODIC_SOLVE",
         "BCX0 = PERIODIC", "BCX1 = PERIODIC", "BCY0 = PERIODIC", "BCY1 = PERIODIC",
@@ -100,7 +106,7 @@ def options_text(N, fd="PADE6", filt="COMPACT10", alpha_f=0.49, rk="TVDRK3", cfl
         for b in "XYZ":
             lines.append(f"PERIODICDISP{a}_{b} = {L if a == b else 0}")
     lines += [
-        "[DOMAIN]", f"NX = {N}", f"NY = {N}", f"NZ = {N}",
+        "[DOMAIN]"] + ([] if restart_file else [f"NX = {N}", f"NY = {N}", f"NZ = {N}"]) + [
         "[PENCILDECOMP]", "PROW = 0", "PCOL = 0",
         "[SOLVER]", f"ALPHAF = {alpha_f!r}", f"MU_REF = {mu_ref!r}", "USETIMING = FALSE",
         f"XFDTYPE = {fds[0]}", f"YFDTYPE = {fds[1]}", f"ZFDTYPE = {fds[2]}",

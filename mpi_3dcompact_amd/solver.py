@@ -13,6 +13,9 @@ _SIGS = [
     ("mc3d_solver_set_metrics", ctypes.c_int, [_VP, ctypes.POINTER(_D)]),
     ("mc3d_solver_set_initial", ctypes.c_int, [_VP, _D, _D, _D, _D, _D]),
     ("mc3d_solver_step", ctypes.c_int, [_VP, ctypes.c_int]),
+    ("mc3d_solver_load_restart", ctypes.c_int, [_VP]),
+    ("mc3d_solver_write_dump", ctypes.c_int, [_VP, ctypes.c_char_p, _D, _D, _D]),
+    ("mc3d_solver_set_output", ctypes.c_int, [_VP, ctypes.c_char_p, _D, _D, _D]),
     ("mc3d_solver_run", ctypes.c_int, [_VP]),
     ("mc3d_solver_get_field", ctypes.c_int, [_VP, ctypes.c_char_p, _D]),
     ("mc3d_solver_field_device", ctypes.c_int, [_VP, ctypes.c_char_p, ctypes.POINTER(_VP)]),
@@ -60,6 +63,26 @@ class Solver:
     def set_initial(self, ic):
         a = [np.ascontiguousarray(ic[k], dtype=np.float64) for k in ("rho", "U", "V", "W", "p")]
         check(self._L.mc3d_solver_set_initial(self._h, *[_p(x) for x in a]))
+
+    def load_restart(self):
+        """FROMRESTART: initial state from the options file's RESTART.FILENAME."""
+        check(self._L.mc3d_solver_load_restart(self._h))
+
+    def _mesh(self, mesh):
+        if mesh is None:
+            return [None, None, None], []
+        arrs = [np.ascontiguousarray(m, dtype=np.float64) for m in mesh]
+        return [_p(a) for a in arrs], arrs
+
+    def write_dump(self, path=None, mesh=None):
+        """SolutionDump of the current state; mesh = (x, y, z) host arrays or None."""
+        ptrs, keep = self._mesh(mesh)
+        check(self._L.mc3d_solver_write_dump(self._h, path.encode() if path else None, *ptrs))
+
+    def set_output(self, directory=None, mesh=None):
+        """Dump every DUMPSTEP steps and at the end of run() into directory."""
+        ptrs, keep = self._mesh(mesh)
+        check(self._L.mc3d_solver_set_output(self._h, directory.encode() if directory else None, *ptrs))
 
     def step(self, n=1):
         check(self._L.mc3d_solver_step(self._h, n))

@@ -3,6 +3,7 @@
 Tolerance: relative L2 error ||gpu - oracle|| / ||oracle|| <= 1e-9 on every
 conserved field after the stated number of steps (FP64).
 """
+import os
 import sys
 
 import numpy as np
@@ -46,11 +47,13 @@ def run_pair(N, steps, seed=0, k0=4.0, Mt=0.3, **case):
     return g, o
 
 
-def check_pair(g, o, tol=TOL, extra=()):
-    errs = {n: rel_l2(g.field(n), o.field(n)) for n in CONS + tuple(extra)}
+def check_pair(g, o, tol=TOL, extra=(), loose=None):
+    """extra: more fields at tol; loose: {field: tolerance} for ill-conditioned outputs."""
+    loose = loose or {}
+    errs = {n: rel_l2(g.field(n), o.field(n)) for n in CONS + tuple(extra) + tuple(loose)}
     _say("  rel L2:", {k: f"{v:.2e}" for k, v in errs.items()})
     for n, e in errs.items():
-        assert e <= tol, (n, e)
+        assert e <= loose.get(n, tol), (n, e)
     t, dt, ts = g.scalars()
     assert ts == o.timestep
     assert abs(t - o.time) <= 1e-12 * max(1.0, abs(o.time))
@@ -84,14 +87,27 @@ def test_scheme_combinations_32(gpu, case):
     check_pair(g, o, extra=extra)
 
 
-# LOCAL averaging (a three-direction test filter) gives large negative eddy
-# viscosities and tau_kk on a coarse field; at 32^3 it stays physical only for
-# a mild field (k0 = 2, M_t = 0.1), which the LOCAL cases use
+# Conditioning of the DSM, measured on the oracle itself by perturbing the
+# initial velocity by 1e-15 (relative): with GLOBAL or XI averaging every
+# output moves by <= 3e-12 after 6 steps at 32^3 (k0 = 3, M_t = 0.3).  LOCAL
+# averaging (a three-direction test filter) divides by near-zero M_ij M_ij and
+# N_i N_i: on that field the conserved variables move by 3e-5 and mu_sgs by
+# O(1) after 6 steps, so its parity is checked on a gentle field (k0 = 1.5,
+# M_t = 0.05, 3 steps: conserved <= 2e-14, mu_sgs and tau_kk <= 1e-11), where
+# k_sgs, C, C_I and Pr_t still move by 1e-5 and are not compared.  Pr_t =
+# -C Delta^2 <N_i N_i> / <N_i K_i> also divides by a small average under XI
+# averaging: on the XI1 case it moves by up to 4.7e-9 under the same 1e-15
+# perturbation, so it is compared at 1e-7 there.
 DSM_CASES = [
-    dict(dsm_filter=1, use_taukk=True, coeff_range=True, les_avg="LOCAL", k0=2.0, Mt=0.1, mu_ref=1e-3),
-    dict(dsm_filter=2, use_taukk=False, coeff_range=False, les_avg="LOCAL", k0=2.0, Mt=0.1, mu_ref=1e-3),
-    dict(dsm_filter=1, use_taukk=True, coeff_range=False, les_avg="GLOBAL", k0=3.0, Mt=0.3, mu_ref=1e-4),
-    dict(dsm_filter=2, use_taukk=True, coeff_range=True, les_avg="XI1_AVG", k0=3.0, Mt=0.3, mu_ref=1e-4),
+    dict(dsm_filter=1, use_taukk=True, coeff_range=True, les_avg="LOCAL", k0=1.5, Mt=0.05,
+         mu_ref=1e-3, steps=3, fields=("mu_sgs", "taukk", "T", "p")),
+    dict(dsm_filter=2, use_taukk=False, coeff_range=False, les_avg="LOCAL", k0=1.5, Mt=0.05,
+         mu_ref=1e-3, steps=3, fields=("mu_sgs", "T", "p")),
+    dict(dsm_filter=1, use_taukk=True, coeff_range=False, les_avg="GLOBAL", k0=3.0, Mt=0.3,
+         mu_ref=1e-4, steps=6, fields=("mu_sgs", "taukk", "k_sgs", "T", "p")),
+    dict(dsm_filter=2, use_taukk=True, coeff_range=True, les_avg="XI1_AVG", k0=3.0, Mt=0.3,
+         mu_ref=1e-4, steps=6, fields=("mu_sgs", "taukk", "k_sgs", "C", "CI", "T", "p"),
+         loose={"Prt": 1e-7}),
 ]
 
 
@@ -99,22 +115,55 @@ DSM_CASES = [
                          ids=["pade-taukk-coeffs-local", "explicit-notaukk-local", "pade-global",
                               "explicit-xi1-coeffs"])
 def test_dsm_32(gpu, case):
-    """Dynamic Smagorinsky model (PENTA10, COMPACT8, KENRK4): the conserved
-    fields and the model outputs (mu_sgs, tau_kk, k_sgs and, when kept, C,
-    C_I, Pr_t) against the oracle after 6 steps.  Without tau_kk the initial
-    temperature is uniform, so k_sgs starts as a ratio of rounding noise and is
-    not compared (its effect, k_sgs times a zero gradient, is nil)."""
-    _say(f"case: 32^3 DSM {case}, 6 steps")
+    """Dynamic Smagorinsky model (PENTA10, COMPACT8, KENRK4) at 32^3: the
+    conserved fields and the well-conditioned model outputs against the oracle."""
     case = dict(case)
-    k0, Mt = case.pop("k0"), case.pop("Mt")
-    g, o = run_pair(32, 6, seed=5, k0=k0, Mt=Mt, fd="PENTA10", filt="COMPACT8", rk="KENRK4",
+    k0, Mt, steps, fields = case.pop("k0"), case.pop("Mt"), case.pop("steps"), case.pop("fields")
+    loose = case.pop("loose", None)
+    _say(f"case: 32^3 DSM {case}, k0 {k0}, M_t {Mt}, {steps} steps")
+    g, o = run_pair(32, steps, seed=5, k0=k0, Mt=Mt, fd="PENTA10", filt="COMPACT8", rk="KENRK4",
                     les="DSM", **case)
-    extra = ("mu_sgs", "T", "p")
-    if case["use_taukk"]:
-        extra += ("taukk", "k_sgs")
-    if case["coeff_range"]:
-        extra += ("C", "CI", "Prt") if case["use_taukk"] else ("C",)
-    check_pair(g, o, extra=extra)
+    check_pair(g, o, extra=fields, loose=loose)
+
+
+def test_dump_and_restart(gpu, tmp_path):
+    """A SolutionDump written after 3 steps restarts a new solver: grid, step
+    and time come from the file, the conserved state survives the primitive
+    round trip of the reference's restart to rounding, and the automatic dumps
+    land every DUMPSTEP steps and at the end of run()."""
+    from mpi_3dcompact_amd import dump
+    from mpi_3dcompact_amd.solver import Solver
+    N = 24
+    case = dict(fd="PADE6", filt="COMPACT10", rk="TVDRK3", mu_ref=1e-3)
+    ic = hit.hit_initial_condition(N, k0=3.0, Mt=0.3, seed=2)
+    met = hit.identity_metrics(N)
+    x = np.broadcast_to(2 * np.pi * np.arange(N) / N, (N, N, N))
+    a = Solver(hit.options_text(N, **case), (N, N, N))
+    a.set_metrics(met)
+    a.set_initial(ic)
+    a.step(3)
+    path = str(tmp_path / "SolutionDump.3")
+    a.write_dump(path, mesh=(x, x.transpose(0, 2, 1), x.transpose(2, 1, 0)))
+    t, dt, ts = a.scalars()
+    d = dump.read(path)
+    assert d["timestep"] == 3 and d["time"] == t and d["dims"] == (N, N, N)
+    assert np.array_equal(d["x"], x)
+    for n in CONS:
+        assert np.array_equal(d[n], a.field(n))
+    b = Solver(hit.options_text(N, restart_file=path, max_steps=7, dump_step=2, **case), (N, N, N))
+    b.set_metrics(met)
+    b.load_restart()
+    t_b, _, ts_b = b.scalars()
+    assert ts_b == 3 and t_b == t
+    for n in CONS:
+        assert rel_l2(b.field(n), a.field(n)) <= 1e-14, n
+    out = tmp_path / "out"
+    out.mkdir()
+    b.set_output(str(out))
+    b.run()
+    assert b.scalars()[2] == 7
+    assert sorted(os.listdir(out)) == ["SolutionDump.4", "SolutionDump.6", "SolutionDump.7"]
+    assert np.array_equal(dump.read(str(out / "SolutionDump.7"))["rho"], b.field("rho"))
 
 
 def test_full_les_256_properties(gpu):
