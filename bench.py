@@ -10,7 +10,10 @@ M_t = 0.3).  A "step" is one full time step (5 RK stages + filter + CFL).
 value = grid-point-steps per second (Mpt-steps/s), inputs resident in HBM.
 roofline = the kernel with the largest share of step time, its algorithmic
 bytes per launch over its mean launch duration (HIP events on the solver
-stream during the timed steps) against 8 TB/s.
+stream during the timed steps) against 8 TB/s; traffic = that kernel's HBM
+bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this
+workload (profiles/les256_traffic.json, counters scaled by their reading for
+a copy of known size), when the profiled kernel is the dominant one.
 cpu_baseline = the CPU oracle (a plain C++ restatement of the reference
 algorithm, OpenMP over lines) timed on this host for one step of the same
 configuration (256^3 by default).
@@ -41,6 +44,19 @@ CASE = dict(fd="PENTA10", filt="COMPACT8", alpha_f=0.49, rk="KENRK4", cfl=0.5,
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def measured_traffic(kernel, n):
+    """HBM bytes per launch of kernel from the committed counter profile, or None."""
+    path = os.path.join(ROOT, "profiles", "les256_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("kernel") != kernel or n != 256:
+        return None
+    return t.get("traffic_bytes_per_launch")
 
 
 def cpu_baseline(n_cpu):
@@ -136,7 +152,7 @@ def main():
                    "grid": [N, N, N], "points": N ** 3, "rk_stages_per_step": 5},
         "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "bytes_per_launch": bytes_per_launch,
+                     "traffic": measured_traffic(dom["kernel"], N), "bytes_per_launch": bytes_per_launch,
                      "avg_launch_ms": round(avg_ms, 4),
                      "share_of_step": round(dom["ms"] / kern_ms, 3)},
         "cpu_baseline": cpu,
