@@ -114,6 +114,27 @@ int mc3d_solver_create(const char *options_text, mc3d_solver **out);
 /* metrics[10] = {J, J11, J12, J13, J21, J22, J23, J31, J32, J33} (host arrays),
  * the fields produced by src/AbstractSingleBlockMesh.cpp:23-1439. */
 int mc3d_solver_set_metrics(mc3d_solver *s, const double *const metrics[10]);
+/* Metric terms of a single-block mesh on the GPU, without a solver:
+ * AbstractSingleBlockMesh::solveForJacobians (src/AbstractSingleBlockMesh.cpp:23-1309).
+ * fd[d] and bc[d] select the derivative scheme and boundary of direction d;
+ * trans[3*d+e] is component e of the coordinate jump across periodic
+ * direction d (PERIODICDISP, :2019-2085; NULL: none).  x, y, z (inputs) and
+ * out[10] = {J, J11 .. J33} are device fields of dims[0]*dims[1]*dims[2]
+ * doubles, x fastest.  Computational coordinates run over [0, 1].  Work is
+ * queued on stream (a hipStream_t, NULL: default) and finished on return;
+ * scratch of 39 fields is allocated for the call. */
+int mc3d_mesh_metrics(const int fd[3], const int bc[3], const int dims[3], const double trans[9],
+                      const double *x, const double *y, const double *z, double *const out[10],
+                      void *stream);
+/* Form the ten metric fields on the GPU from host mesh coordinates x, y, z
+ * (x fastest), as AbstractSingleBlockMesh::solveForJacobians does
+ * (src/AbstractSingleBlockMesh.cpp:23-1309): conservative form, periodic
+ * directions through the transformed-periodic derivative with the wrap values
+ * shifted by [BC] PERIODICDISP<d>_<e> (handlePeriodicStuff, :2019-2085).  The
+ * computational coordinates run over [0, 1] in each direction.  Used instead of
+ * mc3d_solver_set_metrics; CD2 on a periodic direction is MC3D_ECONFIG (the
+ * reference has no such path either, :94-97). */
+int mc3d_solver_set_mesh(mc3d_solver *s, const double *x, const double *y, const double *z);
 /* primitive initial condition; performs setInitialConditions
  * (src/CurvilinearCSolver_BC.cpp:3-63). */
 int mc3d_solver_set_initial(mc3d_solver *s, const double *rho0, const double *U0,
@@ -161,6 +182,70 @@ void mc3d_solver_destroy(mc3d_solver *s);
  * Returns MC3D_EINVAL if buf is too small (the required size goes to *needed). */
 int mc3d_solver_timing(mc3d_solver *s, int enable);
 int mc3d_solver_timing_report(mc3d_solver *s, char *buf, int buflen, int *needed);
+
+/* ---------------------------------------------------------------------------
+ * Learned SGS closure (no counterpart in the reference: it replaces the
+ * dynamic procedure of DSMSGS, src/DSMSGS.hpp:124-444, behind the same
+ * AbstractSGS interface, src/AbstractSGS.hpp:9-42, selected by
+ * [LES] LESMODEL = LEARNED with CLOSUREFILE = <network file> and
+ * CLOSUREFALLBACK = TRUE|FALSE).
+ *
+ * Per point, with the Cartesian velocity gradient A = grad u:
+ *   features f = (theta/a, (s2 - w2)/a^2, s3/a^3, sw2/a^3, s2w2/a^4, log a,
+ *                 log Delta), theta = tr A, S and W the symmetric and
+ *                 antisymmetric parts, s2 = S:S, w2 = W:W, s3 = tr S^3,
+ *                 sw2 = tr S W^2, s2w2 = tr S^2 W^2, a^2 = s2 + w2,
+ *                 Delta = (dxi deta dzeta / J)^(1/3) (src/DSMSGS.hpp:332-333);
+ *   network  7 -> 64 -> 64 -> 64 -> 2, tanh-form GELU, on standardised inputs
+ *            (x - in_mean) / in_std, BF16 matrix products with FP32 accumulation;
+ *   outputs  C = y0 out_std0 + out_mean0, D = y1 out_std1 + out_mean1 (D = C / Pr_t);
+ *            mu_sgs = rho C Delta^2 |S|, k_sgs = cp D rho Delta^2 |S|
+ *            (the DSM's own forms, src/DSMSGS.hpp:319,406-408,440).
+ * A point whose raw features leave [lo, hi] takes the DSM's mu_sgs and k_sgs
+ * on that stage when the fallback is on.
+ *
+ * Parameters: 8962 FP32 values, W1[64][7] b1[64] W2[64][64] b2[64] W3[64][64]
+ * b3[64] W4[2][64] b4[2].  Training data: X[n][7] standardised inputs and
+ * T[n][2] standardised targets, FP32 device arrays.
+ * ------------------------------------------------------------------------- */
+typedef struct mc3d_closure mc3d_closure;
+
+/* weights U(-s, s), s = sqrt(3 / fan_in), zero biases; identity scaling */
+int mc3d_closure_create(unsigned long long seed, mc3d_closure **out);
+void mc3d_closure_destroy(mc3d_closure *c);
+int mc3d_closure_set_params(mc3d_closure *c, const float *params);   /* host[8962]; resets Adam */
+int mc3d_closure_get_params(const mc3d_closure *c, float *params);   /* host[8962] */
+int mc3d_closure_set_scaling(mc3d_closure *c, const float in_mean[7], const float in_std[7],
+                             const float out_mean[2], const float out_std[2], const float lo[7],
+                             const float hi[7]);
+int mc3d_closure_get_scaling(const mc3d_closure *c, float in_mean[7], float in_std[7],
+                             float out_mean[2], float out_std[2], float lo[7], float hi[7]);
+/* gradient of L = sum (y - T)^2 / (2 n) over all n samples into grad
+ * (device[8962]); *loss = L.  Synchronous. */
+int mc3d_closure_gradient(mc3d_closure *c, const float *X, const float *T, long n, float *grad,
+                          double *loss, void *stream);
+/* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with the given gradient */
+int mc3d_closure_adam_step(mc3d_closure *c, const float *grad, float lr, void *stream);
+/* shuffled minibatch Adam; epoch_losses[e] (host, may be NULL) = mean loss of epoch e */
+int mc3d_closure_train(mc3d_closure *c, const float *X, const float *T, long n, int epochs, int batch,
+                       float lr, unsigned long long seed, double *epoch_losses, void *stream);
+/* Y[n][2] = network(X[n][7]) on standardised values (asynchronous) */
+int mc3d_closure_infer(const mc3d_closure *c, const float *X, float *Y, long n, void *stream);
+int mc3d_closure_save(const mc3d_closure *c, const char *path);
+int mc3d_closure_load(mc3d_closure *c, const char *path);
+/* copy the network and its scaling into a LESMODEL LEARNED solver */
+int mc3d_solver_set_closure(mc3d_solver *s, const mc3d_closure *c);
+/* training pairs from a LESMODEL DSM solver with DSMCOEFFRANGE TRUE: at the
+ * first stage of the next step, for i < count and p = offset + i * stride,
+ * X[i][7] = raw features, T[i] = (C, C / Pr_t) of the DSM at p (device FP32) */
+int mc3d_solver_sample_closure(mc3d_solver *s, float *X, float *T, long count, long offset,
+                               long stride);
+/* derivatives and SGS fields (mu_sgs, k_sgs) of the current state without
+ * advancing it (NOMODEL, VREMAN, LEARNED; the DSM modifies the pressure) */
+int mc3d_solver_eval_sgs(mc3d_solver *s);
+/* closure evaluations, stages that fell back to the DSM, and points that did */
+int mc3d_solver_closure_stats(mc3d_solver *s, long *evaluations, long *fallback_stages,
+                              long *fallback_points);
 
 /* ---------------------------------------------------------------------------
  * SolutionDump files (host only, no GPU): five doubles {timeStep, time, Nx,

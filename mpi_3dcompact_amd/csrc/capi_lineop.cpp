@@ -99,7 +99,10 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
             d.NB = dims[db];
         }
         const bool rows_ok = p.periodic ? (p.nbl + p.nbr == 0) : p.dense_ok;
-        op->fast = p.uniform && rows_ok && mc3d::lineop_fast_supported(d, (size_t)lds_limit);
+        // the strided fast kernel addresses a field through a buffer descriptor
+        // with 32-bit byte offsets
+        const bool offsets_ok = npts * 8 < 2147483648L;
+        op->fast = p.uniform && rows_ok && offsets_ok && mc3d::lineop_fast_supported(d, (size_t)lds_limit);
         if (op->fast || p.MM == 16) break;
     }
     const mc3d::Plan &p = op->plan;

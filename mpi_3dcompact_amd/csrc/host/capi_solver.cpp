@@ -7,7 +7,10 @@
 #include <string>
 
 #include "../capi_common.h"
+#include "capi_guard.hpp"
+#include "closure.hpp"
 #include "dump.hpp"
+#include "mesh.hpp"
 #include "mc3d.h"
 #include "solver.hpp"
 
@@ -17,23 +20,7 @@ struct mc3d_solver {
     bool metrics = false, initial = false;
 };
 
-namespace {
-
-template <class F>
-int guarded(F &&f) {
-    try {
-        f();
-        return MC3D_OK;
-    } catch (const mc3d::Error &e) {
-        return mc3d::fail(e.code, e.what());
-    } catch (const std::bad_alloc &) {
-        return mc3d::fail(MC3D_ENOMEM, "host allocation failed");
-    } catch (const std::exception &e) {
-        return mc3d::fail(MC3D_EINVAL, e.what());
-    }
-}
-
-}  // namespace
+using mc3d::guarded;
 
 extern "C" {
 
@@ -78,6 +65,49 @@ int mc3d_solver_set_metrics(mc3d_solver *s, const double *const metrics[10]) {
     for (int k = 0; k < 10; ++k)
         if (!metrics[k]) return mc3d::fail(MC3D_EINVAL, "null metric array");
     return guarded([&] { s->cs->set_metrics_host(metrics); s->metrics = true; });
+}
+
+int mc3d_solver_set_mesh(mc3d_solver *s, const double *x, const double *y, const double *z) {
+    if (!s || !x || !y || !z) return mc3d::fail(MC3D_EINVAL, "null argument");
+    return guarded([&] { s->cs->set_mesh_host(x, y, z); s->metrics = true; });
+}
+
+int mc3d_mesh_metrics(const int fd[3], const int bc[3], const int dims[3], const double trans[9],
+                      const double *x, const double *y, const double *z, double *const out[10],
+                      void *stream) {
+    if (!fd || !bc || !dims || !x || !y || !z || !out) return mc3d::fail(MC3D_EINVAL, "null argument");
+    for (int k = 0; k < 10; ++k)
+        if (!out[k]) return mc3d::fail(MC3D_EINVAL, "null metric array");
+    for (int d = 0; d < 3; ++d) {
+        if (fd[d] != MC3D_CD2 && fd[d] != MC3D_PADE6 && fd[d] != MC3D_PENTA10)
+            return mc3d::fail(MC3D_EINVAL, "unknown derivative scheme");
+        if (bc[d] != MC3D_PERIODIC && bc[d] != MC3D_DIRICHLET) return mc3d::fail(MC3D_EINVAL, "unknown boundary");
+        if (dims[d] < 6) return mc3d::fail(MC3D_EINVAL, "each direction needs at least 6 points");
+        if (bc[d] == MC3D_PERIODIC && fd[d] == MC3D_CD2)
+            return mc3d::fail(MC3D_ECONFIG, "metrics: CD2 on a periodic direction has no transformed-periodic "
+                                            "form (src/AbstractSingleBlockMesh.cpp:94-97)");
+    }
+    return guarded([&] {
+        mc3d::Options o;
+        o.Nx = dims[0]; o.Ny = dims[1]; o.Nz = dims[2];
+        mc3d::Options::BCType *bt[3] = {&o.bcXType, &o.bcYType, &o.bcZType};
+        for (int d = 0; d < 3; ++d)
+            *bt[d] = bc[d] == MC3D_PERIODIC ? mc3d::Options::PERIODIC_SOLVE : mc3d::Options::DIRICHLET_SOLVE;
+        const mc3d::Domain dom(o);
+        const hipStream_t st = static_cast<hipStream_t>(stream);
+        using AD = mc3d::AbstractDerivatives;
+        mc3d::LineDerivative dx(fd[0], dom, o.bcXType, AD::DIRX, st), dy(fd[1], dom, o.bcYType, AD::DIRY, st),
+            dz(fd[2], dom, o.bcZType, AD::DIRZ, st);
+        AD *dv[3] = {&dx, &dy, &dz};
+        mc3d::MeshSpec m{};
+        for (int d = 0; d < 3; ++d) {
+            m.N[d] = dims[d];
+            m.periodic[d] = bc[d] == MC3D_PERIODIC;
+            m.bw7[d] = fd[d] == MC3D_PENTA10;
+            for (int e = 0; e < 3; ++e) m.T[d][e] = trans && m.periodic[d] ? trans[3 * d + e] : 0.0;
+        }
+        mc3d::solve_for_jacobians(dv, m, x, y, z, out, st);
+    });
 }
 
 int mc3d_solver_set_initial(mc3d_solver *s, const double *rho0, const double *U0, const double *V0,
@@ -228,6 +258,45 @@ int mc3d_solver_timing_report(mc3d_solver *s, char *buf, int buflen, int *needed
     if (needed) *needed = (int)j.size() + 1;
     if (!buf || buflen < (int)j.size() + 1) return mc3d::fail(MC3D_EINVAL, "report buffer too small");
     std::memcpy(buf, j.c_str(), j.size() + 1);
+    return MC3D_OK;
+}
+
+int mc3d_solver_set_closure(mc3d_solver *s, const mc3d_closure *c) {
+    if (!s || !c) return mc3d::fail(MC3D_EINVAL, "null argument");
+    auto *l = dynamic_cast<mc3d::LearnedSGS *>(s->cs->les.get());
+    if (!l) return mc3d::fail(MC3D_ECONFIG, "LES.LESMODEL is not LEARNED");
+    return guarded([&] { l->set_network(*mc3d_closure_net(c)); });
+}
+
+int mc3d_solver_sample_closure(mc3d_solver *s, float *X, float *T, long count, long offset, long stride) {
+    if (!s || !X || !T) return mc3d::fail(MC3D_EINVAL, "null argument");
+    mc3d::CurvilinearCSolver *cs = s->cs;
+    if (!cs->les || !cs->les->isDSM || !cs->les->C)
+        return mc3d::fail(MC3D_ECONFIG, "sampling needs LESMODEL DSM with DSMCOEFFRANGE TRUE");
+    if (count <= 0 || offset < 0 || stride <= 0 || offset + (count - 1) * stride >= cs->N)
+        return mc3d::fail(MC3D_EINVAL, "sample points outside the field");
+    cs->sample.X = X;
+    cs->sample.T = T;
+    cs->sample.count = count;
+    cs->sample.offset = offset;
+    cs->sample.stride = stride;
+    cs->sample.armed = true;
+    return MC3D_OK;
+}
+
+int mc3d_solver_eval_sgs(mc3d_solver *s) {
+    if (!s) return mc3d::fail(MC3D_EINVAL, "null solver");
+    if (!s->initial) return mc3d::fail(MC3D_EINVAL, "set the initial condition first");
+    return guarded([&] { s->cs->evalSGS(); });
+}
+
+int mc3d_solver_closure_stats(mc3d_solver *s, long *evaluations, long *fallback_stages, long *fallback_points) {
+    if (!s) return mc3d::fail(MC3D_EINVAL, "null solver");
+    auto *l = dynamic_cast<mc3d::LearnedSGS *>(s->cs->les.get());
+    if (!l) return mc3d::fail(MC3D_ECONFIG, "LES.LESMODEL is not LEARNED");
+    if (evaluations) *evaluations = l->evaluations;
+    if (fallback_stages) *fallback_stages = l->fallback_stages;
+    if (fallback_points) *fallback_points = l->fallback_points;
     return MC3D_OK;
 }
 

@@ -192,11 +192,13 @@ int Options::parse(const std::string &text, Options &o, std::string &err) {
     // NONE is accepted as an alias of NOMODEL (the reference README spells it so,
     // its parser does not: src/Options.cpp:6-15 vs README.md:47-49)
     r.enm("LES.LESMODEL", o.lesModel, {{"NOMODEL", O::NOMODEL}, {"NONE", O::NOMODEL},
-                                       {"VREMAN", O::VREMAN}, {"DSM", O::DSM}});
+                                       {"VREMAN", O::VREMAN}, {"DSM", O::DSM}, {"LEARNED", O::LEARNED}});
     r.enm("LES.LESAVERAGING", o.musgsAvgType, av);
     r.opt("LES.DSMTESTFILTER", o.testFilterType);
     r.opt("LES.DSMUSETAUKK", o.useTaukk);
     r.opt("LES.DSMCOEFFRANGE", o.dumpCoeffRange);
+    r.opt("LES.CLOSUREFILE", o.closureFile);
+    r.opt("LES.CLOSUREFALLBACK", o.closureFallback);
     if (!r.ok) return MC3D_ECONFIG;
     // src/Options.hpp:241 (checked after the sizes are known; with a restart
     // they come from the file header and are checked when it is read)

@@ -16,7 +16,7 @@ struct Options {
     enum SpongeKind { RECTILINEAR, CYLINDRICAL };
     enum FDType { CD2, PADE6, PENTA10 };
     enum FilterType { COMPACT8, COMPACT10 };
-    enum LESModel { NOMODEL, VREMAN, DSM };
+    enum LESModel { NOMODEL, VREMAN, DSM, LEARNED };
     enum StatsAvgType { NONE, XI1_AVG, XI2_AVG, XI3_AVG, LOCAL, GLOBAL };
     enum RKType { TVDRK3, RK4, KENRK4, LSLDDRK4 };
 
@@ -57,6 +57,10 @@ struct Options {
     LESModel lesModel = NOMODEL;
     bool dumpCoeffRange = false, useTaukk = true;
     int testFilterType = 1;
+    // learned closure (no counterpart in the reference): network file written
+    // by mc3d_closure_save, and whether out-of-range points fall back to the DSM
+    std::string closureFile;
+    bool closureFallback = true;
 
     // Parses INI text.  Returns 0, or a negative code with a message.
     static int parse(const std::string &text, Options &out, std::string &err);

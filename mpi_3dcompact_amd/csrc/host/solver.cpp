@@ -8,6 +8,8 @@
 #include <cstring>
 
 #include "../dsm.h"
+#include "closure.hpp"
+#include "mesh.hpp"
 #include "dump.hpp"
 #include "../lineop_api.h"
 
@@ -164,12 +166,12 @@ void VremanSGS::getSGSViscosity(const double *const dU[3], const double *const d
 
 // ---------------------------------------------------------------------------
 // src/DSMSGS.hpp:34-111
-DSMSGS::DSMSGS(CurvilinearCSolver *c) {
+DSMSGS::DSMSGS(CurvilinearCSolver *c, bool fallback_only) {
     cs = c;
-    isDSM = true;
+    isDSM = !fallback_only;
     musgsAvgType = c->opt.musgsAvgType;
-    useTaukk = c->opt.useTaukk;
-    coeffRangeFlag = c->opt.dumpCoeffRange;
+    useTaukk = fallback_only ? false : c->opt.useTaukk;
+    coeffRangeFlag = fallback_only ? false : c->opt.dumpCoeffRange;
     const int bc3[3] = {c->opt.bcXType == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET,
                         c->opt.bcYType == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET,
                         c->opt.bcZType == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET};
@@ -473,6 +475,7 @@ CurvilinearCSolver::CurvilinearCSolver(const Options &o, hipStream_t s)
     // src/CurvilinearCSolver.hpp:232-242
     if (opt.lesModel == Options::VREMAN) les.reset(new VremanSGS(this));
     else if (opt.lesModel == Options::DSM) les.reset(new DSMSGS(this));
+    else if (opt.lesModel == Options::LEARNED) les.reset(new LearnedSGS(this));
     // src/MPI_3DCompact.cpp:164-176
     switch (opt.rkType) {
         case Options::TVDRK3: rk.reset(new TVDRK3(this)); break;
@@ -519,6 +522,62 @@ void CurvilinearCSolver::setInitialConditions_host(const double *rho0, const dou
         hip_check(hipMemcpyAsync(d[k], src[k], sizeof(double) * N, hipMemcpyHostToDevice, st), "IC upload");
     hip_check(launch_init_state(d[0], d[1], d[2], d[3], d[4], r1, pr, msh, ig, N, st), "initial state");
     hip_check(hipStreamSynchronize(st), "initial state");
+}
+
+// handlePeriodicStuff, src/AbstractSingleBlockMesh.cpp:2019-2085: a periodic
+// direction translates by its PERIODICDISP row when the faces are PERIODIC
+// (for x the reference reads the y faces, :2034), by zero when they are
+// INTERNALLY_PERIODIC; anything else is reported (the reference then leaves
+// the translation unset; zero here).
+void CurvilinearCSolver::periodicTranslations(double T[3][3]) const {
+    const Options::BCType bt[3] = {opt.bcXType, opt.bcYType, opt.bcZType};
+    const Options::BCKind lo[3] = {opt.bcX0, opt.bcY0, opt.bcZ0};
+    const Options::BCKind hi[3] = {opt.bcX1, opt.bcY1, opt.bcZ1};
+    for (int d = 0; d < 3; ++d) {
+        for (int e = 0; e < 3; ++e) T[d][e] = 0.0;
+        if (bt[d] != Options::PERIODIC_SOLVE) continue;
+        const int checked = d == 0 ? 1 : d;
+        if (lo[checked] == Options::PERIODIC && hi[checked] == Options::PERIODIC) {
+            for (int e = 0; e < 3; ++e) T[d][e] = opt.periodicDisp[d][e];
+        } else if (!(lo[d] == Options::INTERNALLY_PERIODIC && hi[d] == Options::INTERNALLY_PERIODIC)) {
+            std::printf("  Incompatable periodic bc inputs in %c-direction! \n", "xyz"[d]);
+        }
+    }
+}
+
+// AbstractSingleBlockMesh::solveForJacobians (src/AbstractSingleBlockMesh.cpp:23-1309)
+// with this solver's derivative objects and the translations of its options.
+void CurvilinearCSolver::solveForJacobians(const double *x, const double *y, const double *z) {
+    const Options::BCType bt[3] = {opt.bcXType, opt.bcYType, opt.bcZType};
+    const Options::FDType ft[3] = {opt.xFDType, opt.yFDType, opt.zFDType};
+    MeshSpec m{};
+    m.N[0] = dom.gNx; m.N[1] = dom.gNy; m.N[2] = dom.gNz;
+    for (int d = 0; d < 3; ++d) {
+        m.periodic[d] = bt[d] == Options::PERIODIC_SOLVE;
+        m.bw7[d] = ft[d] == Options::PENTA10;
+        if (m.periodic[d] && ft[d] == Options::CD2)
+            throw Error(MC3D_ECONFIG, "metrics: CD2 on a periodic direction has no transformed-periodic form "
+                                      "(src/AbstractSingleBlockMesh.cpp:94-97)");
+    }
+    periodicTranslations(m.T);
+    AbstractDerivatives *dv[3] = {derivX.get(), derivY.get(), derivZ.get()};
+    double *out[10];
+    for (int k = 0; k < 10; ++k) out[k] = msh_rw(k);
+    solve_for_jacobians(dv, m, x, y, z, out, st);
+}
+
+void CurvilinearCSolver::set_mesh_host(const double *x, const double *y, const double *z) {
+    void *mem = nullptr;
+    hip_check(hipMalloc(&mem, sizeof(double) * 3 * (size_t)N), "mesh upload");
+    struct Free {
+        void *p;
+        ~Free() { (void)hipFree(p); }
+    } guard{mem};
+    double *d = static_cast<double *>(mem);
+    const double *h[3] = {x, y, z};
+    for (int k = 0; k < 3; ++k)
+        hip_check(hipMemcpyAsync(d + k * N, h[k], sizeof(double) * N, hipMemcpyHostToDevice, st), "mesh upload");
+    solveForJacobians(d, d + N, d + 2 * N);
 }
 
 // src/MPI_3DCompact.cpp:214-271: rho from the file, U = rhoU / rho (IdealGas
@@ -586,11 +645,31 @@ void CurvilinearCSolver::preStep() {
 }
 
 // src/CurvilinearCSolver_Core.cpp:468-787
+void CurvilinearCSolver::evalSGS() {
+    if (les && les->isDSM)
+        throw Error(MC3D_ECONFIG, "the DSM changes the pressure through tau_kk; its fields exist only inside a step");
+    const int saved = rkStep;
+    rkStep = 1;  // the current solution is the *1 register set
+    try {
+        gradientsAndSGS();
+    } catch (...) {
+        rkStep = saved;
+        throw;
+    }
+    rkStep = saved;
+    hip_check(hipStreamSynchronize(st), "SGS evaluation");
+}
+
 void CurvilinearCSolver::preSubStep() {
+    gradientsAndSGS();
+    flux();
+}
+
+// src/CurvilinearCSolver_Core.cpp:500-594: velocity and temperature
+// derivatives and the SGS fields of the current stage's state
+void CurvilinearCSolver::gradientsAndSGS() {
     const Cons &s = state();
     AbstractDerivatives *dv[3] = {derivX.get(), derivY.get(), derivZ.get()};
-    // gradients of U, V, W and T per direction (T does not depend on the SGS
-    // model here; the DSM branch, which modifies T first, is not built)
     if (les && les->isDSM) {
         // DSM changes p, T, mu and sos before the temperature gradient (:568-594)
         for (int d = 0; d < 3; ++d) {
@@ -607,6 +686,18 @@ void CurvilinearCSolver::preSubStep() {
             dv[d]->calc1stDerivFields(1, in, out, false);
         }
         les->calcKSGS(dT, s.q[0], s.q[1], s.q[2], s.q[3], pr.T);
+        if (sample.armed) {
+            sample.armed = false;
+            SampleArgs sa{};
+            for (int d = 0; d < 3; ++d) { sa.dU[d] = dU[d]; sa.dV[d] = dV[d]; sa.dW[d] = dW[d]; }
+            sa.m = msh;
+            sa.C = les->C;
+            sa.Prt = les->Prt;
+            sa.vol = dom.dx * dom.dy * dom.dz;
+            sa.offset = sample.offset; sa.stride = sample.stride; sa.count = sample.count;
+            sa.X = sample.X; sa.T = sample.T;
+            hip_check(launch_closure_samples(sa, st), "closure samples");
+        }
     } else {
         for (int d = 0; d < 3; ++d) {
             const double *in[4] = {pr.U, pr.V, pr.W, pr.T};
@@ -615,6 +706,11 @@ void CurvilinearCSolver::preSubStep() {
         }
         if (les) les->getSGSViscosity(dU, dV, dW, s.q[0]);
     }
+}
+
+// src/CurvilinearCSolver_Core.cpp:615-773: fluxes and their divergence
+void CurvilinearCSolver::flux() {
+    const Cons &s = state();
     FluxArgs fa{};
     for (int d = 0; d < 3; ++d) { fa.dU[d] = dU[d]; fa.dV[d] = dV[d]; fa.dW[d] = dW[d]; fa.dT[d] = dT[d]; }
     fa.m = msh;
@@ -623,7 +719,8 @@ void CurvilinearCSolver::preSubStep() {
     fa.k_sgs = les ? les->k_sgs : nullptr;
     fa.rho = s.q[0]; fa.rhoU = s.q[1]; fa.rhoV = s.q[2]; fa.rhoW = s.q[3]; fa.rhoE = s.q[4];
     for (int e = 0; e < 15; ++e) fa.pre[e] = pre[e];
-    fa.les = (opt.lesModel == Options::NOMODEL) ? 0 : (opt.lesModel == Options::DSM ? 2 : 1);
+    // 1: mu_sgs with k_sgs from Pr_t (Vreman); 2: mu_sgs and k_sgs fields (DSM, learned)
+    fa.les = opt.lesModel == Options::NOMODEL ? 0 : opt.lesModel == Options::VREMAN ? 1 : 2;
     fa.Pr_t = Pr_t;
     // read: 12 derivatives, 10 metrics, U V W p mu, 5 conserved (+ mu_sgs, k_sgs); written: 15
     const double flux_fields = 32.0 + 15.0 + (fa.les ? 1.0 : 0.0) + (fa.les == 2 ? 1.0 : 0.0);

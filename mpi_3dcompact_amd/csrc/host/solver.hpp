@@ -192,7 +192,8 @@ class VremanSGS : public AbstractSGS {
 // as batched line-operator launches over all the fields it filters.
 class DSMSGS : public AbstractSGS {
   public:
-    explicit DSMSGS(CurvilinearCSolver *cs);
+    // fallback_only: the learned closure's fallback (no tau_kk, no coefficient output)
+    explicit DSMSGS(CurvilinearCSolver *cs, bool fallback_only = false);
     void getSGSViscosity(const double *const *, const double *const *, const double *const *,
                          const double *) override {}
     void calcMuSGSTaukk(const double *const dU[3], const double *const dV[3],
@@ -304,6 +305,11 @@ class CurvilinearCSolver {
 
     double *alloc_field();
     void set_metrics_host(const double *const m[10]);
+    // metrics from mesh coordinates (device x, y, z), src/AbstractSingleBlockMesh.cpp:23-1309
+    void solveForJacobians(const double *x, const double *y, const double *z);
+    void set_mesh_host(const double *x, const double *y, const double *z);
+    void periodicTranslations(double T[3][3]) const;
+    double *msh_rw(int k) { return Jm[k]; }
     // FROMRESTART: conserved fields of a SolutionDump file converted to the
     // primitive initial state (src/MPI_3DCompact.cpp:214-271), then set up
     void load_restart_host(const std::string &path);
@@ -320,6 +326,8 @@ class CurvilinearCSolver {
     // stage methods, in the reference's call order
     void preStep();               // calcDtFromCFL
     void preSubStep();            // preStepDerivatives
+    void gradientsAndSGS();       //   its derivative and SGS part
+    void flux();                  //   its flux and divergence part
     void solveEqnSet() {}         // k2 assembly is fused into the RK stage kernel
     void postSubStep() {}         // periodic boundaries: no boundary fix-ups
     void updateData();            // filter (last stage / substeps) + EOS
@@ -333,6 +341,17 @@ class CurvilinearCSolver {
 
     const Cons &state() const { return rkStep == 1 ? r1 : rk_; }
     double *field_by_name(const std::string &name);
+
+    // learned closure: training pairs taken at the first stage of the next
+    // step, from the DSM's coefficients and the gradients it used
+    struct SampleRequest {
+        float *X = nullptr, *T = nullptr;
+        long count = 0, offset = 0, stride = 1;
+        bool armed = false;
+    } sample;
+    // derivatives and SGS fields of the current state without advancing it
+    // (models that leave the thermodynamic state unchanged)
+    void evalSGS();
 
   private:
     std::chrono::steady_clock::time_point last_check_ = std::chrono::steady_clock::now();

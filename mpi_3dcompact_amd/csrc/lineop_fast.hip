@@ -4,7 +4,10 @@
 // and one chunk of MM rows, so every global load and store is a 512-byte
 // contiguous run and the chunk index is wave-uniform: the factor tables are
 // read with scalar loads.  Each lane keeps its chunk plus the stencil halo in
-// registers; LDS is only used for the interface exchange.
+// registers; LDS is only used for the interface exchange.  The window loads
+// and the stores go through a buffer descriptor (one 32-bit lane offset, the
+// row offset in a scalar register), and all window loads are issued before
+// any other work of the lane.
 //
 // Contiguous direction (x): the workgroup stages 32 (periodic) or 64 lines
 // into LDS in a transposed [point][line] layout (row stride lines + 1) with
@@ -87,6 +90,17 @@ __device__ __forceinline__ void chunk_solve(const double *lu, double *y) {
     }
 }
 
+// raw buffer access: descriptor in SGPRs, one 32-bit lane offset, row offset in an SGPR
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+constexpr int kRawBufferWord3 = 0x00020000;  // gfx9-family raw buffer descriptor, dword 3
+__device__ __forceinline__ double bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+__device__ __forceinline__ void bstore(double v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+}
+
 __device__ __forceinline__ bool pass_line(const OpDev &op, int dir, int q, int Nx, int Ny) {
     if (!(op.passA | op.passB)) return false;
     int ia, ib;
@@ -115,17 +129,33 @@ __global__ __launch_bounds__(1024) void lineop_strided(OpDev op, LaunchArgs a) {
         ps = (long)a.Nx * a.Ny;
     }
     const int cs = c * MM;
+    // window loads through a buffer descriptor (fields below 2^28 points)
+    const unsigned fbytes = (unsigned)a.Nx * (unsigned)a.Ny * (unsigned)a.Nz * 8u;
+    const __amdgpu_buffer_rsrc_t rin =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(in), (short)0, (int)fbytes, kRawBufferWord3);
+    const unsigned boff = (unsigned)base * 8u, rowb = (unsigned)ps * 8u;
     double w[MM + 2 * H];
+    // row offsets: consecutive rows from (cs - H); only the halo rows of the
+    // first and last chunk wrap (periodic) or fall outside the line (zero)
+    const int s0 = cs - H;
+    const bool first = c == 0, last = c == P - 1;
 #pragma unroll
     for (int j = 0; j < MM + 2 * H; ++j) {
-        int p = cs - H + j;
-        if (PER) {
-            p = p < 0 ? p + N : (p >= N ? p - N : p);
-            w[j] = in[base + p * ps];
+        if (j < H || j >= MM + H) {
+            int p = s0 + j;
+            const bool outside = (j < H && first) || (j >= MM + H && last);
+            if (PER) {
+                if (outside) p = j < H ? p + N : p - N;
+                w[j] = bload(rin, boff, (unsigned)p * rowb);
+            } else {
+                w[j] = outside ? 0.0 : bload(rin, boff, (unsigned)p * rowb);
+            }
         } else {
-            w[j] = (p >= 0 && p < N) ? in[base + p * ps] : 0.0;
+            w[j] = bload(rin, boff, (unsigned)(s0 + j) * rowb);
         }
     }
+    // issue the window loads before any table load or arithmetic
+    __builtin_amdgcn_sched_barrier(0);
     double y[MM];
     chunk_rhs<H, MM, PER>(op, w, y, c == 0, c == P - 1);
     if (W > 0) {
@@ -160,14 +190,18 @@ __global__ __launch_bounds__(1024) void lineop_strided(OpDev op, LaunchArgs a) {
         }
     }
     if (!live) return;
-    const bool pass = pass_line(op, op.dir, q, a.Nx, a.Ny);
+    const __amdgpu_buffer_rsrc_t rout =
+        __builtin_amdgcn_make_buffer_rsrc(out, (short)0, (int)fbytes, kRawBufferWord3);
+    if (pass_line(op, op.dir, q, a.Nx, a.Ny)) {
 #pragma unroll
-    for (int i = 0; i < MM; ++i) {
-        const long g = base + (long)(cs + i) * ps;
-        const double v = pass ? in[g] : y[i];
-        if (a.accumulate) out[g] += v;
-        else out[g] = v;
+        for (int i = 0; i < MM; ++i) y[i] = bload(rin, boff, (unsigned)(cs + i) * rowb);
     }
+    if (a.accumulate) {
+#pragma unroll
+        for (int i = 0; i < MM; ++i) y[i] = bload(rout, boff, (unsigned)(cs + i) * rowb) + y[i];
+    }
+#pragma unroll
+    for (int i = 0; i < MM; ++i) bstore(y[i], rout, boff, (unsigned)(cs + i) * rowb);
 }
 
 // lines per workgroup of the contiguous kernel: 32 for periodic lines (every

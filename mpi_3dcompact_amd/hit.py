@@ -80,9 +80,10 @@ def options_text(N, fd="PADE6", filt="COMPACT10", alpha_f=0.49, rk="TVDRK3", cfl
                  mu_ref=1.0 / 1600.0, les="NOMODEL", les_avg="NONE", max_steps=100000,
                  filter_step=1, substep_filtering=False, dsm_filter=1, use_taukk=True,
                  coeff_range=False, ts_type="CONST_CFL", dt=None, restart_file=None,
-                 dump_step=1000000):
+                 dump_step=1000000, disp=None):
     """MPI_3DCompact.in text for a periodic HIT box of N^3 points (restart_file:
-    start from that SolutionDump, which also sets the grid)."""
+    start from that SolutionDump, which also sets the grid; disp: 3x3
+    PERIODICDISP rows, default the 2 pi box)."""
     fds = fd if isinstance(fd, (list, tuple)) else (fd, fd, fd)
     lines = ["[RESTART]"]
     if restart_file:
@@ -101,10 +102,11 @@ def options_text(N, fd="PADE6", filt="COMPACT10", alpha_f=0.49, rk="TVDRK3", cfl
         "BCX0 = PERIODIC", "BCX1 = PERIODIC", "BCY0 = PERIODIC", "BCY1 = PERIODIC",
         "BCZ0 = PERIODIC", "BCZ1 = PERIODIC",
     ]
-    L = repr(2 * np.pi)
-    for a in "XYZ":
-        for b in "XYZ":
-            lines.append(f"PERIODICDISP{a}_{b} = {L if a == b else 0}")
+    if disp is None:
+        disp = 2 * np.pi * np.eye(3)
+    for i, a in enumerate("XYZ"):
+        for j, b in enumerate("XYZ"):
+            lines.append(f"PERIODICDISP{a}_{b} = {float(disp[i][j])!r}")
     lines += [
         "[DOMAIN]"] + ([] if restart_file else [f"NX = {N}", f"NY = {N}", f"NZ = {N}"]) + [
         "[PENCILDECOMP]", "PROW = 0", "PCOL = 0",

@@ -11,6 +11,7 @@ _D = ctypes.POINTER(ctypes.c_double)
 _SIGS = [
     ("mc3d_solver_create", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_VP)]),
     ("mc3d_solver_set_metrics", ctypes.c_int, [_VP, ctypes.POINTER(_D)]),
+    ("mc3d_solver_set_mesh", ctypes.c_int, [_VP, _D, _D, _D]),
     ("mc3d_solver_set_initial", ctypes.c_int, [_VP, _D, _D, _D, _D, _D]),
     ("mc3d_solver_step", ctypes.c_int, [_VP, ctypes.c_int]),
     ("mc3d_solver_load_restart", ctypes.c_int, [_VP]),
@@ -59,6 +60,11 @@ class Solver:
         ms = [np.ascontiguousarray(m, dtype=np.float64) for m in metrics]
         arr = (_D * 10)(*[_p(m) for m in ms])
         check(self._L.mc3d_solver_set_metrics(self._h, arr))
+
+    def set_mesh(self, x, y, z):
+        """Metrics from mesh coordinates (AbstractSingleBlockMesh::solveForJacobians)."""
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, z)]
+        check(self._L.mc3d_solver_set_mesh(self._h, *[_p(v) for v in a]))
 
     def set_initial(self, ic):
         a = [np.ascontiguousarray(ic[k], dtype=np.float64) for k in ("rho", "U", "V", "W", "p")]

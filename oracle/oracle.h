@@ -61,6 +61,17 @@ int orc_deriv_field_tpb(int scheme, int dir, int Nx, int Ny, int Nz, double dd,
 int orc_filter_field(int kind, double alphaF, int dir, const int bc3[3], int Nx, int Ny,
                      int Nz, const double *in, double *out);
 
+/* ---- metrics from mesh coordinates (solveForJacobians) ----
+ * fd/bc per direction, N = {Nx, Ny, Nz}, trans[d] = translation vector of
+ * periodic direction d; x, y, z and out[10] = {J, J11, ..., J33} are x-fastest
+ * fields.  Returns 0, or nonzero for a periodic direction with a scheme the
+ * reference's metric path rejects (CD2). */
+int orc_metrics(const int fd[3], const int bc[3], const int N[3], const double trans[3][3],
+                const double *x, const double *y, const double *z, double *const out[10]);
+/* handlePeriodicStuff: face kinds 0 PERIODIC, 1 INTERNALLY_PERIODIC */
+void orc_periodic_translations(const int bc[3], const int face_lo[3], const int face_hi[3],
+                               const double disp[3][3], double trans[3][3], int set[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -98,6 +98,28 @@ def deriv_field(scheme, direction, bc, f, dd):
     return out
 
 
+def metrics(fd, bc, x, y, z, trans):
+    """J, J11..J33 (list of 10 arrays shaped like x) from mesh coordinates."""
+    L = lib()
+    if not getattr(L, "_metrics_bound", False):
+        L.orc_metrics.restype = ctypes.c_int
+        L.orc_metrics.argtypes = [ctypes.POINTER(ctypes.c_int)] * 3 + [ctypes.POINTER(ctypes.c_double)] + \
+            [_D] * 3 + [ctypes.POINTER(_D)]
+        L._metrics_bound = True
+    shape = x.shape
+    N = (ctypes.c_int * 3)(shape[2], shape[1], shape[0])
+    fdc = (ctypes.c_int * 3)(*fd)
+    bcc = (ctypes.c_int * 3)(*bc)
+    tr = (ctypes.c_double * 9)(*[float(t) for row in trans for t in row])
+    xs = [_c(np.ascontiguousarray(a, dtype=np.float64)) for a in (x, y, z)]
+    outs = [np.empty(shape) for _ in range(10)]
+    arr = (_D * 10)(*[o.ctypes.data_as(_D) for o in outs])
+    rc = L.orc_metrics(fdc, bcc, N, tr, *[a.ctypes.data_as(_D) for a in xs], arr)
+    if rc:
+        raise RuntimeError(f"orc_metrics failed ({rc})")
+    return outs
+
+
 def deriv_field_tpb(scheme, direction, f, dd, wrap):
     """wrap: list of 6 arrays (Nm3,Nm2,Nm1,Np1,Np2,Np3), each one value per line."""
     f = _c(f)
