@@ -146,7 +146,9 @@ int mc3d_solver_run(mc3d_solver *s);
 /* copy a named field to host memory / get its device address.  Names:
  * rho rhoU rhoV rhoW rhoE (the *1 registers), rhok.. rhoEk, rhok2.. rhoEk2,
  * rho2.. rhoE2, U V W p T mu sos Ucurv Vcurv Wcurv mu_sgs k_sgs taukk,
- * J J11 .. J33. */
+ * C CI Prt (DSMCOEFFRANGE), J J11 .. J33, and dU0 dU1 dU2 dV0 .. dW2 dT0 ..
+ * dT2, the computational derivatives (d/dxi, d/deta, d/dzeta) of the last
+ * evaluated stage. */
 int mc3d_solver_get_field(mc3d_solver *s, const char *name, double *host_out);
 /* FROMRESTART ([RESTART] FILENAME): the conserved fields of that SolutionDump
  * file become the initial state through primitive variables, as
@@ -167,6 +169,12 @@ int mc3d_solver_write_dump(mc3d_solver *s, const char *path, const double *x, co
  * into dir (NULL: working directory), with the given host mesh (copied). */
 int mc3d_solver_set_output(mc3d_solver *s, const char *dir, const double *x, const double *y,
                            const double *z);
+/* Decaying-HIT statistics of the current state, the turbdata.out hook of
+ * tests/hit/Samtaney_D4/MPI_3DCompact.cpp:380-556 (device reductions):
+ * stats = {time, tau, Re_lambda, M_t, epsilon = <2 mu S*_ij du_i/dx_j> / <rho>,
+ * u', Kolmogorov length, rho', dilatation', vorticity', <rho u.u>}.
+ * Overwrites the velocity-derivative work fields.  Synchronous. */
+int mc3d_solver_hit_stats(mc3d_solver *s, double stats[11]);
 int mc3d_solver_field_device(mc3d_solver *s, const char *name, double **dev_ptr);
 int mc3d_solver_scalars(mc3d_solver *s, double *time, double *dt, int *timestep);
 /* the solver's HIP stream (as void*), for timing with events */
@@ -235,9 +243,10 @@ int mc3d_closure_save(const mc3d_closure *c, const char *path);
 int mc3d_closure_load(mc3d_closure *c, const char *path);
 /* copy the network and its scaling into a LESMODEL LEARNED solver */
 int mc3d_solver_set_closure(mc3d_solver *s, const mc3d_closure *c);
-/* training pairs from a LESMODEL DSM solver with DSMCOEFFRANGE TRUE: at the
- * first stage of the next step, for i < count and p = offset + i * stride,
- * X[i][7] = raw features, T[i] = (C, C / Pr_t) of the DSM at p (device FP32) */
+/* training pairs from a LESMODEL DSM solver with DSMCOEFFRANGE TRUE: the DSM
+ * procedure runs on the current state (which is left unchanged) and, for
+ * i < count and p = offset + i * stride, X[i][7] = raw features and
+ * T[i] = (C, C / Pr_t) of the DSM at p (device FP32).  Synchronous. */
 int mc3d_solver_sample_closure(mc3d_solver *s, float *X, float *T, long count, long offset,
                                long stride);
 /* derivatives and SGS fields (mu_sgs, k_sgs) of the current state without

@@ -270,18 +270,16 @@ int mc3d_solver_set_closure(mc3d_solver *s, const mc3d_closure *c) {
 
 int mc3d_solver_sample_closure(mc3d_solver *s, float *X, float *T, long count, long offset, long stride) {
     if (!s || !X || !T) return mc3d::fail(MC3D_EINVAL, "null argument");
-    mc3d::CurvilinearCSolver *cs = s->cs;
-    if (!cs->les || !cs->les->isDSM || !cs->les->C)
-        return mc3d::fail(MC3D_ECONFIG, "sampling needs LESMODEL DSM with DSMCOEFFRANGE TRUE");
-    if (count <= 0 || offset < 0 || stride <= 0 || offset + (count - 1) * stride >= cs->N)
+    if (!s->initial) return mc3d::fail(MC3D_EINVAL, "set the initial condition first");
+    if (count <= 0 || offset < 0 || stride <= 0 || offset + (count - 1) * stride >= s->cs->N)
         return mc3d::fail(MC3D_EINVAL, "sample points outside the field");
-    cs->sample.X = X;
-    cs->sample.T = T;
-    cs->sample.count = count;
-    cs->sample.offset = offset;
-    cs->sample.stride = stride;
-    cs->sample.armed = true;
-    return MC3D_OK;
+    return guarded([&] { s->cs->sampleClosure(X, T, count, offset, stride); });
+}
+
+int mc3d_solver_hit_stats(mc3d_solver *s, double stats[11]) {
+    if (!s || !stats) return mc3d::fail(MC3D_EINVAL, "null argument");
+    if (!s->initial) return mc3d::fail(MC3D_EINVAL, "set the initial condition first");
+    return guarded([&] { s->cs->hitStats(stats); });
 }
 
 int mc3d_solver_eval_sgs(mc3d_solver *s) {

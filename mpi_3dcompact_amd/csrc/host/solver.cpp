@@ -4,10 +4,12 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <vector>
 #include <cstdio>
 #include <cstring>
 
 #include "../dsm.h"
+#include "../stats.h"
 #include "closure.hpp"
 #include "mesh.hpp"
 #include "dump.hpp"
@@ -660,6 +662,93 @@ void CurvilinearCSolver::evalSGS() {
     hip_check(hipStreamSynchronize(st), "SGS evaluation");
 }
 
+void CurvilinearCSolver::sampleClosure(float *X, float *T, long count, long offset, long stride) {
+    if (!les || !les->isDSM || !les->C)
+        throw Error(MC3D_ECONFIG, "sampling needs LESMODEL DSM with DSMCOEFFRANGE TRUE");
+    double *keep[4] = {pr.p, pr.T, pr.mu, pr.sos};
+    for (int k = 0; k < 4; ++k)
+        hip_check(hipMemcpyAsync(tmp[k], keep[k], sizeof(double) * N, hipMemcpyDeviceToDevice, st), "save state");
+    const int saved = rkStep;
+    rkStep = 1;  // the current solution is the *1 register set
+    try {
+        gradientsAndSGS();
+    } catch (...) {
+        rkStep = saved;
+        throw;
+    }
+    rkStep = saved;
+    SampleArgs sa{};
+    for (int d = 0; d < 3; ++d) { sa.dU[d] = dU[d]; sa.dV[d] = dV[d]; sa.dW[d] = dW[d]; }
+    sa.m = msh;
+    sa.C = les->C;
+    sa.Prt = les->Prt;
+    sa.vol = dom.dx * dom.dy * dom.dz;
+    sa.offset = offset; sa.stride = stride; sa.count = count;
+    sa.X = X; sa.T = T;
+    hip_check(launch_closure_samples(sa, st), "closure samples");
+    for (int k = 0; k < 4; ++k)
+        hip_check(hipMemcpyAsync(keep[k], tmp[k], sizeof(double) * N, hipMemcpyDeviceToDevice, st), "restore state");
+    hip_check(hipStreamSynchronize(st), "closure samples");
+}
+
+void CurvilinearCSolver::hitStats(double out[11]) {
+    AbstractDerivatives *dv[3] = {derivX.get(), derivY.get(), derivZ.get()};
+    for (int d = 0; d < 3; ++d) {
+        const double *in[3] = {pr.U, pr.V, pr.W};
+        double *o[3] = {dU[d], dV[d], dW[d]};
+        dv[d]->calc1stDerivFields(3, in, o, false);
+    }
+    const int nb = stats_blocks(N);
+    std::vector<double> part((size_t)nb * kStatSums), part2((size_t)nb);
+    void *mem = nullptr;
+    hip_check(hipMalloc(&mem, sizeof(double) * (size_t)nb * (kStatSums + 1)), "statistics scratch");
+    struct Free {
+        void *p;
+        hipStream_t s;
+        ~Free() {
+            (void)hipStreamSynchronize(s);
+            (void)hipFree(p);
+        }
+    } guard{mem, st};
+    StatsArgs a{};
+    for (int d = 0; d < 3; ++d) { a.dU[d] = dU[d]; a.dV[d] = dV[d]; a.dW[d] = dW[d]; }
+    a.m = msh;
+    a.rho = r1.q[0]; a.U = pr.U; a.V = pr.V; a.W = pr.W; a.mu = pr.mu; a.sos = pr.sos;
+    a.partial = static_cast<double *>(mem);
+    a.partial2 = a.partial + (size_t)nb * kStatSums;
+    hip_check(launch_hit_stats(a, N, st), "statistics");
+    hip_check(hipMemcpyAsync(part.data(), a.partial, sizeof(double) * part.size(), hipMemcpyDeviceToHost, st),
+              "statistics");
+    StepScalars h;
+    hip_check(hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, st), "statistics");
+    hip_check(hipStreamSynchronize(st), "statistics");
+    double m[kStatSums] = {};
+    for (int b = 0; b < nb; ++b)
+        for (int k = 0; k < kStatSums; ++k) m[k] += part[(size_t)b * kStatSums + k];
+    const double Nall = (double)N;
+    for (double &v : m) v /= Nall;
+    hip_check(launch_hit_stats_rho(r1.q[0], m[0], a.partial2, N, st), "statistics");
+    hip_check(hipMemcpyAsync(part2.data(), a.partial2, sizeof(double) * nb, hipMemcpyDeviceToHost, st), "statistics");
+    hip_check(hipStreamSynchronize(st), "statistics");
+    double rhovar = 0.0;
+    for (int b = 0; b < nb; ++b) rhovar += part2[(size_t)b];
+    rhovar /= Nall;
+    // :521-543
+    const double meanRho = m[0], meanMu = m[1], meanSOS = m[6], meanKE = m[8];
+    const double meanNu = meanMu / meanRho;
+    const double meanTurbDiss = m[2] / meanRho;
+    const double uprime = std::sqrt(m[3]), uiprime = std::sqrt(m[4]), urms = std::sqrt(m[5]);
+    const double turbMach = urms / meanSOS;
+    const double taylorMicro2 = uprime / uiprime;
+    const double taylorReyn3 = uprime * taylorMicro2 / meanNu;
+    const double k0 = 8.0;
+    const double tau = std::sqrt(2 * M_PI) / k0 / uprime;
+    const double kolNu = std::pow(std::pow(meanNu, 3.0) / meanTurbDiss, 0.25);
+    const double v[11] = {h.time, tau, taylorReyn3, turbMach, meanTurbDiss, uprime, kolNu,
+                          std::sqrt(rhovar), std::sqrt(m[9]), std::sqrt(m[10]), meanKE};
+    for (int k = 0; k < 11; ++k) out[k] = v[k];
+}
+
 void CurvilinearCSolver::preSubStep() {
     gradientsAndSGS();
     flux();
@@ -686,18 +775,6 @@ void CurvilinearCSolver::gradientsAndSGS() {
             dv[d]->calc1stDerivFields(1, in, out, false);
         }
         les->calcKSGS(dT, s.q[0], s.q[1], s.q[2], s.q[3], pr.T);
-        if (sample.armed) {
-            sample.armed = false;
-            SampleArgs sa{};
-            for (int d = 0; d < 3; ++d) { sa.dU[d] = dU[d]; sa.dV[d] = dV[d]; sa.dW[d] = dW[d]; }
-            sa.m = msh;
-            sa.C = les->C;
-            sa.Prt = les->Prt;
-            sa.vol = dom.dx * dom.dy * dom.dz;
-            sa.offset = sample.offset; sa.stride = sample.stride; sa.count = sample.count;
-            sa.X = sample.X; sa.T = sample.T;
-            hip_check(launch_closure_samples(sa, st), "closure samples");
-        }
     } else {
         for (int d = 0; d < 3; ++d) {
             const double *in[4] = {pr.U, pr.V, pr.W, pr.T};
@@ -898,6 +975,12 @@ double *CurvilinearCSolver::field_by_name(const std::string &n) {
     if (n == "C") return les ? les->C : nullptr;
     if (n == "CI") return les ? les->CI : nullptr;
     if (n == "Prt") return les ? les->Prt : nullptr;
+    // computational derivatives of the last evaluated stage: dU0 = dU/dxi, dU1 = dU/deta, ...
+    if (n.size() == 3 && n[0] == 'd' && (n[1] == 'U' || n[1] == 'V' || n[1] == 'W' || n[1] == 'T') &&
+        n[2] >= '0' && n[2] <= '2') {
+        const int d = n[2] - '0';
+        return n[1] == 'U' ? dU[d] : n[1] == 'V' ? dV[d] : n[1] == 'W' ? dW[d] : dT[d];
+    }
     static const char *met[10] = {"J", "J11", "J12", "J13", "J21", "J22", "J23", "J31", "J32", "J33"};
     for (int k = 0; k < 10; ++k)
         if (n == met[k]) return Jm[k];

@@ -342,16 +342,18 @@ class CurvilinearCSolver {
     const Cons &state() const { return rkStep == 1 ? r1 : rk_; }
     double *field_by_name(const std::string &name);
 
-    // learned closure: training pairs taken at the first stage of the next
-    // step, from the DSM's coefficients and the gradients it used
-    struct SampleRequest {
-        float *X = nullptr, *T = nullptr;
-        long count = 0, offset = 0, stride = 1;
-        bool armed = false;
-    } sample;
+    // learned closure: training pairs (features, DSM coefficients) of the
+    // current state; the DSM runs on it and the fields its tau_kk update
+    // changes are restored afterwards
+    void sampleClosure(float *X, float *T, long count, long offset, long stride);
     // derivatives and SGS fields of the current state without advancing it
     // (models that leave the thermodynamic state unchanged)
     void evalSGS();
+    // decaying-HIT statistics of the current state, the turbdata.out columns
+    // of tests/hit/Samtaney_D4/MPI_3DCompact.cpp:545-556: time, tau,
+    // Re_lambda, M_t, epsilon, u', Kolmogorov scale, rho', dil', omega',
+    // <rho u.u>
+    void hitStats(double out[11]);
 
   private:
     std::chrono::steady_clock::time_point last_check_ = std::chrono::steady_clock::now();

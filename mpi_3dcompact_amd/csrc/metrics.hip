@@ -15,14 +15,22 @@ constexpr int kBlock = 256;
 
 inline dim3 grid_for(long n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 
-__global__ void k_tpb_wraps(WrapArgs a, int dir, int Nx, int Ny, int Nz, long nlines) {
+// line geometry of direction DIR (line order of the line operators); a
+// compile-time direction keeps the three cases apart in the generated code
+template <int DIR>
+__device__ __forceinline__ void line_geometry(long q, int Nx, int Ny, int Nz, long &base, long &stride, int &n) {
+    if (DIR == 0) { base = q * Nx; stride = 1; n = Nx; }
+    if (DIR == 1) { base = (q / Nx) * (long)Nx * Ny + q % Nx; stride = Nx; n = Ny; }
+    if (DIR == 2) { base = q; stride = (long)Nx * Ny; n = Nz; }
+}
+
+template <int DIR>
+__global__ void k_tpb_wraps(WrapArgs a, int Nx, int Ny, int Nz, long nlines) {
     const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nlines) return;
     long base, stride;
     int n;
-    if (dir == 0) { base = q * Nx; stride = 1; n = Nx; }
-    else if (dir == 1) { base = (q / Nx) * (long)Nx * Ny + q % Nx; stride = Nx; n = Ny; }
-    else { base = q; stride = (long)Nx * Ny; n = Nz; }
+    line_geometry<DIR>(q, Nx, Ny, Nz, base, stride, n);
     for (int m = 0; m < 6; ++m) {
         const bool lo = m < 3;  // Nm slots: the line's last three points, shifted down
         const long p = base + (lo ? (n - 3 + m) : (m - 3)) * stride;
@@ -88,9 +96,12 @@ __global__ void k_metric_final9(const double *J1, const double *J2, const double
 }  // namespace
 
 hipError_t launch_tpb_wraps(const WrapArgs &a, int dir, int Nx, int Ny, int Nz, hipStream_t st) {
+    if (dir < 0 || dir > 2) return hipErrorInvalidValue;
     const int n = dir == 0 ? Nx : dir == 1 ? Ny : Nz;
     const long nlines = (long)Nx * Ny * Nz / n;
-    hipLaunchKernelGGL(k_tpb_wraps, grid_for(nlines), dim3(kBlock), 0, st, a, dir, Nx, Ny, Nz, nlines);
+    if (dir == 0) hipLaunchKernelGGL(k_tpb_wraps<0>, grid_for(nlines), dim3(kBlock), 0, st, a, Nx, Ny, Nz, nlines);
+    if (dir == 1) hipLaunchKernelGGL(k_tpb_wraps<1>, grid_for(nlines), dim3(kBlock), 0, st, a, Nx, Ny, Nz, nlines);
+    if (dir == 2) hipLaunchKernelGGL(k_tpb_wraps<2>, grid_for(nlines), dim3(kBlock), 0, st, a, Nx, Ny, Nz, nlines);
     return hipGetLastError();
 }
 

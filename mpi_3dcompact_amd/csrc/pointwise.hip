@@ -323,15 +323,16 @@ __global__ void k_fill(double *p, double v, long n) {
 
 // src/AbstractSGS.cpp:15-131: each term divided by the line length, summed in
 // index order, then broadcast back along the line
-__global__ void k_xi_average(const double *phi, double *phiF, int dir, int Nx, int Ny, int Nz) {
+template <int DIR>
+__global__ void k_xi_average(const double *phi, double *phiF, int Nx, int Ny, int Nz) {
     const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int nl = dir == 0 ? Nx : dir == 1 ? Ny : Nz;
+    const int nl = DIR == 0 ? Nx : DIR == 1 ? Ny : Nz;
     const long nlines = (long)Nx * Ny * Nz / nl;
     if (q >= nlines) return;
-    long base, stride;
-    if (dir == 0) { base = q * Nx; stride = 1; }
-    else if (dir == 1) { base = (q / Nx) * (long)Nx * Ny + q % Nx; stride = Nx; }
-    else { base = q; stride = (long)Nx * Ny; }
+    long base = 0, stride = 1;
+    if (DIR == 0) { base = q * Nx; stride = 1; }
+    if (DIR == 1) { base = (q / Nx) * (long)Nx * Ny + q % Nx; stride = Nx; }
+    if (DIR == 2) { base = q; stride = (long)Nx * Ny; }
     double acc = 0.0;
     for (int i = 0; i < nl; ++i) acc += phi[base + i * stride] / (double)nl;
     for (int i = 0; i < nl; ++i) phiF[base + i * stride] = acc;
@@ -458,9 +459,12 @@ hipError_t launch_fill(double *p, double v, long n, hipStream_t st) {
 
 hipError_t launch_xi_average(const double *phi, double *phiF, int dir, int Nx, int Ny, int Nz,
                              hipStream_t st) {
+    if (dir < 0 || dir > 2) return hipErrorInvalidValue;
     const int nl = dir == 0 ? Nx : dir == 1 ? Ny : Nz;
     const long nlines = (long)Nx * Ny * Nz / nl;
-    hipLaunchKernelGGL(k_xi_average, grid_for(nlines), dim3(kBlock), 0, st, phi, phiF, dir, Nx, Ny, Nz);
+    if (dir == 0) hipLaunchKernelGGL(k_xi_average<0>, grid_for(nlines), dim3(kBlock), 0, st, phi, phiF, Nx, Ny, Nz);
+    if (dir == 1) hipLaunchKernelGGL(k_xi_average<1>, grid_for(nlines), dim3(kBlock), 0, st, phi, phiF, Nx, Ny, Nz);
+    if (dir == 2) hipLaunchKernelGGL(k_xi_average<2>, grid_for(nlines), dim3(kBlock), 0, st, phi, phiF, Nx, Ny, Nz);
     return hipGetLastError();
 }
 

@@ -80,7 +80,8 @@ def options_text(N, fd="PADE6", filt="COMPACT10", alpha_f=0.49, rk="TVDRK3", cfl
                  mu_ref=1.0 / 1600.0, les="NOMODEL", les_avg="NONE", max_steps=100000,
                  filter_step=1, substep_filtering=False, dsm_filter=1, use_taukk=True,
                  coeff_range=False, ts_type="CONST_CFL", dt=None, restart_file=None,
-                 dump_step=1000000, disp=None):
+                 dump_step=1000000, disp=None, closure_file=None, closure_fallback=True,
+                 faces=None, sponge=None):
     """MPI_3DCompact.in text for a periodic HIT box of N^3 points (restart_file:
     start from that SolutionDump, which also sets the grid; disp: 3x3
     PERIODICDISP rows, default the 2 pi box)."""
@@ -90,6 +91,7 @@ def options_text(N, fd="PADE6", filt="COMPACT10", alpha_f=0.49, rk="TVDRK3", cfl
         lines += ["FROMRESTART = TRUE", "ONLYGRIDFROMRESTART = FALSE", f"FILENAME = {restart_file}"]
     else:
         lines += ["FROMRESTART = FALSE", "ONLYGRIDFROMRESTART = FALSE"]
+    fk = face_kinds(faces)
     lines += [
         "SPONGEAVGFROMRESTART = FALSE",
         "[TIMESTEPPING]", f"TSTYPE = {ts_type}",
@@ -98,13 +100,20 @@ def options_text(N, fd="PADE6", filt="COMPACT10", alpha_f=0.49, rk="TVDRK3", cfl
         f"SUBSTEPFILTERING = {'TRUE' if substep_filtering else 'FALSE'}",
         "CHECKSTEP = 1000000", f"DUMPSTEP = {dump_step}",
         "[STATS]", "STATSAVGTYPE = NONE", "VELOCITYSTATS = FALSE", "THERMOSTATS = FALSE",
-        "[BC]", "BCXTYPE = PERIODIC_SOLVE", "BCYTYPE = PERIODIC_SOLVE", "BCZTYPE = PERIODIC_SOLVE",
-        "BCX0 = PERIODIC", "BCX1 = PERIODIC", "BCY0 = PERIODIC", "BCY1 = PERIODIC",
-        "BCZ0 = PERIODIC", "BCZ1 = PERIODIC",
-    ]
+        "[BC]"]
+    for d, a in enumerate("XYZ"):
+        per = fk[2 * d] == "PERIODIC"
+        lines.append(f"BC{a}TYPE = {'PERIODIC_SOLVE' if per else 'DIRICHLET_SOLVE'}")
+        lines += [f"BC{a}0 = {fk[2 * d]}", f"BC{a}1 = {fk[2 * d + 1]}"]
+    if sponge is not None:
+        lines.append("[SPONGE]")
+        lines += [f"{k} = {v}" for k, v in sponge.items()]
+        lines.append("[BC]")
     if disp is None:
         disp = 2 * np.pi * np.eye(3)
     for i, a in enumerate("XYZ"):
+        if fk[2 * i] != "PERIODIC":
+            continue
         for j, b in enumerate("XYZ"):
             lines.append(f"PERIODICDISP{a}_{b} = {float(disp[i][j])!r}")
     lines += [
@@ -116,8 +125,32 @@ def options_text(N, fd="PADE6", filt="COMPACT10", alpha_f=0.49, rk="TVDRK3", cfl
         "[LES]", f"LESMODEL = {les}", f"LESAVERAGING = {les_avg}", f"DSMTESTFILTER = {dsm_filter}",
         f"DSMUSETAUKK = {'TRUE' if use_taukk else 'FALSE'}",
         f"DSMCOEFFRANGE = {'TRUE' if coeff_range else 'FALSE'}",
-    ]
+        f"CLOSUREFALLBACK = {'TRUE' if closure_fallback else 'FALSE'}",
+    ] + ([f"CLOSUREFILE = {closure_file}"] if closure_file else [])
     return "\n".join(lines) + "\n"
+
+
+FACE_NAMES = ("X0", "X1", "Y0", "Y1", "Z0", "Z1")
+_FACE = {"PERIODIC": 0, "INTERNALLY_PERIODIC": 1, "SPONGE": 2, "ADIABATIC_WALL": 3, "CONST_T_WALL": 4,
+         "MOVING_ADIABATIC_WALL": 5, "INLET": 6}
+
+
+def face_kinds(faces):
+    """faces: {"Y0": "ADIABATIC_WALL", ...} (missing faces are PERIODIC) -> six kind names."""
+    faces = faces or {}
+    out = [faces.get(f, "PERIODIC") for f in FACE_NAMES]
+    for d in range(3):
+        if (out[2 * d] == "PERIODIC") != (out[2 * d + 1] == "PERIODIC"):
+            raise ValueError("both faces of a direction must be periodic or neither")
+    return out
+
+
+def box_mesh(N, faces=None, L=2 * np.pi):
+    """Coordinates x = L xi of the HIT box, xi in [0, 1) periodic or [0, 1] otherwise ((N, N, N) arrays)."""
+    fk = face_kinds(faces)
+    axes = [np.arange(N) / (N if fk[2 * d] == "PERIODIC" else N - 1) * L for d in range(3)]
+    z, y, x = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+    return x, y, z
 
 
 _FD = {"CD2": 0, "PADE6": 1, "PENTA10": 2}
@@ -130,14 +163,17 @@ _AVG = {"NONE": 0, "XI1_AVG": 1, "XI2_AVG": 2, "XI3_AVG": 3, "LOCAL": 4, "GLOBAL
 def oracle_config(N, fd="PADE6", filt="COMPACT10", alpha_f=0.49, rk="TVDRK3", cfl=0.5,
                   mu_ref=1.0 / 1600.0, les="NOMODEL", les_avg="NONE", filter_step=1,
                   substep_filtering=False, dsm_filter=1, use_taukk=True, coeff_range=False,
-                  ts_type="CONST_CFL", dt=None, **_):
+                  ts_type="CONST_CFL", dt=None, faces=None, **_):
     """The same case as options_text(), as the field dict of the CPU oracle's config."""
     fds = fd if isinstance(fd, (list, tuple)) else (fd, fd, fd)
-    return dict(Nx=N, Ny=N, Nz=N, bc=(0, 0, 0), fd=tuple(_FD[f] for f in fds),
+    fk = face_kinds(faces)
+    bc = tuple(0 if fk[2 * d] == "PERIODIC" else 1 for d in range(3))
+    sp = [1.0 / N if b == 0 else 1.0 / (N - 1) for b in bc]
+    return dict(Nx=N, Ny=N, Nz=N, bc=bc, face=tuple(_FACE[k] for k in fk), fd=tuple(_FD[f] for f in fds),
                 filter=_FILT[filt], alphaF=alpha_f, rk=_RK[rk], les=_LES[les],
                 les_avg=_AVG[les_avg], dsm_test_filter=dsm_filter, dsm_use_taukk=int(use_taukk),
                 dsm_coeff_range=int(coeff_range), mu_ref=mu_ref,
                 ts_type=1 if ts_type == "CONST_CFL" else 0, cfl=cfl if cfl else 0.0,
                 dt=dt if dt else 0.0, filter_step=filter_step,
-                substep_filtering=int(substep_filtering), dx=1.0 / N, dy=1.0 / N, dz=1.0 / N,
+                substep_filtering=int(substep_filtering), dx=sp[0], dy=sp[1], dz=sp[2],
                 timestep0=0, time0=0.0)

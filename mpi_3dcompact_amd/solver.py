@@ -25,6 +25,12 @@ _SIGS = [
     ("mc3d_solver_sync", ctypes.c_int, [_VP]),
     ("mc3d_solver_destroy", None, [_VP]),
     ("mc3d_solver_timing", ctypes.c_int, [_VP, ctypes.c_int]),
+    ("mc3d_solver_set_closure", ctypes.c_int, [_VP, _VP]),
+    ("mc3d_solver_sample_closure", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_long, ctypes.c_long, ctypes.c_long]),
+    ("mc3d_solver_eval_sgs", ctypes.c_int, [_VP]),
+    ("mc3d_solver_hit_stats", ctypes.c_int, [_VP, _D]),
+    ("mc3d_solver_closure_stats", ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long),
+                                                 ctypes.POINTER(ctypes.c_long)]),
     ("mc3d_solver_timing_report", ctypes.c_int, [_VP, ctypes.c_char_p, ctypes.c_int,
                                                  ctypes.POINTER(ctypes.c_int)]),
 ]
@@ -65,6 +71,34 @@ class Solver:
         """Metrics from mesh coordinates (AbstractSingleBlockMesh::solveForJacobians)."""
         a = [np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, z)]
         check(self._L.mc3d_solver_set_mesh(self._h, *[_p(v) for v in a]))
+
+    def set_closure(self, closure):
+        """Use a trained mpi_3dcompact_amd.closure.Closure (LESMODEL LEARNED)."""
+        check(self._L.mc3d_solver_set_closure(self._h, closure.handle))
+
+    def sample_closure(self, X, T, count, offset=0, stride=1):
+        """Training pairs of the current DSM state into float32 GPU tensors X (>= count, 7), T (>= count, 2)."""
+        if X.shape[0] < count or T.shape[0] < count:
+            raise ValueError("sample buffers too small")
+        check(self._L.mc3d_solver_sample_closure(self._h, X.data_ptr(), T.data_ptr(), count, offset, stride))
+
+    def eval_sgs(self):
+        """Derivatives and SGS fields of the current state (not advanced)."""
+        check(self._L.mc3d_solver_eval_sgs(self._h))
+
+    HIT_STATS = ("time", "tau", "re_lambda", "mach_t", "dissipation", "uprime", "kolmogorov", "rhoprime",
+                 "dilprime", "vortprime", "rho_uu")
+
+    def hit_stats(self):
+        """Decaying-HIT statistics (the reference's turbdata.out columns) as a dict."""
+        out = (ctypes.c_double * 11)()
+        check(self._L.mc3d_solver_hit_stats(self._h, out))
+        return dict(zip(self.HIT_STATS, out[:]))
+
+    def closure_stats(self):
+        a, b, c = ctypes.c_long(), ctypes.c_long(), ctypes.c_long()
+        check(self._L.mc3d_solver_closure_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return dict(evaluations=a.value, fallback_stages=b.value, fallback_points=c.value)
 
     def set_initial(self, ic):
         a = [np.ascontiguousarray(ic[k], dtype=np.float64) for k in ("rho", "U", "V", "W", "p")]

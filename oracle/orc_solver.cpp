@@ -1,6 +1,6 @@
 // CPU oracle of the CurvilinearCSolver time step (TEST INFRASTRUCTURE ONLY).
-// Each block cites the reference lines it restates.  Periodic boundaries only:
-// the wall / sponge branches of src/CurvilinearCSolver_BC.cpp are no-ops there.
+// Each block cites the reference lines it restates, including the wall and
+// sponge handling of src/CurvilinearCSolver_BC.cpp.
 #include "orc_solver.h"
 
 #include <algorithm>
@@ -55,6 +55,137 @@ struct orc_solver {
     int timeStep = 0, rkStep = 0, filterTimeStep = 0, dualStep = 1;
     bool rkLast = false;
     double time = 0.0, dt = 0.0, cfl = 0.0;
+    // sponge (src/SpongeBC.hpp:38-45)
+    bool spongeFlag = false;
+    Vec sigma, spgAvg[5];
+
+    // ---------------- boundary faces ----------------
+    long idx(int i, int j, int k) const { return ((long)k * c.Ny + j) * c.Nx + i; }
+    bool is_wall(int f) const {
+        return c.face[f] == ORC_FACE_ADIABATIC_WALL || c.face[f] == ORC_FACE_CONST_T_WALL ||
+               c.face[f] == ORC_FACE_MOVING_ADIABATIC_WALL;
+    }
+    // fn(ip, in[6]) for every point of face f (0 X0, 1 X1, 2 Y0, 3 Y1, 4 Z0, 5 Z1);
+    // in = the six points inward along the face normal (the FILL_GETMAJIND_YPEN_*
+    // index lists of src/Macros.hpp:175-186, of which calcNeumann reads six)
+    template <class F>
+    void for_face(int f, F fn) {
+        const int Nx = c.Nx, Ny = c.Ny, Nz = c.Nz;
+        const int d = f / 2, hi = f % 2;
+        const int n0 = d == 0 ? Ny : Nx, n1 = d == 2 ? Ny : Nz;
+        for (int b = 0; b < n1; ++b)
+            for (int a = 0; a < n0; ++a) {
+                long ip, in[6];
+                for (int m = 0; m < 6; ++m) {
+                    const int s = hi ? -(m + 1) : (m + 1);
+                    if (d == 0) in[m] = idx(hi ? Nx - 1 + s : s, a, b);
+                    else if (d == 1) in[m] = idx(a, hi ? Ny - 1 + s : s, b);
+                    else in[m] = idx(a, b, hi ? Nz - 1 + s : s);
+                }
+                if (d == 0) ip = idx(hi ? Nx - 1 : 0, a, b);
+                else if (d == 1) ip = idx(a, hi ? Ny - 1 : 0, b);
+                else ip = idx(a, b, hi ? Nz - 1 : 0);
+                fn(ip, in);
+            }
+    }
+    double neumann_T(int f, const long in[6]) const {
+        double v[10] = {};
+        for (int m = 0; m < 6; ++m) v[m] = T[in[m]];
+        return orc_neumann(c.fd[f / 2], v);
+    }
+    void curv(long ip) {
+        Ucurv[ip] = J11[ip] * U[ip] + J12[ip] * V[ip] + J13[ip] * W[ip];
+        Vcurv[ip] = J21[ip] * U[ip] + J22[ip] * V[ip] + J23[ip] * W[ip];
+        Wcurv[ip] = J31[ip] * U[ip] + J32[ip] * V[ip] + J33[ip] * W[ip];
+    }
+
+    // src/CurvilinearCSolver_BC.cpp:63-322: walls at the initial condition.
+    // The static walls come first (X0..Z1), then the moving walls; the wall
+    // velocities are zero (src/CurvilinearCSolver.hpp:219-221).  The X1 face
+    // applies the Neumann temperature for CONST_T_WALL instead of
+    // ADIABATIC_WALL (:103), as the reference does.
+    void wallIC() {
+        bool any = false;
+        for (int f = 0; f < 6; ++f) {
+            const int k = c.face[f];
+            if (k != ORC_FACE_ADIABATIC_WALL && k != ORC_FACE_CONST_T_WALL) continue;
+            any = true;
+            const bool neu = f == 1 ? k == ORC_FACE_CONST_T_WALL : k == ORC_FACE_ADIABATIC_WALL;
+            for_face(f, [&](long ip, const long in[6]) {
+                U[ip] = 0.0; V[ip] = 0.0; W[ip] = 0.0;
+                rhoU1[ip] = 0.0; rhoV1[ip] = 0.0; rhoW1[ip] = 0.0;
+                if (neu) T[ip] = neumann_T(f, in);
+            });
+        }
+        for (int f = 0; f < 6; ++f) {
+            if (c.face[f] != ORC_FACE_MOVING_ADIABATIC_WALL) continue;
+            any = true;
+            for_face(f, [&](long ip, const long in[6]) {
+                U[ip] = 0.0; V[ip] = 0.0; W[ip] = 0.0;
+                rhoU1[ip] = rho1[ip] * 0.0; rhoV1[ip] = rho1[ip] * 0.0; rhoW1[ip] = rho1[ip] * 0.0;
+                T[ip] = neumann_T(f, in);
+            });
+        }
+        if (!any) return;
+        for (long ip = 0; ip < n; ++ip) {  // :311-322
+            p[ip] = rho1[ip] * ig.R_gas * T[ip];
+            sos[ip] = ig.solveSOS(rho1[ip], p[ip]);
+            rhoE1[ip] = ig.solverhoE(rho1[ip], p[ip], U[ip], V[ip], W[ip]);
+            curv(ip);
+        }
+    }
+
+    // src/CurvilinearCSolver_BC.cpp:372-700, at the start of every stage
+    void preStepBCHandling() {
+        const bool first = rkStep == 1;
+        Vec &rhoP = first ? rho1 : rhok, &rhoUP = first ? rhoU1 : rhoUk, &rhoVP = first ? rhoV1 : rhoVk,
+            &rhoWP = first ? rhoW1 : rhoWk, &rhoEP = first ? rhoE1 : rhoEk;
+        auto fix = [&](long ip) {
+            p[ip] = rhoP[ip] * ig.R_gas * T[ip];
+            rhoEP[ip] = ig.solverhoE(rhoP[ip], p[ip], U[ip], V[ip], W[ip]);
+            curv(ip);
+        };
+        for (int f = 0; f < 6; ++f) {
+            const int k = c.face[f];
+            if (k != ORC_FACE_ADIABATIC_WALL && k != ORC_FACE_CONST_T_WALL) continue;
+            for_face(f, [&](long ip, const long in[6]) {
+                if (f >= 2 && k == ORC_FACE_ADIABATIC_WALL) T[ip] = neumann_T(f, in);
+                U[ip] = 0.0; V[ip] = 0.0; W[ip] = 0.0;
+                rhoUP[ip] = 0.0; rhoVP[ip] = 0.0; rhoWP[ip] = 0.0;
+                if (f < 2 && k == ORC_FACE_ADIABATIC_WALL) T[ip] = neumann_T(f, in);
+                fix(ip);
+            });
+        }
+        for (int f = 0; f < 6; ++f) {
+            if (c.face[f] != ORC_FACE_MOVING_ADIABATIC_WALL) continue;
+            for_face(f, [&](long ip, const long in[6]) {
+                if (f >= 2) T[ip] = neumann_T(f, in);
+                U[ip] = 0.0; V[ip] = 0.0; W[ip] = 0.0;
+                rhoUP[ip] = rhoP[ip] * 0.0; rhoVP[ip] = rhoP[ip] * 0.0; rhoWP[ip] = rhoP[ip] * 0.0;
+                if (f < 2) T[ip] = neumann_T(f, in);
+                fix(ip);
+            });
+        }
+    }
+
+    // src/CurvilinearCSolver_BC.cpp:702-842, after the k2 assembly: wall faces
+    // keep only the normal continuity flux, sponge faces keep nothing
+    void postStepBCHandling() {
+        for (int f = 0; f < 6; ++f) {
+            if (!is_wall(f)) continue;
+            const Vec &cont = div[5 * (f / 2)];
+            for_face(f, [&](long ip, const long *) {
+                rhok2[ip] = -dt * J[ip] * cont[ip];
+                rhoUk2[ip] = 0.0; rhoVk2[ip] = 0.0; rhoWk2[ip] = 0.0; rhoEk2[ip] = 0.0;
+            });
+        }
+        for (int f = 0; f < 6; ++f) {
+            if (c.face[f] != ORC_FACE_SPONGE) continue;
+            for_face(f, [&](long ip, const long *) {
+                rhok2[ip] = 0.0; rhoUk2[ip] = 0.0; rhoVk2[ip] = 0.0; rhoWk2[ip] = 0.0; rhoEk2[ip] = 0.0;
+            });
+        }
+    }
 
     // ---------------- operators ----------------
     void deriv(int dir, const Vec &in, Vec &out) const {
@@ -462,17 +593,25 @@ struct orc_solver {
     // src/CurvilinearCSolver_Core.cpp:790-947 (sources are zero; the momentum
     // increments accumulate into the previous k2, as the reference writes them)
     void solveEqnSet() {
+        const bool first = rkStep == 1;
+        const Vec *q[5] = {first ? &rho1 : &rhok, first ? &rhoU1 : &rhoUk, first ? &rhoV1 : &rhoVk,
+                           first ? &rhoW1 : &rhoWk, first ? &rhoE1 : &rhoEk};
 #pragma omp parallel for
         for (long ip = 0; ip < n; ++ip) {
-            const double z = (0.0 + 0.0) / J[ip];
-            rhok2[ip] = dt * J[ip] * (-div[0][ip] - div[5][ip] - div[10][ip] + z);
-            rhoUk2[ip] += -div[1][ip] - div[6][ip] - div[11][ip] + z;
+            // calcSpongeSource = sigma (avg - phi), src/CurvilinearCSolver.hpp:293-295
+            double z[5];
+            for (int e = 0; e < 5; ++e) {
+                const double spg = spongeFlag ? sigma[ip] * (spgAvg[e][ip] - (*q[e])[ip]) : 0.0;
+                z[e] = (spg + 0.0) / J[ip];
+            }
+            rhok2[ip] = dt * J[ip] * (-div[0][ip] - div[5][ip] - div[10][ip] + z[0]);
+            rhoUk2[ip] += -div[1][ip] - div[6][ip] - div[11][ip] + z[1];
             rhoUk2[ip] *= dt * J[ip];
-            rhoVk2[ip] += -div[2][ip] - div[7][ip] - div[12][ip] + z;
+            rhoVk2[ip] += -div[2][ip] - div[7][ip] - div[12][ip] + z[2];
             rhoVk2[ip] *= dt * J[ip];
-            rhoWk2[ip] += -div[3][ip] - div[8][ip] - div[13][ip] + z;
+            rhoWk2[ip] += -div[3][ip] - div[8][ip] - div[13][ip] + z[3];
             rhoWk2[ip] *= dt * J[ip];
-            rhoEk2[ip] = -div[4][ip] - div[9][ip] - div[14][ip] + z;
+            rhoEk2[ip] = -div[4][ip] - div[9][ip] - div[14][ip] + z[4];
             rhoEk2[ip] *= dt * J[ip];
         }
     }
@@ -600,8 +739,10 @@ struct orc_solver {
         for (int s = 1; s <= S; ++s) {
             rkStep = s;
             if (s == S) rkLast = true;
+            preStepBCHandling();
             preStepDerivatives();
             solveEqnSet();
+            postStepBCHandling();
             updateConservedData();
             if (rkLast || c.substep_filtering) filterConservedData();
             updateNonConservedData();
@@ -656,11 +797,65 @@ orc_solver *orc_solver_create(const orc_config *cfg, const double *const metrics
         s->mu[ip] = s->ig.solveMu(s->T[ip]);
         s->sos[ip] = s->ig.solveSOS(s->rho1[ip], s->p[ip]);
     }
+    s->wallIC();
     s->timeStep = cfg->timestep0;
     s->time = cfg->time0;
     s->dt = cfg->dt;
     s->cfl = cfg->cfl;
     return s;
+}
+
+int orc_solver_set_sponge(orc_solver *s, const double *sigma) {
+    s->spongeFlag = true;
+    s->sigma.assign(sigma, sigma + s->n);
+    const Vec *q[5] = {&s->rho1, &s->rhoU1, &s->rhoV1, &s->rhoW1, &s->rhoE1};
+    for (int e = 0; e < 5; ++e) s->spgAvg[e] = *q[e];
+    return 0;
+}
+
+// src/SpongeBC.cpp:60-260: sigma = max over sponge faces of
+// strength (0.068 s^2 + 0.845 s^8), s the normalised depth into the layer
+int orc_sponge_sigma(int Nx, int Ny, int Nz, const double *x, const double *y, const double *z,
+                     const int faces[6], int kind, double strength, const double perc[6], int cyl_orient,
+                     const double axis[3], double rmin, double *sigma) {
+    const long n = (long)Nx * Ny * Nz;
+    const double *X[3] = {x, y, z};
+    double lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {  // msh->x_min / x_max
+        lo[d] = *std::min_element(X[d], X[d] + n);
+        hi[d] = *std::max_element(X[d], X[d] + n);
+    }
+    for (long ip = 0; ip < n; ++ip) sigma[ip] = 0.0;
+    auto ramp = [&](double s) { return strength * (0.068 * std::pow(s, 2.0) + 0.845 * std::pow(s, 8.0)); };
+    if (kind == 0) {
+        for (int f = 0; f < 6; ++f) {
+            if (faces[f] != ORC_FACE_SPONGE) continue;
+            const int d = f / 2;
+            const double L = perc[f] * (hi[d] - lo[d]);
+            for (long ip = 0; ip < n; ++ip) {
+                const double v = X[d][ip];
+                if (f % 2 == 0) {
+                    const double dd = v - lo[d];
+                    if (dd < L) sigma[ip] = std::fmax(ramp((L - dd) / L), sigma[ip]);
+                } else {
+                    const double dd = hi[d] - v;
+                    if (dd < L) sigma[ip] = std::fmax(ramp((v - (hi[d] - L)) / L), sigma[ip]);
+                }
+            }
+        }
+        return 0;
+    }
+    if (cyl_orient != 2) return -1;
+    const double rx = std::fmax(std::fabs(lo[0] - axis[0]), std::fabs(hi[0] - axis[0]));
+    const double ry = std::fmax(std::fabs(lo[1] - axis[1]), std::fabs(hi[1] - axis[1]));
+    const double rmax = std::fmax(rx, ry);
+    if (faces[3] == ORC_FACE_SPONGE) {  // only the Y1 face, :205-219
+        for (long ip = 0; ip < n; ++ip) {
+            const double r = std::sqrt(std::pow(x[ip] - axis[0], 2.0) + std::pow(y[ip] - axis[1], 2.0));
+            if (r > rmin) sigma[ip] = std::fmax(ramp((r - rmin) / (rmax - rmin)), sigma[ip]);
+        }
+    }
+    return 0;
 }
 
 int orc_solver_steps(orc_solver *s, int n) {

@@ -3,7 +3,8 @@
  * see oracle.h).  Restates src/CurvilinearCSolver_Core.cpp:189-1198, the RK
  * schemes (src/TVDRK3.hpp, src/RK4.hpp, src/KenRK4.hpp, src/LSLDDRK4.hpp),
  * src/IdealGas.hpp, src/VremanSGS.hpp, src/DSMSGS.hpp and
- * src/AbstractSGS.cpp for periodic boundaries (no walls, no sponge).
+ * src/AbstractSGS.cpp, with the boundary handling of
+ * src/CurvilinearCSolver_BC.cpp:3-842 (walls, sponge).
  */
 #ifndef MC3D_ORC_SOLVER_H
 #define MC3D_ORC_SOLVER_H
@@ -39,7 +40,13 @@ typedef struct {
     double dx, dy, dz;      /* computational spacings (src/Domain.hpp:48-81) */
     int timestep0;
     double time0;
+    int face[6];            /* face kinds X0 X1 Y0 Y1 Z0 Z1, ORC_FACE_* (0: periodic) */
 } orc_config;
+
+/* face kinds, in the order of src/Options.hpp BCKind */
+enum { ORC_FACE_PERIODIC = 0, ORC_FACE_INTERNALLY_PERIODIC = 1, ORC_FACE_SPONGE = 2,
+       ORC_FACE_ADIABATIC_WALL = 3, ORC_FACE_CONST_T_WALL = 4, ORC_FACE_MOVING_ADIABATIC_WALL = 5,
+       ORC_FACE_INLET = 6 };
 
 /* field ids for orc_solver_field */
 enum {
@@ -72,6 +79,19 @@ void orc_solver_destroy(orc_solver *s);
  * z = lambda dt, run through the same register-update code (dual_step selects
  * the 5- or 6-stage half of LSLDDRK4). */
 int orc_solver_rhs(orc_solver *s, double *const out[5]);
+
+/* Sponge (src/SpongeBC.cpp, src/CurvilinearCSolver_BC.cpp:324-345): strength
+ * field sigma (x fastest) for a solver with SPONGE faces; the sponge averages
+ * start as the current conserved fields.  Call after create, before steps. */
+int orc_solver_set_sponge(orc_solver *s, const double *sigma);
+
+/* sigma of SpongeBC::initRectSpongeBC / initCylSpongeBC (src/SpongeBC.cpp:60-260)
+ * from mesh coordinates: kind 0 rectilinear with perc[6] = RECT{X0,X1,Y0,Y1,Z0,Z1}PERC,
+ * kind 1 cylindrical (orientation 2 only) with axis[3] and rmin; faces[6] as
+ * in orc_config.  Returns 0, or -1 for an unsupported cylinder orientation. */
+int orc_sponge_sigma(int Nx, int Ny, int Nz, const double *x, const double *y, const double *z,
+                     const int faces[6], int kind, double strength, const double perc[6],
+                     int cyl_orient, const double axis[3], double rmin, double *sigma);
 double orc_rk_amplification(int rk, int dual_step, double z);
 
 #ifdef __cplusplus

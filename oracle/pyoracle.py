@@ -204,7 +204,28 @@ class OrcConfig(ctypes.Structure):
                 ("dt", ctypes.c_double), ("filter_step", ctypes.c_int),
                 ("substep_filtering", ctypes.c_int), ("dx", ctypes.c_double),
                 ("dy", ctypes.c_double), ("dz", ctypes.c_double), ("timestep0", ctypes.c_int),
-                ("time0", ctypes.c_double)]
+                ("time0", ctypes.c_double), ("face", ctypes.c_int * 6)]
+
+
+# face kinds (src/Options.hpp BCKind order)
+FACE = {"PERIODIC": 0, "INTERNALLY_PERIODIC": 1, "SPONGE": 2, "ADIABATIC_WALL": 3, "CONST_T_WALL": 4,
+        "MOVING_ADIABATIC_WALL": 5, "INLET": 6}
+
+
+def sponge_sigma(x, y, z, faces, kind=0, strength=1.0, perc=(0,) * 6, cyl_orient=2, axis=(0, 0, 0), rmin=0.0):
+    """Sponge strength field of src/SpongeBC.cpp from mesh coordinates ((Nz, Ny, Nx) arrays)."""
+    L = lib()
+    L.orc_sponge_sigma.restype = ctypes.c_int
+    Nz, Ny, Nx = x.shape
+    xs = [_c(a) for a in (x, y, z)]
+    out = np.empty_like(xs[0])
+    I6, D6, D3 = ctypes.c_int * 6, ctypes.c_double * 6, ctypes.c_double * 3
+    rc = L.orc_sponge_sigma(ctypes.c_int(Nx), ctypes.c_int(Ny), ctypes.c_int(Nz), *[_p(a) for a in xs],
+                            I6(*faces), ctypes.c_int(kind), ctypes.c_double(strength), D6(*perc),
+                            ctypes.c_int(cyl_orient), D3(*axis), ctypes.c_double(rmin), _p(out))
+    if rc:
+        raise ValueError("unsupported sponge configuration")
+    return out
 
 
 def _solver_lib():
@@ -240,7 +261,7 @@ class Solver:
         L = _solver_lib()
         c = OrcConfig()
         for k, v in cfg.items():
-            if k in ("bc", "fd"):
+            if k in ("bc", "fd", "face"):
                 getattr(c, k)[:] = list(v)
             else:
                 setattr(c, k, v)
@@ -250,6 +271,11 @@ class Solver:
         r, u, v, w, p = self._keep[10:]
         self._h = L.orc_solver_create(ctypes.byref(c), mp, _p(r), _p(u), _p(v), _p(w), _p(p))
         self._L = L
+
+    def set_sponge(self, sigma):
+        self._keep.append(_c(sigma))
+        self._L.orc_solver_set_sponge.restype = ctypes.c_int
+        self._L.orc_solver_set_sponge(ctypes.c_void_p(self._h), _p(self._keep[-1]))
 
     def steps(self, n):
         self._L.orc_solver_steps(self._h, n)

@@ -1,12 +1,14 @@
 """GPU metric terms (mc3d_mesh_metrics and the solver's set_mesh) against the
 CPU oracle of AbstractSingleBlockMesh::solveForJacobians on the same meshes.
 
-Tolerance: max |gpu - oracle| <= 1e-11 * scale, scale = max |J| for J and
+Tolerance: max |gpu - oracle| <= tol * scale, scale = max |J| for J and
 the largest |J_ij| over all nine for each J_ij (some are zero).  The
 metrics nest three derivatives (x -> V = x_xi y -> J_ij / J -> x J_i), so
-rounding differences of the line solves (1e-16 relative) grow by about
-1 / d(xi) per level; 1e-11 leaves room for that at these sizes while an index,
-sign, wrap-slot or translation mistake is O(1).
+rounding differences of the line solves (1e-16 relative) grow by about the
+operator norm per level: ~1 / d(xi) for periodic lines (tol 1e-11) and
+several times that at the one-sided PENTA10 closures (tol 1e-9, the bound the
+oracle's own affine-exactness check needs there).  An index, sign, wrap-slot
+or translation mistake is O(1).
 """
 import sys
 
@@ -64,6 +66,10 @@ def max_rel_errs(got, ref):
     return [np.max(np.abs(g - r)) / (sj if k == 0 else sm) for k, (g, r) in enumerate(zip(got, ref))]
 
 
+def tol_for(bc):
+    return TOL if all(b == PER for b in bc) else 1e-9
+
+
 def compare(torch, fd, bc, X, trans, label):
     from mpi_3dcompact_amd.mesh import mesh_metrics
     N = X[0].shape[::-1]
@@ -73,7 +79,7 @@ def compare(torch, fd, bc, X, trans, label):
     got = [o.cpu().numpy() for o in mesh_metrics(fd, bc, *dev, trans)]
     errs = max_rel_errs(got, ref)
     _say("   max rel err per field:", " ".join(f"{e:.1e}" for e in errs))
-    assert max(errs) <= TOL, errs
+    assert max(errs) <= tol_for(bc), errs
     return got, ref
 
 

@@ -78,8 +78,11 @@ def test_plain_hit_64_tvdrk3_100_steps(gpu):
          les_avg="NONE", mu_ref=5e-4, filter_step=2, substep_filtering=True),
     dict(fd="PADE6", filt="COMPACT10", rk="TVDRK3", les="NOMODEL", mu_ref=1e-3, ts_type="CONST_DT",
          dt=2e-3, cfl=None),
+    dict(fd="PADE6", filt="COMPACT10", rk="RK4", les="VREMAN", les_avg="XI3_AVG", mu_ref=1e-3),
+    dict(fd="PENTA10", filt="COMPACT8", rk="TVDRK3", les="VREMAN", les_avg="XI1_AVG", mu_ref=1e-3),
 ], ids=["penta10-c8-kenrk4-vreman-local", "pade6-c10-rk4", "penta10-c10-lsldd-vreman-global",
-        "cd2-c8-tvdrk3-vreman-xi2", "mixed-substepfilter", "const-dt"])
+        "cd2-c8-tvdrk3-vreman-xi2", "mixed-substepfilter", "const-dt", "pade6-c10-rk4-vreman-xi3",
+        "penta10-c8-tvdrk3-vreman-xi1"])
 def test_scheme_combinations_32(gpu, case):
     _say(f"case: 32^3 {case}, 12 steps")
     g, o = run_pair(32, 12, seed=3, k0=3.0, **case)
@@ -187,3 +190,45 @@ def test_full_les_256_properties(gpu):
     ke = (0.5 * g.field("rhoU") ** 2 / g.field("rho")).sum()
     assert 0.5 * ke0 < ke < ke0
     assert np.isfinite(g.field("mu_sgs")).all()
+
+
+def test_hit_stats_match_numpy(gpu):
+    """Device HIT statistics against the same formulas evaluated in NumPy on the
+    solver's own fields (tests/hit/Samtaney_D4/MPI_3DCompact.cpp:413-556)."""
+    from mpi_3dcompact_amd.solver import Solver
+    N = 32
+    g = Solver(hit.options_text(N, fd="PENTA10", filt="COMPACT8", rk="KENRK4"), (N, N, N))
+    g.set_metrics(hit.identity_metrics(N))
+    g.set_initial(hit.hit_initial_condition(N, k0=4.0, Mt=0.3, seed=8))
+    g.step(3)
+    st = g.hit_stats()
+    J = {k: g.field(k) for k in ("J11", "J12", "J13", "J21", "J22", "J23", "J31", "J32", "J33")}
+    G = np.empty((N, N, N, 3, 3))
+    for i, u in enumerate("UVW"):
+        d = [g.field(f"d{u}{k}") for k in range(3)]
+        for j in range(3):
+            G[..., i, j] = J[f"J1{j + 1}"] * d[0] + J[f"J2{j + 1}"] * d[1] + J[f"J3{j + 1}"] * d[2]
+    # the derivative fields are those of the current velocities
+    ref_dU0 = P.deriv_field(P.PENTA10, 0, P.PERIODIC, g.field("U"), 1.0 / N)
+    assert np.max(np.abs(g.field("dU0") - ref_dU0)) <= 1e-12 * np.max(np.abs(ref_dU0))
+    rho, mu, sos = g.field("rho"), g.field("mu"), g.field("sos")
+    U, V, W = g.field("U"), g.field("V"), g.field("W")
+    dil = G[..., 0, 0] + G[..., 1, 1] + G[..., 2, 2]
+    S = 0.5 * (G + np.swapaxes(G, -1, -2)) - dil[..., None, None] / 3 * np.eye(3)
+    diss = 2 * mu * np.sum(S * G, axis=(-2, -1))
+    om2 = (G[..., 2, 1] - G[..., 1, 2]) ** 2 + (G[..., 0, 2] - G[..., 2, 0]) ** 2 + (G[..., 1, 0] - G[..., 0, 1]) ** 2
+    uu = U * U + V * V + W * W
+    mrho, mmu = rho.mean(), mu.mean()
+    eps = diss.mean() / mrho
+    up = np.sqrt((uu / 3).mean())
+    uip = np.sqrt(((G[..., 0, 0] ** 2 + G[..., 1, 1] ** 2 + G[..., 2, 2] ** 2) / 3).mean())
+    nu = mmu / mrho
+    ref = dict(tau=np.sqrt(2 * np.pi) / 8.0 / up, re_lambda=up * (up / uip) / nu,
+               mach_t=np.sqrt(uu.mean()) / sos.mean(), dissipation=eps, uprime=up,
+               kolmogorov=(nu ** 3 / eps) ** 0.25, rhoprime=np.sqrt(((rho - mrho) ** 2).mean()),
+               dilprime=np.sqrt((dil ** 2).mean()), vortprime=np.sqrt(om2.mean()), rho_uu=(rho * uu).mean())
+    _say("  hit stats:", {k: f"{v:.6e}" for k, v in st.items()})
+    for k, v in ref.items():
+        assert abs(st[k] - v) <= 1e-10 * abs(v), (k, st[k], v)
+    t, _, _ = g.scalars()
+    assert st["time"] == t
