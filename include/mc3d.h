@@ -209,8 +209,10 @@ int mc3d_solver_timing_report(mc3d_solver *s, char *buf, int buflen, int *needed
  *   outputs  C = y0 out_std0 + out_mean0, D = y1 out_std1 + out_mean1 (D = C / Pr_t);
  *            mu_sgs = rho C Delta^2 |S|, k_sgs = cp D rho Delta^2 |S|
  *            (the DSM's own forms, src/DSMSGS.hpp:319,406-408,440).
- * A point whose raw features leave [lo, hi] takes the DSM's mu_sgs and k_sgs
- * on that stage when the fallback is on.
+ * A point whose raw features leave [lo, hi] is evaluated on inputs clamped to
+ * the range; when more than CLOSUREFALLBACKFRACTION of the points of a stage
+ * are out and the fallback is on, the DSM runs on that stage and its mu_sgs
+ * and k_sgs replace the network's at those points.
  *
  * Parameters: 8962 FP32 values, W1[64][7] b1[64] W2[64][64] b2[64] W3[64][64]
  * b3[64] W4[2][64] b4[2].  Training data: X[n][7] standardised inputs and
@@ -252,9 +254,12 @@ int mc3d_solver_sample_closure(mc3d_solver *s, float *X, float *T, long count, l
 /* derivatives and SGS fields (mu_sgs, k_sgs) of the current state without
  * advancing it (NOMODEL, VREMAN, LEARNED; the DSM modifies the pressure) */
 int mc3d_solver_eval_sgs(mc3d_solver *s);
-/* closure evaluations, stages that fell back to the DSM, and points that did */
-int mc3d_solver_closure_stats(mc3d_solver *s, long *evaluations, long *fallback_stages,
-                              long *fallback_points);
+/* closure evaluations (stages), points whose inputs left the trusted range
+ * (clamped to it), stages that fell back to the DSM (more than
+ * [LES] CLOSUREFALLBACKFRACTION of the points out, default 1e-3), and the
+ * points of those stages */
+int mc3d_solver_closure_stats(mc3d_solver *s, long *evaluations, long *outside_points,
+                              long *fallback_stages, long *fallback_points);
 
 /* ---------------------------------------------------------------------------
  * SolutionDump files (host only, no GPU): five doubles {timeStep, time, Nx,

@@ -82,6 +82,10 @@ def standardise(X, T, torch):
     T = torch.minimum(torch.maximum(T, lo_t), hi_t)
     xm, xs = X.double().mean(0), X.double().std(0)
     tm, ts = T.double().mean(0), T.double().std(0)
+    # a feature without spread (log Delta on a uniform grid) is passed through
+    # unscaled; its trusted range is then the single value it took
+    xs = torch.where(xs > 0, xs, torch.ones_like(xs))
+    ts = torch.where(ts > 0, ts, torch.ones_like(ts))
     xmin, xmax = X.min(0).values.double(), X.max(0).values.double()
     span = xmax - xmin
     sc = dict(in_mean=xm.cpu().numpy(), in_std=xs.cpu().numpy(), out_mean=tm.cpu().numpy(),
@@ -138,7 +142,8 @@ def main(argv=None):
     ap.add_argument("--mu", type=float, default=1e-4)
     ap.add_argument("--Mt", type=float, default=0.3)
     ap.add_argument("--k0", type=float, default=4.0)
-    ap.add_argument("--avg", default="LOCAL", help="LESAVERAGING of the DSM (and of its fallback)")
+    ap.add_argument("--avg", default="LOCAL", help="LESAVERAGING of the DSM (and of its fallback); the "
+                    "LOCAL-averaged DSM diverges after a few hundred steps at 256^3 (XI1_AVG runs 1000)")
     ap.add_argument("--train-seed", type=int, default=1)
     ap.add_argument("--eval-seed", type=int, default=0)
     ap.add_argument("--gen-steps", type=int, default=400)

@@ -56,8 +56,9 @@ hipError_t launch_mlp_infer(const float *params, const float *X, float *Y, long 
 // features, network, then
 //     mu_sgs = rho C Delta^2 |S|,  k_sgs = cp (C / Pr_t) rho Delta^2 |S|,
 // Delta = (dxi deta dzeta / J)^(1/3).  flag[p] = 1 where a raw feature lies
-// outside [lo, hi] (mu_sgs, k_sgs are still written there); blockcount[b] =
-// number of flagged points of workgroup b.
+// outside [lo, hi]; such inputs are clamped to the range for the network (and
+// mu_sgs, k_sgs are written there too); blockcount[b] = number of flagged
+// points of workgroup b.
 struct ClosureArgs {
     const double *dU[3], *dV[3], *dW[3];
     Metrics m;

@@ -554,8 +554,11 @@ __global__ __launch_bounds__(256) void k_learned_sgs(ClosureArgs a, long n) {
             features(A, vol_J, f, Smag);
 #pragma unroll
             for (int k = 0; k < kMlpIn; ++k) {
-                out = out || !(f[k] >= a.sc.lo[k] && f[k] <= a.sc.hi[k]);
-                x[k] = (f[k] - a.sc.in_mean[k]) / a.sc.in_std[k];
+                const bool in = f[k] >= a.sc.lo[k] && f[k] <= a.sc.hi[k];
+                out = out || !in;
+                // outside the trusted range the input is clamped to it
+                const float fc = in ? f[k] : (f[k] < a.sc.lo[k] ? a.sc.lo[k] : a.sc.hi[k]);
+                x[k] = (fc - a.sc.in_mean[k]) / a.sc.in_std[k];
             }
             const double d = cbrt(vol_J);
             base = (a.rho[p] * (d * d)) * Smag;

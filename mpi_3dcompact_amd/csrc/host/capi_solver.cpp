@@ -288,11 +288,13 @@ int mc3d_solver_eval_sgs(mc3d_solver *s) {
     return guarded([&] { s->cs->evalSGS(); });
 }
 
-int mc3d_solver_closure_stats(mc3d_solver *s, long *evaluations, long *fallback_stages, long *fallback_points) {
+int mc3d_solver_closure_stats(mc3d_solver *s, long *evaluations, long *outside_points, long *fallback_stages,
+                              long *fallback_points) {
     if (!s) return mc3d::fail(MC3D_EINVAL, "null solver");
     auto *l = dynamic_cast<mc3d::LearnedSGS *>(s->cs->les.get());
     if (!l) return mc3d::fail(MC3D_ECONFIG, "LES.LESMODEL is not LEARNED");
     if (evaluations) *evaluations = l->evaluations;
+    if (outside_points) *outside_points = l->outside_points;
     if (fallback_stages) *fallback_stages = l->fallback_stages;
     if (fallback_points) *fallback_points = l->fallback_points;
     return MC3D_OK;

@@ -237,7 +237,8 @@ void LearnedSGS::getSGSViscosity(const double *const dU[3], const double *const 
     int n_out = 0;
     hip_check(hipMemcpyAsync(&n_out, count_, sizeof(int), hipMemcpyDeviceToHost, cs->st), "closure count");
     hip_check(hipStreamSynchronize(cs->st), "closure count");
-    if (n_out == 0) return;
+    outside_points += n_out;
+    if (n_out == 0 || (double)n_out <= cs->opt.closureFallbackFraction * (double)cs->N) return;
     fallback_points += n_out;
     ++fallback_stages;
     // the DSM on this stage (without tau_kk, which the closure does not model),

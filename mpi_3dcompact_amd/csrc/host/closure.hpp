@@ -66,7 +66,9 @@ class LearnedSGS : public AbstractSGS {
     void set_network(const Closure &c);
     bool has_network() const { return ready_; }
     bool fallback = true;
-    long fallback_points = 0, fallback_stages = 0, evaluations = 0;
+    // points outside the trusted range (all stages), and the stages and points
+    // that fell back to the DSM (more than CLOSUREFALLBACKFRACTION of the points out)
+    long outside_points = 0, fallback_points = 0, fallback_stages = 0, evaluations = 0;
 
   private:
     float *params_ = nullptr;

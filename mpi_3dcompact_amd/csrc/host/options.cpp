@@ -199,6 +199,7 @@ int Options::parse(const std::string &text, Options &o, std::string &err) {
     r.opt("LES.DSMCOEFFRANGE", o.dumpCoeffRange);
     r.opt("LES.CLOSUREFILE", o.closureFile);
     r.opt("LES.CLOSUREFALLBACK", o.closureFallback);
+    r.opt("LES.CLOSUREFALLBACKFRACTION", o.closureFallbackFraction);
     if (!r.ok) return MC3D_ECONFIG;
     // src/Options.hpp:241 (checked after the sizes are known; with a restart
     // they come from the file header and are checked when it is read)

@@ -61,6 +61,7 @@ struct Options {
     // by mc3d_closure_save, and whether out-of-range points fall back to the DSM
     std::string closureFile;
     bool closureFallback = true;
+    double closureFallbackFraction = 1e-3;  // DSM stage when more points than this leave the range
 
     // Parses INI text.  Returns 0, or a negative code with a message.
     static int parse(const std::string &text, Options &out, std::string &err);

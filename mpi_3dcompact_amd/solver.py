@@ -29,8 +29,7 @@ _SIGS = [
     ("mc3d_solver_sample_closure", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_long, ctypes.c_long, ctypes.c_long]),
     ("mc3d_solver_eval_sgs", ctypes.c_int, [_VP]),
     ("mc3d_solver_hit_stats", ctypes.c_int, [_VP, _D]),
-    ("mc3d_solver_closure_stats", ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long),
-                                                 ctypes.POINTER(ctypes.c_long)]),
+    ("mc3d_solver_closure_stats", ctypes.c_int, [_VP] + [ctypes.POINTER(ctypes.c_long)] * 4),
     ("mc3d_solver_timing_report", ctypes.c_int, [_VP, ctypes.c_char_p, ctypes.c_int,
                                                  ctypes.POINTER(ctypes.c_int)]),
 ]
@@ -96,9 +95,9 @@ class Solver:
         return dict(zip(self.HIT_STATS, out[:]))
 
     def closure_stats(self):
-        a, b, c = ctypes.c_long(), ctypes.c_long(), ctypes.c_long()
-        check(self._L.mc3d_solver_closure_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
-        return dict(evaluations=a.value, fallback_stages=b.value, fallback_points=c.value)
+        v = [ctypes.c_long() for _ in range(4)]
+        check(self._L.mc3d_solver_closure_stats(self._h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("evaluations", "outside_points", "fallback_stages", "fallback_points"), [x.value for x in v]))
 
     def set_initial(self, ic):
         a = [np.ascontiguousarray(ic[k], dtype=np.float64) for k in ("rho", "U", "V", "W", "p")]

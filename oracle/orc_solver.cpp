@@ -805,11 +805,14 @@ orc_solver *orc_solver_create(const orc_config *cfg, const double *const metrics
     return s;
 }
 
-int orc_solver_set_sponge(orc_solver *s, const double *sigma) {
+int orc_solver_set_sponge(orc_solver *s, const double *sigma, const double *const avg[5]) {
     s->spongeFlag = true;
     s->sigma.assign(sigma, sigma + s->n);
     const Vec *q[5] = {&s->rho1, &s->rhoU1, &s->rhoV1, &s->rhoW1, &s->rhoE1};
-    for (int e = 0; e < 5; ++e) s->spgAvg[e] = *q[e];
+    for (int e = 0; e < 5; ++e) {
+        if (avg) s->spgAvg[e].assign(avg[e], avg[e] + s->n);
+        else s->spgAvg[e] = *q[e];
+    }
     return 0;
 }
 

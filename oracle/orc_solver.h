@@ -82,8 +82,9 @@ int orc_solver_rhs(orc_solver *s, double *const out[5]);
 
 /* Sponge (src/SpongeBC.cpp, src/CurvilinearCSolver_BC.cpp:324-345): strength
  * field sigma (x fastest) for a solver with SPONGE faces; the sponge averages
- * start as the current conserved fields.  Call after create, before steps. */
-int orc_solver_set_sponge(orc_solver *s, const double *sigma);
+ * are avg[5] (rho, rhoU, rhoV, rhoW, rhoE), or the current conserved fields
+ * when avg is NULL (the reference's start).  Call after create, before steps. */
+int orc_solver_set_sponge(orc_solver *s, const double *sigma, const double *const avg[5]);
 
 /* sigma of SpongeBC::initRectSpongeBC / initCylSpongeBC (src/SpongeBC.cpp:60-260)
  * from mesh coordinates: kind 0 rectilinear with perc[6] = RECT{X0,X1,Y0,Y1,Z0,Z1}PERC,

@@ -272,10 +272,16 @@ class Solver:
         self._h = L.orc_solver_create(ctypes.byref(c), mp, _p(r), _p(u), _p(v), _p(w), _p(p))
         self._L = L
 
-    def set_sponge(self, sigma):
+    def set_sponge(self, sigma, avg=None):
+        """sigma field; avg: dict rho, rhoU, rhoV, rhoW, rhoE (default: the current state)."""
         self._keep.append(_c(sigma))
         self._L.orc_solver_set_sponge.restype = ctypes.c_int
-        self._L.orc_solver_set_sponge(ctypes.c_void_p(self._h), _p(self._keep[-1]))
+        arr = None
+        if avg is not None:
+            a = [_c(avg[k]) for k in ("rho", "rhoU", "rhoV", "rhoW", "rhoE")]
+            self._keep += a
+            arr = (_D * 5)(*[_p(v) for v in a])
+        self._L.orc_solver_set_sponge(ctypes.c_void_p(self._h), _p(self._keep[-6 if avg is not None else -1]), arr)
 
     def steps(self, n):
         self._L.orc_solver_steps(self._h, n)
