@@ -100,7 +100,8 @@ int mc3d_lineop_info(const mc3d_lineop *op, int *chunks, int *chunk_rows, int *l
 const char *mc3d_lineop_kernel(const mc3d_lineop *op);
 
 /* ---------------------------------------------------------------------------
- * Whole solver step (CurvilinearCSolver + RK + SGS), configured by the
+ * Whole solver step (CurvilinearCSolver + RK + SGS + wall and sponge
+ * boundaries, src/CurvilinearCSolver_BC.cpp), configured by the
  * reference's options file text (MPI_3DCompact.in: sections [TIMESTEPPING],
  * [BC], [DOMAIN], [SOLVER], [RESTART], [STATS], [LES], ...).
  * Replaces the construction sequence of src/MPI_3DCompact.cpp:34-316
@@ -126,6 +127,13 @@ int mc3d_solver_set_metrics(mc3d_solver *s, const double *const metrics[10]);
 int mc3d_mesh_metrics(const int fd[3], const int bc[3], const int dims[3], const double trans[9],
                       const double *x, const double *y, const double *z, double *const out[10],
                       void *stream);
+/* Sponge strength sigma (host, x fastest) for a solver with SPONGE faces, as
+ * SpongeBC builds it (src/SpongeBC.cpp:60-260); mc3d_solver_set_mesh computes
+ * it from the mesh itself.  avg (host rho, rhoU, rhoV, rhoW, rhoE; NULL: the
+ * initial conserved fields, src/CurvilinearCSolver_BC.cpp:324-345) are the
+ * states the sponge relaxes towards.  Required before stepping when a face is
+ * a SPONGE and no mesh was given. */
+int mc3d_solver_set_sponge(mc3d_solver *s, const double *sigma, const double *const avg[5]);
 /* Form the ten metric fields on the GPU from host mesh coordinates x, y, z
  * (x fastest), as AbstractSingleBlockMesh::solveForJacobians does
  * (src/AbstractSingleBlockMesh.cpp:23-1309): conservative form, periodic
@@ -133,7 +141,8 @@ int mc3d_mesh_metrics(const int fd[3], const int bc[3], const int dims[3], const
  * shifted by [BC] PERIODICDISP<d>_<e> (handlePeriodicStuff, :2019-2085).  The
  * computational coordinates run over [0, 1] in each direction.  Used instead of
  * mc3d_solver_set_metrics; CD2 on a periodic direction is MC3D_ECONFIG (the
- * reference has no such path either, :94-97). */
+ * reference has no such path either, :94-97).  With SPONGE faces the sponge
+ * strength is formed from the same mesh (see mc3d_solver_set_sponge). */
 int mc3d_solver_set_mesh(mc3d_solver *s, const double *x, const double *y, const double *z);
 /* primitive initial condition; performs setInitialConditions
  * (src/CurvilinearCSolver_BC.cpp:3-63). */

@@ -17,7 +17,7 @@
 struct mc3d_solver {
     hipStream_t stream = nullptr;
     mc3d::CurvilinearCSolver *cs = nullptr;
-    bool metrics = false, initial = false;
+    bool metrics = false, initial = false, sponge = false;
 };
 
 using mc3d::guarded;
@@ -69,7 +69,11 @@ int mc3d_solver_set_metrics(mc3d_solver *s, const double *const metrics[10]) {
 
 int mc3d_solver_set_mesh(mc3d_solver *s, const double *x, const double *y, const double *z) {
     if (!s || !x || !y || !z) return mc3d::fail(MC3D_EINVAL, "null argument");
-    return guarded([&] { s->cs->set_mesh_host(x, y, z); s->metrics = true; });
+    return guarded([&] {
+        s->cs->set_mesh_host(x, y, z);
+        s->metrics = true;
+        if (s->cs->hasSponge()) s->sponge = true;
+    });
 }
 
 int mc3d_mesh_metrics(const int fd[3], const int bc[3], const int dims[3], const double trans[9],
@@ -171,9 +175,29 @@ int mc3d_dump_write(const char *path, int timestep, double time, const int dims[
     });
 }
 
-int mc3d_solver_step(mc3d_solver *s, int nsteps) {
+namespace {
+int ready(const mc3d_solver *s) {
     if (!s) return mc3d::fail(MC3D_EINVAL, "null solver");
     if (!s->initial) return mc3d::fail(MC3D_EINVAL, "initial condition not set");
+    if (s->cs->hasSponge() && !s->sponge)
+        return mc3d::fail(MC3D_ECONFIG, "SPONGE faces need the sponge strength: mc3d_solver_set_mesh or "
+                                        "mc3d_solver_set_sponge");
+    return MC3D_OK;
+}
+}  // namespace
+
+int mc3d_solver_set_sponge(mc3d_solver *s, const double *sigma, const double *const avg[5]) {
+    if (!s || !sigma) return mc3d::fail(MC3D_EINVAL, "null argument");
+    if (avg)
+        for (int q = 0; q < 5; ++q)
+            if (!avg[q]) return mc3d::fail(MC3D_EINVAL, "null sponge average");
+    const int rc = guarded([&] { s->cs->setSponge(sigma, avg); });
+    if (rc == MC3D_OK) s->sponge = true;
+    return rc;
+}
+
+int mc3d_solver_step(mc3d_solver *s, int nsteps) {
+    if (const int rc = ready(s)) return rc;
     if (nsteps < 0) return mc3d::fail(MC3D_EINVAL, "negative step count");
     return guarded([&] {
         for (int i = 0; i < nsteps; ++i) s->cs->rk->step();
@@ -181,8 +205,7 @@ int mc3d_solver_step(mc3d_solver *s, int nsteps) {
 }
 
 int mc3d_solver_run(mc3d_solver *s) {
-    if (!s) return mc3d::fail(MC3D_EINVAL, "null solver");
-    if (!s->initial) return mc3d::fail(MC3D_EINVAL, "initial condition not set");
+    if (const int rc = ready(s)) return rc;
     return guarded([&] { s->cs->rk->executeSolverLoop(); });
 }
 

@@ -171,6 +171,32 @@ int Options::parse(const std::string &text, Options &o, std::string &err) {
             r.fail(std::string("periodic face kinds in a Dirichlet solve of ") + ax[d]);
         }
     }
+    // [SPONGE] (src/Options.hpp:313-362): read when a face is a sponge
+    {
+        const BCKind faces[6] = {o.bcX0, o.bcX1, o.bcY0, o.bcY1, o.bcZ0, o.bcZ1};
+        bool any = false;
+        for (BCKind f : faces) any = any || f == SPONGE;
+        if (any) {
+            r.enm("SPONGE.KIND", o.spongeKind, {{"RECTILINEAR", O::RECTILINEAR}, {"CYLINDRICAL", O::CYLINDRICAL}});
+            r.opt("SPONGE.AVGTIME", o.spongeAvgT);
+            r.opt("SPONGE.PINF", o.spongeP);
+            r.opt("SPONGE.STRENGTH", o.spongeStrength);
+            if (o.spongeKind == RECTILINEAR) {
+                const char *key[6] = {"SPONGE.RECTX0PERC", "SPONGE.RECTX1PERC", "SPONGE.RECTY0PERC",
+                                      "SPONGE.RECTY1PERC", "SPONGE.RECTZ0PERC", "SPONGE.RECTZ1PERC"};
+                for (int f = 0; f < 6; ++f)
+                    if (faces[f] == SPONGE) r.req(key[f], o.spongeRectPerc[f]);
+            } else {
+                r.req("SPONGE.CYLAXIS_ORIENT", o.spongeCylAxisOrient);
+                if (o.spongeCylAxisOrient < 0 || o.spongeCylAxisOrient > 2)
+                    r.fail("sponge cylinder orientation must be 0, 1 or 2");
+                r.req("SPONGE.CYLAXIS_X", o.spongeCylAxis[0]);
+                r.req("SPONGE.CYLAXIS_Y", o.spongeCylAxis[1]);
+                r.req("SPONGE.CYLAXIS_Z", o.spongeCylAxis[2]);
+                r.req("SPONGE.RMIN", o.rMin);
+            }
+        }
+    }
     r.req("RESTART.FROMRESTART", o.fromRestart);
     r.req("RESTART.ONLYGRIDFROMRESTART", o.onlyGridFromRestart);
     if (o.fromRestart || o.onlyGridFromRestart) r.req("RESTART.FILENAME", o.filename);

@@ -33,6 +33,9 @@ struct Options {
     // [SPONGE]
     SpongeKind spongeKind = RECTILINEAR;
     double spongeP = 1.0 / 1.4, spongeAvgT = 1.0, spongeStrength = 1.0;
+    double spongeRectPerc[6] = {0, 0, 0, 0, 0, 0};  // RECT{X0,X1,Y0,Y1,Z0,Z1}PERC
+    int spongeCylAxisOrient = 2;
+    double spongeCylAxis[3] = {0, 0, 0}, rMin = 0.0;
     // [DOMAIN]
     int Nx = 0, Ny = 0, Nz = 0;
     // [PENCILDECOMP]

@@ -424,10 +424,9 @@ CurvilinearCSolver::CurvilinearCSolver(const Options &o, hipStream_t s)
     ig.mu_ref = opt.mu_ref;
     ig.R_gas = p_ref / rho_ref / ig.T_ref;
     ig.cp = ig.R_gas * ig.gamma / (ig.gamma - 1.0);
-    if (opt.bcXType != Options::PERIODIC_SOLVE || opt.bcYType != Options::PERIODIC_SOLVE ||
-        opt.bcZType != Options::PERIODIC_SOLVE)
-        throw Error(MC3D_ECONFIG, "the solver step supports periodic boundaries in all directions; "
-                                  "wall and sponge boundaries are not implemented");
+    if (opt.spongeFromRestart)
+        throw Error(MC3D_ECONFIG, "RESTART.SPONGEAVGFROMRESTART is not supported (the averages start "
+                                  "from the initial state)");
     // field slab
     const int nfields = 10 + 20 + 10 + 12 + 15 + 5 + 10 + 8;
     slab_bytes_ = sizeof(double) * (size_t)N * nfields + 4096;
@@ -453,6 +452,7 @@ CurvilinearCSolver::CurvilinearCSolver(const Options &o, hipStream_t s)
     StepScalars s0{opt.dt, opt.CFL, opt.time, 0.0};
     hip_check(hipMemcpyAsync(sc, &s0, sizeof(s0), hipMemcpyHostToDevice, st), "scalar init");
 
+    setupBoundaries();
     // src/CurvilinearCSolver.hpp:162-217
     const Options::BCType bt[3] = {opt.bcXType, opt.bcYType, opt.bcZType};
     const Options::FDType ft[3] = {opt.xFDType, opt.yFDType, opt.zFDType};
@@ -514,7 +514,8 @@ void CurvilinearCSolver::set_metrics_host(const double *const m[10]) {
     hip_check(hipStreamSynchronize(st), "metric upload");
 }
 
-// src/CurvilinearCSolver_BC.cpp:3-63 (periodic: no wall / sponge branches)
+// src/CurvilinearCSolver_BC.cpp:3-63, then the wall initial fixes (:63-322) and
+// the sponge averages (:324-345)
 void CurvilinearCSolver::setInitialConditions_host(const double *rho0, const double *U0,
                                                    const double *V0, const double *W0,
                                                    const double *p0) {
@@ -523,6 +524,8 @@ void CurvilinearCSolver::setInitialConditions_host(const double *rho0, const dou
     for (int k = 0; k < 5; ++k)
         hip_check(hipMemcpyAsync(d[k], src[k], sizeof(double) * N, hipMemcpyHostToDevice, st), "IC upload");
     hip_check(launch_init_state(d[0], d[1], d[2], d[3], d[4], r1, pr, msh, ig, N, st), "initial state");
+    wallInitialConditions();
+    initSpongeAverages();
     hip_check(hipStreamSynchronize(st), "initial state");
 }
 
@@ -580,6 +583,7 @@ void CurvilinearCSolver::set_mesh_host(const double *x, const double *y, const d
     for (int k = 0; k < 3; ++k)
         hip_check(hipMemcpyAsync(d + k * N, h[k], sizeof(double) * N, hipMemcpyHostToDevice, st), "mesh upload");
     solveForJacobians(d, d + N, d + 2 * N);
+    setSpongeFromMesh(x, y, z);
 }
 
 // src/MPI_3DCompact.cpp:214-271: rho from the file, U = rhoU / rho (IdealGas
@@ -750,6 +754,7 @@ void CurvilinearCSolver::hitStats(double out[11]) {
 }
 
 void CurvilinearCSolver::preSubStep() {
+    preStepBCHandling();
     gradientsAndSGS();
     flux();
 }
@@ -810,6 +815,14 @@ void CurvilinearCSolver::flux() {
     derivX->calc1stDerivFields(5, p1, div, false);
     derivY->calc1stDerivFields(5, p2, div, true);
     derivZ->calc1stDerivFields(5, p3, div, true);
+    // the wall rows keep the continuity flux of their own direction alone (:702-760)
+    AbstractDerivatives *dv[3] = {derivX.get(), derivY.get(), derivZ.get()};
+    for (int d = 0; d < 3; ++d)
+        if (cont_[d]) {
+            const double *in[1] = {pre[5 * d]};
+            double *out[1] = {cont_[d]};
+            dv[d]->calc1stDerivFields(1, in, out, false);
+        }
 }
 
 void CurvilinearCSolver::stage_update(int op, double a, double b, double c) {
@@ -826,6 +839,7 @@ void CurvilinearCSolver::stage_update(int op, double a, double b, double c) {
     ra.eos = (writes_rk && !rkLast && !opt.subStepFiltering) ? 1 : 0;
     ra.pr = pr;
     ra.g = ig;
+    ra.bc = rkBoundaryArgs();
     eos_done_ = ra.eos != 0;
     // fields: div (5) + J + previous momentum k2 (3) read, k2 (5) written, plus the
     // registers, plus U V W p T mu sos written when the update rides along
@@ -981,6 +995,10 @@ double *CurvilinearCSolver::field_by_name(const std::string &n) {
         const int d = n[2] - '0';
         return n[1] == 'U' ? dU[d] : n[1] == 'V' ? dV[d] : n[1] == 'W' ? dW[d] : dT[d];
     }
+    if (n == "sigma") return sigma_;
+    if (n.size() > 3 && n.compare(n.size() - 3, 3, "Avg") == 0)  // sponge averages rhoAvg .. rhoEAvg
+        for (int q = 0; q < 5; ++q)
+            if (n == std::string(cons[q]) + "Avg") return spgAvg_[q];
     static const char *met[10] = {"J", "J11", "J12", "J13", "J21", "J22", "J23", "J31", "J32", "J33"};
     for (int k = 0; k < 10; ++k)
         if (n == met[k]) return Jm[k];

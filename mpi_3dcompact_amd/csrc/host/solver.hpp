@@ -342,6 +342,19 @@ class CurvilinearCSolver {
     const Cons &state() const { return rkStep == 1 ? r1 : rk_; }
     double *field_by_name(const std::string &name);
 
+    // boundary handling (src/CurvilinearCSolver_BC.cpp, src/SpongeBC.cpp)
+    void setupBoundaries();
+    void wallInitialConditions();   // :63-322
+    void preStepBCHandling();       // :372-700
+    RkBc rkBoundaryArgs() const;    // :702-842 and the sponge source, applied in the stage kernel
+    void initSpongeAverages();      // :324-345 (from the current state unless given)
+    // sponge strength sigma: from host mesh coordinates (SpongeBC::initRectSpongeBC /
+    // initCylSpongeBC) or given; averages optional (null: the current state)
+    void setSpongeFromMesh(const double *x, const double *y, const double *z);
+    void setSponge(const double *sigma_host, const double *const avg_host[5]);
+    bool hasSponge() const { return sponge_; }
+    double *sigmaField() const { return sigma_; }
+    double *spongeAvg(int q) const { return spgAvg_[q]; }
     // learned closure: training pairs (features, DSM coefficients) of the
     // current state; the DSM runs on it and the fields its tau_kk update
     // changes are restored afterwards
@@ -361,6 +374,11 @@ class CurvilinearCSolver {
     std::string out_dir_;
     std::vector<double> mesh_[3];
     bool eos_done_ = false;  // the last stage kernel also updated the non-conserved data
+    // boundaries
+    signed char role_[6] = {0, 0, 0, 0, 0, 0};
+    bool walls_ = false, sponge_ = false, sigma_set_ = false, avg_given_ = false;
+    double *cont_[3] = {nullptr, nullptr, nullptr};
+    double *sigma_ = nullptr, *spgAvg_[5] = {};
     void *slab_ = nullptr;
     size_t slab_bytes_ = 0, slab_used_ = 0;
     std::vector<void *> extra_;

@@ -36,6 +36,25 @@ enum RkOp {
     kLsFirst, kLsMid, kLsLast         // 2N low-storage (KenRK4, LSLDDRK4)
 };
 
+// face ids X0 X1 Y0 Y1 Z0 Z1 = 0..5; a face of direction d = f / 2, the high end when f % 2
+enum FaceRole : signed char { kFaceNone = 0, kFaceWall = 1, kFaceSponge = 2 };
+
+// Post-stage face handling and the sponge source, applied inside the RK stage
+// kernel: walls keep only the normal continuity flux, k2_rho = -dt J cont_d
+// (faces in the order X0..Z1), sponge faces keep nothing (:702-842); the
+// sponge source sigma (avg - q) / J enters every equation (:790-947,
+// src/CurvilinearCSolver.hpp:293-295).
+struct RkBc {
+    int active;                // any wall or sponge face
+    signed char role[6];       // FaceRole per face
+    int Nx, Ny, Nz;
+    const double *cont[3];     // d/dxi_d of the direction-d continuity pre-flux (wall directions)
+    int sponge;                // sponge source on
+    const double *sigma;
+    const double *avg[5];
+    Cons qs;                   // stage state the sponge relaxes
+};
+
 struct RkArgs {
     Cons r1, rk, k2, r2;
     const double *div[5];  // summed flux divergences (cont, mom1..3, engy)
@@ -46,6 +65,7 @@ struct RkArgs {
     int eos;               // also update the non-conserved data from the new rk state
     Prim pr;
     Gas g;
+    RkBc bc;
 };
 
 struct FluxArgs {

@@ -233,27 +233,48 @@ __global__ __launch_bounds__(kBlock) void k_rhs_rk(RkArgs a, long n) {
     if (ip >= n) return;
     const double J = a.J[ip];
     const double dtJ = a.sc->dt * J;
-    const double z = (0.0 + 0.0) / J;
-    double nq[5];
-    // continuity
-    {
-        const double k2 = dtJ * (-a.div[0][ip] + z);
-        a.k2.q[0][ip] = k2;
-        nq[0] = rk_update(a, ip, 0, k2);
+    double z[5];
+    if (a.bc.sponge) {  // calcSpongeSource = sigma (avg - phi), src/CurvilinearCSolver.hpp:293-295
+        const double sg = a.bc.sigma[ip];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) z[q] = (sg * (a.bc.avg[q][ip] - a.bc.qs.q[q][ip]) + 0.0) / J;
+    } else {
+        const double z0 = (0.0 + 0.0) / J;
+#pragma unroll
+        for (int q = 0; q < 5; ++q) z[q] = z0;
     }
+    double k2[5];
+    k2[0] = dtJ * (-a.div[0][ip] + z[0]);
 #pragma unroll
     for (int q = 1; q < 4; ++q) {
-        double *kk = a.k2.q[q];
-        double k2 = kk[ip] + (-a.div[q][ip] + z);
-        k2 = k2 * dtJ;
-        kk[ip] = k2;
-        nq[q] = rk_update(a, ip, q, k2);
+        double v = a.k2.q[q][ip] + (-a.div[q][ip] + z[q]);
+        k2[q] = v * dtJ;
     }
     {
-        double k2 = -a.div[4][ip] + z;
-        k2 = k2 * dtJ;
-        a.k2.q[4][ip] = k2;
-        nq[4] = rk_update(a, ip, 4, k2);
+        double v = -a.div[4][ip] + z[4];
+        k2[4] = v * dtJ;
+    }
+    if (a.bc.active) {  // postStepBCHandling, src/CurvilinearCSolver_BC.cpp:702-842
+        const long NxNy = (long)a.bc.Nx * a.bc.Ny;
+        const int i = (int)(ip % a.bc.Nx), j = (int)((ip / a.bc.Nx) % a.bc.Ny), k = (int)(ip / NxNy);
+        const bool on[6] = {i == 0, i == a.bc.Nx - 1, j == 0, j == a.bc.Ny - 1, k == 0, k == a.bc.Nz - 1};
+#pragma unroll
+        for (int f = 0; f < 6; ++f)
+            if (on[f] && a.bc.role[f] == kFaceWall) {
+                k2[0] = -a.sc->dt * J * a.bc.cont[f / 2][ip];
+                k2[1] = 0.0; k2[2] = 0.0; k2[3] = 0.0; k2[4] = 0.0;
+            }
+#pragma unroll
+        for (int f = 0; f < 6; ++f)
+            if (on[f] && a.bc.role[f] == kFaceSponge) {
+                k2[0] = 0.0; k2[1] = 0.0; k2[2] = 0.0; k2[3] = 0.0; k2[4] = 0.0;
+            }
+    }
+    double nq[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        a.k2.q[q][ip] = k2[q];
+        nq[q] = rk_update(a, ip, q, k2[q]);
     }
     if (a.eos) {  // as k_eos
         const Gas &g = a.g;

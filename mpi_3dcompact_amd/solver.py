@@ -13,6 +13,7 @@ _SIGS = [
     ("mc3d_solver_set_metrics", ctypes.c_int, [_VP, ctypes.POINTER(_D)]),
     ("mc3d_solver_set_mesh", ctypes.c_int, [_VP, _D, _D, _D]),
     ("mc3d_solver_set_initial", ctypes.c_int, [_VP, _D, _D, _D, _D, _D]),
+    ("mc3d_solver_set_sponge", ctypes.c_int, [_VP, _D, ctypes.POINTER(_D)]),
     ("mc3d_solver_step", ctypes.c_int, [_VP, ctypes.c_int]),
     ("mc3d_solver_load_restart", ctypes.c_int, [_VP]),
     ("mc3d_solver_write_dump", ctypes.c_int, [_VP, ctypes.c_char_p, _D, _D, _D]),
@@ -70,6 +71,18 @@ class Solver:
         """Metrics from mesh coordinates (AbstractSingleBlockMesh::solveForJacobians)."""
         a = [np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, z)]
         check(self._L.mc3d_solver_set_mesh(self._h, *[_p(v) for v in a]))
+
+    def set_sponge(self, sigma, avg=None):
+        """Sponge strength field; avg: dict rho, rhoU, rhoV, rhoW, rhoE of the sponge
+        averages (default: the state at set_initial)."""
+        sg = np.ascontiguousarray(sigma, dtype=np.float64)
+        arr = None
+        keep = [sg]
+        if avg is not None:
+            a = [np.ascontiguousarray(avg[k], dtype=np.float64) for k in ("rho", "rhoU", "rhoV", "rhoW", "rhoE")]
+            keep += a
+            arr = (_D * 5)(*[_p(v) for v in a])
+        check(self._L.mc3d_solver_set_sponge(self._h, _p(sg), arr))
 
     def set_closure(self, closure):
         """Use a trained mpi_3dcompact_amd.closure.Closure (LESMODEL LEARNED)."""
