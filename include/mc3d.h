@@ -294,6 +294,48 @@ int mc3d_plan_tables(int cls, int kind, int bc, int n, double dd, double alphaF,
                      int sizes[10], double *lu, double *sp, double *G, double *rows,
                      double *drows, double *interior, int *cstart, int *clen);
 
+/* ---------------------------------------------------------------------------
+ * Ensembles (no counterpart in the reference, which runs one box per job):
+ * many independent periodic boxes of uneven sizes, each a run of the solver
+ * configured by the options text (schemes, filter, RK, CFL, LES model NOMODEL or
+ * VREMAN with LESAVERAGING NONE / LOCAL), advanced together.  Members of equal
+ * size share every launch (stacked along z in device memory); each keeps its
+ * own dt from its own CFL reduction, its own time and its own MU_REF.  From
+ * the second step on, one step of the whole ensemble is a replayed HIP graph.
+ * A member's results equal those of mc3d_solver on the same inputs with the
+ * same line-operator launch shapes (bit for bit), and differ by rounding when
+ * tuning chose other shapes.  Host arrays are x fastest, one member's box.
+ */
+typedef struct mc3d_ensemble mc3d_ensemble;
+
+/* dims: 3 per member (nx, ny, nz; each at least 8); mu_ref: per member (NULL:
+ * SOLVER.MU_REF of the options).  The options' NX/NY/NZ are not used. */
+int mc3d_ensemble_create(const char *options_text, int members, const int *dims, const double *mu_ref,
+                         mc3d_ensemble **out);
+int mc3d_ensemble_set_metrics(mc3d_ensemble *e, int member, const double *const metrics[10]);
+/* before the first step only */
+int mc3d_ensemble_set_initial(mc3d_ensemble *e, int member, const double *rho0, const double *U0,
+                              const double *V0, const double *W0, const double *p0);
+/* Measure and select, per box size and direction, the launch shape of the
+ * derivative and filter operators (rows per chunk, kernel family, lines per
+ * workgroup).  cache_path (NULL: none) is a text file of earlier results, read
+ * first and written back with the new ones.  Synchronous. */
+int mc3d_ensemble_tune(mc3d_ensemble *e, const char *cache_path);
+int mc3d_ensemble_step(mc3d_ensemble *e, int nsteps);
+/* graphs on (default) or off: off issues every step's launches directly */
+int mc3d_ensemble_use_graphs(mc3d_ensemble *e, int on);
+/* same names as mc3d_solver_get_field */
+int mc3d_ensemble_get_field(mc3d_ensemble *e, int member, const char *name, double *host_out);
+int mc3d_ensemble_scalars(mc3d_ensemble *e, int member, double *time, double *dt, int *timestep);
+/* the ensemble's HIP stream (as void*): every step's work is ordered on it */
+int mc3d_ensemble_stream(mc3d_ensemble *e, void **stream);
+int mc3d_ensemble_sync(mc3d_ensemble *e);
+/* JSON: buckets (box size, members, kernels of the derivative and filter
+ * operators per direction, tuning results), graphs captured, graph launches.
+ * MC3D_EINVAL if buf is too small (the required size goes to *needed). */
+int mc3d_ensemble_info(mc3d_ensemble *e, char *buf, int buflen, int *needed);
+void mc3d_ensemble_destroy(mc3d_ensemble *e);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,6 +6,7 @@
 #include <string>
 
 #include "capi_common.h"
+#include "lineop_api.h"
 #include "lineop_launch.h"
 #include "lineop_plan.h"
 #include "mc3d.h"
@@ -15,6 +16,7 @@ struct mc3d_lineop {
     mc3d::OpDev dev{};
     int cls = 0, kind = 0, dir = 0, bc = 0;
     int dims[3] = {0, 0, 0};
+    int nbatch = 1;         // members stacked along z
     void *dmem = nullptr;
     int device = 0;
     bool fast = false;      // plain applications use the fast kernels
@@ -53,7 +55,7 @@ int upload(mc3d_lineop *op) {
 }
 
 int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[3], double dd,
-           double alphaF, mc3d_lineop **out) {
+           double alphaF, int nbatch, const mc3d::LineTune &tune, mc3d_lineop **out) {
     if (!out) return mc3d::fail(MC3D_EINVAL, "null output handle");
     *out = nullptr;
     if (!dims || dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0)
@@ -61,8 +63,13 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
     if (dir < 0 || dir > 2) return mc3d::fail(MC3D_EINVAL, "dir must be 0, 1 or 2");
     if (bc != MC3D_PERIODIC && bc != MC3D_DIRICHLET) return mc3d::fail(MC3D_EINVAL, "bad boundary type");
     if (cls == mc3d::kDeriv && !(dd > 0.0)) return mc3d::fail(MC3D_EINVAL, "dd must be positive");
-    const long npts = (long)dims[0] * dims[1] * dims[2];
+    if (nbatch < 1) return mc3d::fail(MC3D_EINVAL, "nbatch must be positive");
+    const long npts = (long)dims[0] * dims[1] * dims[2] * nbatch;
     if (npts > (long)2147483647) return mc3d::fail(MC3D_EINVAL, "field larger than 2^31 points");
+    if (tune.mm != 0 && tune.mm != 16 && tune.mm != 32) return mc3d::fail(MC3D_EINVAL, "chunk rows must be 16 or 32");
+    if (tune.family < 0 || tune.family > 2) return mc3d::fail(MC3D_EINVAL, "bad kernel family");
+    if (tune.lines < 0 || (tune.lines & (tune.lines - 1)) != 0)
+        return mc3d::fail(MC3D_EINVAL, "lines per workgroup must be a power of two");
     int dev = 0, lds_limit = 65536;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
@@ -71,7 +78,8 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
     std::string err;
     int rc = 0;
     // 32-row chunks only pay off in the fast kernels; otherwise plan 16-row chunks
-    for (int mm : {0, 16}) {
+    const int mms[2] = {tune.mm, tune.mm ? tune.mm : 16};
+    for (int mm : mms) {
         mc3d::PlanRequest rq{cls, kind, bc, dims[dir], dd, alphaF, mm};
         op->plan = mc3d::Plan();
         rc = mc3d::build_plan(rq, op->plan, err);
@@ -103,14 +111,31 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
         // with 32-bit byte offsets
         const bool offsets_ok = npts * 8 < 2147483648L;
         op->fast = p.uniform && rows_ok && offsets_ok && mc3d::lineop_fast_supported(d, (size_t)lds_limit);
-        if (op->fast || p.MM == 16) break;
+        if (tune.family == 2) op->fast = false;
+        if (op->fast || p.MM == 16 || tune.mm) break;
     }
     const mc3d::Plan &p = op->plan;
+    if (tune.family == 1 && !op->fast) {
+        delete op;
+        return mc3d::fail(MC3D_EINVAL, "the fast kernels do not take this line operator and chunking");
+    }
+    if (nbatch > 1 && (op->dev.passA || op->dev.passB)) {
+        delete op;
+        return mc3d::fail(MC3D_EINVAL, "stacked members need periodic lines in the other directions");
+    }
+    if (tune.lines) {
+        op->dev.L = op->plan.L = tune.lines;
+        if (tune.lines * p.P > 256) {
+            delete op;
+            return mc3d::fail(MC3D_EINVAL, "lines per workgroup times chunks exceeds 256 lanes");
+        }
+    }
     if (mc3d::lineop_lds_bytes(op->dev) > 65536) {
         delete op;
         return mc3d::fail(MC3D_EINVAL, "line too long for the LDS tile");
     }
     op->cls = cls; op->kind = kind; op->dir = dir; op->bc = bc;
+    op->nbatch = nbatch;
     std::memcpy(op->dims, dims, sizeof(op->dims));
     const std::string per = p.periodic ? "true" : "false";
     if (op->fast)
@@ -141,8 +166,11 @@ int apply(const mc3d_lineop *op, const double *const *in, double *const *out, in
         a.out[f] = out[f];
     }
     for (int k = 0; k < 6; ++k) a.wrap[k] = wrap ? wrap[k] : nullptr;
+    if (mode == mc3d::kModeTPB && op->nbatch > 1)
+        return mc3d::fail(MC3D_EINVAL, "transformed-periodic derivatives take one member");
     a.Nx = op->dims[0]; a.Ny = op->dims[1]; a.Nz = op->dims[2];
-    a.nlines = (int)((long)a.Nx * a.Ny * a.Nz / op->dims[op->dir]);
+    a.nbatch = op->nbatch;
+    a.nlines = (int)((long)a.Nx * a.Ny * a.Nz * op->nbatch / op->dims[op->dir]);
     a.accumulate = accumulate;
     const hipStream_t st = static_cast<hipStream_t>(stream);
     hipError_t e = (op->fast && mode != mc3d::kModeTPB) ? mc3d::lineop_launch_fast(op->dev, a, nfields, st)
@@ -157,7 +185,7 @@ extern "C" {
 
 int mc3d_deriv_create(int scheme, int dir, int bc, const int dims[3], double dd, mc3d_lineop **out) {
     if (scheme < MC3D_CD2 || scheme > MC3D_PENTA10) return mc3d::fail(MC3D_EINVAL, "unknown derivative scheme");
-    return create(mc3d::kDeriv, scheme, dir, bc, nullptr, dims, dd, 0.0, out);
+    return create(mc3d::kDeriv, scheme, dir, bc, nullptr, dims, dd, 0.0, 1, mc3d::LineTune{}, out);
 }
 
 int mc3d_deriv_apply(const mc3d_lineop *op, const double *const *in, double *const *out, int nfields,
@@ -187,7 +215,7 @@ int mc3d_filter_create(int kind, double alphaF, int dir, const int bc3[3], const
     if (kind < MC3D_COMPACT8 || kind > MC3D_COMMN5EXP) return mc3d::fail(MC3D_EINVAL, "unknown filter kind");
     if (!bc3) return mc3d::fail(MC3D_EINVAL, "null bc3");
     if (dir < 0 || dir > 2) return mc3d::fail(MC3D_EINVAL, "dir must be 0, 1 or 2");
-    return create(mc3d::kFilter, kind, dir, bc3[dir], bc3, dims, 1.0, alphaF, out);
+    return create(mc3d::kFilter, kind, dir, bc3[dir], bc3, dims, 1.0, alphaF, 1, mc3d::LineTune{}, out);
 }
 
 int mc3d_filter_apply(const mc3d_lineop *op, const double *const *in, double *const *out, int nfields,
@@ -249,6 +277,26 @@ int mc3d_plan_tables(int cls, int kind, int bc, int n, double dd, double alphaF,
 // internal accessors used by the solver
 namespace mc3d {
 const char *lineop_kernel_name(const mc3d_lineop *op) { return op->kname.c_str(); }
+
+int lineop_create(int cls, int kind, int dir, const int bc3[3], const int dims[3], double dd, double alphaF,
+                  int nbatch, const LineTune &tune, mc3d_lineop **out) {
+    if (!bc3) return fail(MC3D_EINVAL, "null bc3");
+    if (dir < 0 || dir > 2) return fail(MC3D_EINVAL, "dir must be 0, 1 or 2");
+    if (cls == kDeriv) {
+        if (kind < MC3D_CD2 || kind > MC3D_PENTA10) return fail(MC3D_EINVAL, "unknown derivative scheme");
+        return create(cls, kind, dir, bc3[dir], nullptr, dims, dd, 0.0, nbatch, tune, out);
+    }
+    if (kind < MC3D_COMPACT8 || kind > MC3D_COMMN5EXP) return fail(MC3D_EINVAL, "unknown filter kind");
+    return create(cls, kind, dir, bc3[dir], bc3, dims, 1.0, alphaF, nbatch, tune, out);
+}
+
+LineTune lineop_shape(const mc3d_lineop *op) {
+    LineTune t;
+    t.mm = op->plan.MM;
+    t.family = op->fast ? 1 : 2;
+    t.lines = op->fast ? 0 : op->plan.L;
+    return t;
+}
 
 int lineop_launch_accumulate(const mc3d_lineop *op, const double *const *in, double *const *out,
                              int nfields, int accumulate, void *stream) {

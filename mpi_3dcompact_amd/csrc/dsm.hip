@@ -137,7 +137,7 @@ __global__ __launch_bounds__(kBlock) void k_dsm_taukk_eos(const double *rho, con
     const double T = p / (r * g.R_gas);
     pr.p[ip] = p;
     pr.T[ip] = T;
-    pr.mu[ip] = g.mu_ref * pow(T / g.T_ref, 0.76);
+    pr.mu[ip] = member_mu_ref(g, ip) * pow(T / g.T_ref, 0.76);
     pr.sos[ip] = sqrt(g.gamma * p / r);
 }
 

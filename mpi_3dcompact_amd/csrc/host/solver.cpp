@@ -41,7 +41,7 @@ Domain::Domain(const Options &o) {
 
 // ---------------------------------------------------------------------------
 LineDerivative::LineDerivative(int scheme, const Domain &d, Options::BCType bc, Direct dir,
-                               hipStream_t st)
+                               hipStream_t st, const LineTune &shape)
     : scheme_(scheme), st_(st) {
     Nx = d.gNx; Ny = d.gNy; Nz = d.gNz;
     currentDir = dir;
@@ -51,8 +51,9 @@ LineDerivative::LineDerivative(int scheme, const Domain &d, Options::BCType bc, 
     lhsBandwidth = scheme == MC3D_PENTA10 ? BW5 : scheme == MC3D_PADE6 ? BW3 : BW3;
     rhsBandwidth = scheme == MC3D_PENTA10 ? BW7 : scheme == MC3D_PADE6 ? BW5 : BW3;
     const int dims[3] = {Nx, Ny, Nz};
-    mc3d_check(mc3d_deriv_create(scheme, (int)dir, bc == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET,
-                                 dims, dd, &op_),
+    const int b = bc == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET;
+    const int bc3[3] = {b, b, b};
+    mc3d_check(lineop_create(kDeriv, scheme, (int)dir, bc3, dims, dd, 0.0, d.members, shape, &op_),
                "derivative setup");
 }
 
@@ -91,14 +92,15 @@ void LineDerivative::calc1stDerivFields(int n, const double *const *in, double *
 }
 
 LineFilter::LineFilter(int kind, double alphaF, const Domain &d, const int bc3[3],
-                       AbstractDerivatives::Direct dir, hipStream_t st)
+                       AbstractDerivatives::Direct dir, hipStream_t st, const LineTune &shape)
     : st_(st) {
     Nx = d.gNx; Ny = d.gNy; Nz = d.gNz;
     currentDir = dir;
     bcType = bc3[dir] == MC3D_PERIODIC ? Options::PERIODIC_SOLVE : Options::DIRICHLET_SOLVE;
     N = dir == AbstractDerivatives::DIRX ? Nx : dir == AbstractDerivatives::DIRY ? Ny : Nz;
     const int dims[3] = {Nx, Ny, Nz};
-    mc3d_check(mc3d_filter_create(kind, alphaF, (int)dir, bc3, dims, &op_), "filter setup");
+    mc3d_check(lineop_create(kFilter, kind, (int)dir, bc3, dims, 1.0, alphaF, d.members, shape, &op_),
+               "filter setup");
 }
 
 LineFilter::~LineFilter() { mc3d_lineop_destroy(op_); }
@@ -322,6 +324,21 @@ void AbstractRK::executeSolverLoop() {
     while (!cs->endFlag) step();
 }
 
+void AbstractRK::step() {
+    advance();
+    cs->postStep();
+}
+
+// src/TVDRK3.hpp:135-176, src/RK4.hpp:167-218, src/KenRK4.hpp:154-215,
+// src/LSLDDRK4.hpp:324-460: preStep, the stages, then (in step) postStep
+void AbstractRK::advance() {
+    const int S = stages();
+    cs->rkLast = false;
+    cs->preStep();
+    for (int s = 1; s <= S; ++s) cs->run_stage(this, s, S);
+    finishStep();
+}
+
 void CurvilinearCSolver::run_stage(AbstractRK *r, int s, int S) {
     rkStep = s;
     if (s == S) rkLast = true;
@@ -332,13 +349,7 @@ void CurvilinearCSolver::run_stage(AbstractRK *r, int s, int S) {
     updateData();
 }
 
-// src/TVDRK3.hpp:28-37 coefficients, :50-176 stages
-void TVDRK3::step() {
-    cs->rkLast = false;
-    cs->preStep();
-    for (int s = 1; s <= 3; ++s) cs->run_stage(this, s, 3);
-    cs->postStep();
-}
+// src/TVDRK3.hpp:28-37 coefficients, :50-133 stages
 void TVDRK3::updateConservedData() {
     switch (cs->rkStep) {
         case 1: cs->stage_update(kTvd1, 0.0, 0.0, 1.0); break;
@@ -347,13 +358,7 @@ void TVDRK3::updateConservedData() {
     }
 }
 
-// src/RK4.hpp:27-34, :47-178
-void RK4::step() {
-    cs->rkLast = false;
-    cs->preStep();
-    for (int s = 1; s <= 4; ++s) cs->run_stage(this, s, 4);
-    cs->postStep();
-}
+// src/RK4.hpp:27-34, :47-165
 void RK4::updateConservedData() {
     const double a[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
     const double b[3] = {1.0 / 2.0, 1.0 / 2.0, 1.0};
@@ -379,12 +384,6 @@ static void lowstorage_update(CurvilinearCSolver *cs, const double *A, const dou
     else cs->stage_update(kLsLast, A[s - 1], B[s - 1], 0.0);
 }
 
-void KenRK4::step() {
-    cs->rkLast = false;
-    cs->preStep();
-    for (int s = 1; s <= 5; ++s) cs->run_stage(this, s, 5);
-    cs->postStep();
-}
 void KenRK4::updateConservedData() { lowstorage_update(cs, kKenA, kKenB, 5); }
 
 // src/LSLDDRK4.hpp:40-78, :93-460
@@ -397,14 +396,6 @@ static const double kLs2A[6] = {0.0, -0.44127377153877382565, -1.073982008079781
 static const double kLs2B[6] = {0.11584888181285561688, 0.37287699051652864918, 0.73795368921435295698,
                                 0.57981109366311039538, 1.031284991300145194, 0.15};
 
-void LSLDDRK4::step() {
-    const int S = stages();
-    cs->rkLast = false;
-    cs->preStep();
-    for (int s = 1; s <= S; ++s) cs->run_stage(this, s, S);
-    cs->postStep();
-    rkDualStepStep = rkDualStepStep == 1 ? 2 : 1;
-}
 void LSLDDRK4::updateConservedData() {
     if (rkDualStepStep == 1) lowstorage_update(cs, kLs1A, kLs1B, 5);
     else lowstorage_update(cs, kLs2A, kLs2B, 6);
@@ -412,9 +403,22 @@ void LSLDDRK4::updateConservedData() {
 
 // ---------------------------------------------------------------------------
 // src/CurvilinearCSolver.hpp:120-245
-CurvilinearCSolver::CurvilinearCSolver(const Options &o, hipStream_t s)
+CurvilinearCSolver::CurvilinearCSolver(const Options &o, hipStream_t s, int members, const double *mu_members)
     : opt(o), dom(o), st(s), timer(s) {
-    N = dom.gN;
+    if (members < 1) throw Error(MC3D_EINVAL, "member count must be positive");
+    dom.members = members;
+    N = dom.gN * members;
+    if (members > 1) {
+        const bool periodic = opt.bcXType == Options::PERIODIC_SOLVE && opt.bcYType == Options::PERIODIC_SOLVE &&
+                              opt.bcZType == Options::PERIODIC_SOLVE;
+        if (!periodic) throw Error(MC3D_ECONFIG, "stacked members must be periodic boxes");
+        if (opt.lesModel == Options::DSM || opt.lesModel == Options::LEARNED)
+            throw Error(MC3D_ECONFIG, "stacked members take no LES model or VREMAN");
+        if (opt.lesModel == Options::VREMAN && opt.musgsAvgType != Options::NONE &&
+            opt.musgsAvgType != Options::LOCAL)
+            throw Error(MC3D_ECONFIG, "stacked members take LESAVERAGING NONE or LOCAL");
+        if (N > 268435455L) throw Error(MC3D_EINVAL, "stacked members above 2^28 points per field");
+    }
     timeStep = opt.timeStep;
     // src/IdealGas.hpp:35-48
     ig.gamma = 1.4;
@@ -424,6 +428,8 @@ CurvilinearCSolver::CurvilinearCSolver(const Options &o, hipStream_t s)
     ig.mu_ref = opt.mu_ref;
     ig.R_gas = p_ref / rho_ref / ig.T_ref;
     ig.cp = ig.R_gas * ig.gamma / (ig.gamma - 1.0);
+    ig.mu_member = nullptr;
+    ig.member_pts = 0;
     if (opt.spongeFromRestart)
         throw Error(MC3D_ECONFIG, "RESTART.SPONGEAVGFROMRESTART is not supported (the averages start "
                                   "from the initial state)");
@@ -443,14 +449,26 @@ CurvilinearCSolver::CurvilinearCSolver(const Options &o, hipStream_t s)
     // zero everything: k2 registers must start at zero (momentum increments accumulate)
     hip_check(hipMemsetAsync(slab_, 0, slab_bytes_, st), "field clear");
     void *p = nullptr;
-    hip_check(hipMalloc(&p, sizeof(double) * 4096), "scratch allocation");
+    hip_check(hipMalloc(&p, sizeof(double) * std::max<size_t>(4096, (size_t)cfl_blocks(dom.gN) * members)),
+              "scratch allocation");
     extra_.push_back(p);
     cfl_partial = static_cast<double *>(p);
-    hip_check(hipMalloc(&p, sizeof(StepScalars)), "scalar allocation");
+    hip_check(hipMalloc(&p, sizeof(StepScalars) * members), "scalar allocation");
     extra_.push_back(p);
     sc = static_cast<StepScalars *>(p);
-    StepScalars s0{opt.dt, opt.CFL, opt.time, 0.0};
-    hip_check(hipMemcpyAsync(sc, &s0, sizeof(s0), hipMemcpyHostToDevice, st), "scalar init");
+    const std::vector<StepScalars> s0((size_t)members, StepScalars{opt.dt, opt.CFL, opt.time, 0.0});
+    hip_check(hipMemcpy(sc, s0.data(), sizeof(StepScalars) * members, hipMemcpyHostToDevice), "scalar init");
+    if (members > 1) {
+        // per-member MU_REF (the options' value for every member unless given)
+        std::vector<double> mu((size_t)members, opt.mu_ref);
+        if (mu_members) mu.assign(mu_members, mu_members + members);
+        hip_check(hipMalloc(&p, sizeof(double) * members), "member allocation");
+        extra_.push_back(p);
+        hip_check(hipMemcpy(p, mu.data(), sizeof(double) * members, hipMemcpyHostToDevice), "member upload");
+        ig.mu_member = static_cast<const double *>(p);
+        ig.member_pts = (unsigned)dom.gN;
+        mu_members_ = mu;
+    }
 
     setupBoundaries();
     // src/CurvilinearCSolver.hpp:162-217
@@ -527,6 +545,97 @@ void CurvilinearCSolver::setInitialConditions_host(const double *rho0, const dou
     wallInitialConditions();
     initSpongeAverages();
     hip_check(hipStreamSynchronize(st), "initial state");
+}
+
+// member b of the stacked boxes: the initial conversion of its slice with its
+// own MU_REF (the same kernel as the whole-field form)
+void CurvilinearCSolver::setMemberInitial(int b, const double *rho0, const double *U0, const double *V0,
+                                          const double *W0, const double *p0) {
+    if (b < 0 || b >= dom.members) throw Error(MC3D_EINVAL, "member index out of range");
+    if (dom.members == 1) return setInitialConditions_host(rho0, U0, V0, W0, p0);
+    const long off = (long)b * dom.gN;
+    const double *src[5] = {rho0, U0, V0, W0, p0};
+    for (int k = 0; k < 5; ++k)
+        hip_check(hipMemcpyAsync(tmp[k] + off, src[k], sizeof(double) * dom.gN, hipMemcpyHostToDevice, st),
+                  "IC upload");
+    Cons r{};
+    for (int q = 0; q < 5; ++q) r.q[q] = r1.q[q] + off;
+    Prim p{pr.U + off, pr.V + off, pr.W + off, pr.p + off, pr.T + off,
+           pr.mu + off, pr.sos + off, pr.Ucurv + off, pr.Vcurv + off, pr.Wcurv + off};
+    Metrics m{msh.J + off, msh.J11 + off, msh.J12 + off, msh.J13 + off, msh.J21 + off,
+              msh.J22 + off, msh.J23 + off, msh.J31 + off, msh.J32 + off, msh.J33 + off};
+    Gas g = ig;
+    g.mu_member = nullptr;
+    g.member_pts = 0;
+    g.mu_ref = mu_members_[(size_t)b];
+    hip_check(launch_init_state(tmp[0] + off, tmp[1] + off, tmp[2] + off, tmp[3] + off, tmp[4] + off, r, p, m, g,
+                                dom.gN, st),
+              "initial state");
+    hip_check(hipStreamSynchronize(st), "initial state");
+}
+
+void CurvilinearCSolver::setMemberMetrics(int b, const double *const m[10]) {
+    if (b < 0 || b >= dom.members) throw Error(MC3D_EINVAL, "member index out of range");
+    const long off = (long)b * dom.gN;
+    for (int k = 0; k < 10; ++k)
+        hip_check(hipMemcpyAsync(Jm[k] + off, m[k], sizeof(double) * dom.gN, hipMemcpyHostToDevice, st),
+                  "metric upload");
+    hip_check(hipStreamSynchronize(st), "metric upload");
+}
+
+double *CurvilinearCSolver::memberField(int b, const std::string &name) {
+    if (b < 0 || b >= dom.members) throw Error(MC3D_EINVAL, "member index out of range");
+    double *p = field_by_name(name);
+    return p ? p + (long)b * dom.gN : nullptr;
+}
+
+StepScalars CurvilinearCSolver::memberScalars(int b) {
+    if (b < 0 || b >= dom.members) throw Error(MC3D_EINVAL, "member index out of range");
+    StepScalars h{};
+    hip_check(hipMemcpyAsync(&h, sc + b, sizeof(h), hipMemcpyDeviceToHost, st), "scalar read");
+    hip_check(hipStreamSynchronize(st), "scalar read");
+    return h;
+}
+
+void CurvilinearCSolver::setLineShapes(const LineTune deriv[3], const LineTune filt[3]) {
+    hip_check(hipStreamSynchronize(st), "operator rebuild");
+    const Options::BCType bt[3] = {opt.bcXType, opt.bcYType, opt.bcZType};
+    const Options::FDType ft[3] = {opt.xFDType, opt.yFDType, opt.zFDType};
+    std::unique_ptr<AbstractDerivatives> *dv[3] = {&derivX, &derivY, &derivZ};
+    std::unique_ptr<AbstractFilter> *fv[3] = {&filtX, &filtY, &filtZ};
+    const int bc3[3] = {bt[0] == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET,
+                        bt[1] == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET,
+                        bt[2] == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET};
+    for (int d = 0; d < 3; ++d) {
+        const auto dir = (AbstractDerivatives::Direct)d;
+        const int scheme = ft[d] == Options::CD2 ? MC3D_CD2 : ft[d] == Options::PADE6 ? MC3D_PADE6 : MC3D_PENTA10;
+        auto *nd = new LineDerivative(scheme, dom, bt[d], dir, st, deriv[d]);
+        nd->timer = &timer;
+        dv[d]->reset(nd);
+        const int kind = opt.filterType == Options::COMPACT8 ? MC3D_COMPACT8 : MC3D_COMPACT10;
+        auto *nf = new LineFilter(kind, opt.alphaF, dom, bc3, dir, st, filt[d]);
+        nf->timer = &timer;
+        fv[d]->reset(nf);
+    }
+}
+
+// The launches of a step depend on the host state through: the first step's
+// time reset, whether this step filters and in which direction order
+// (filterTimeStep % 3), and the RK scheme's step form.
+int CurvilinearCSolver::stepKey() const {
+    if (timeStep == opt.timeStep) return -1;
+    const bool filt = (timeStep + 1) % opt.filterStep == 0;
+    return rk->variant() * 8 + (filt ? 1 + filterTimeStep % 3 : 0);
+}
+
+void CurvilinearCSolver::skipAdvance() {
+    const int S = rk->stages();
+    timeStep++;
+    if (timeStep % opt.filterStep == 0) filterTimeStep += opt.subStepFiltering ? S : 1;
+    rkStep = S;
+    rkLast = true;
+    eos_done_ = false;
+    rk->finishStep();
 }
 
 // handlePeriodicStuff, src/AbstractSingleBlockMesh.cpp:2019-2085: a periodic
@@ -642,9 +751,11 @@ void CurvilinearCSolver::set_output(const std::string &dir, const double *const 
 
 // calcDtFromCFL, src/CurvilinearCSolver_Core.cpp:189-237
 void CurvilinearCSolver::preStep() {
-    timed("k_cfl", 7.0, [&] { hip_check(launch_cfl(pr, msh, dom.dx, dom.dy, dom.dz, cfl_partial, N, st), "cfl"); });
+    timed("k_cfl", 7.0, [&] {
+        hip_check(launch_cfl(pr, msh, dom.dx, dom.dy, dom.dz, cfl_partial, dom.gN, dom.members, st), "cfl");
+    });
     const bool first = (timeStep == opt.timeStep);
-    hip_check(launch_cfl_finish(cfl_partial, cfl_blocks(N), sc, opt.TSType == Options::CONST_CFL,
+    hip_check(launch_cfl_finish(cfl_partial, cfl_blocks(dom.gN), sc, dom.members, opt.TSType == Options::CONST_CFL,
                                 opt.CFL, first ? 1 : 0, opt.time, st),
               "cfl finish");
     timeStep++;
@@ -831,6 +942,7 @@ void CurvilinearCSolver::stage_update(int op, double a, double b, double c) {
     for (int e = 0; e < 5; ++e) ra.div[e] = div[e];
     ra.J = msh.J;
     ra.sc = sc;
+    ra.member_pts = dom.members > 1 ? (unsigned)dom.gN : 0u;
     ra.op = op;
     ra.a = a; ra.b = b; ra.c = c;
     // the non-conserved update of a stage that writes rk and is not followed by

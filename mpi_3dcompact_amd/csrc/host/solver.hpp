@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "../lineop_api.h"
 #include "../pointwise.h"
 #include "mc3d.h"
 #include "options.hpp"
@@ -32,8 +33,9 @@ void hip_check(hipError_t e, const char *what);
 // src/MPI_3DCompact.cpp:132): spacing 1/N periodic, 1/(N-1) Dirichlet.
 struct Domain {
     int gNx, gNy, gNz;
-    long gN;
+    long gN;          // points of one member
     double dx, dy, dz;
+    int members = 1;  // boxes of this size stacked along z in every field
     explicit Domain(const Options &o);
 };
 
@@ -68,7 +70,8 @@ class AbstractDerivatives {
 
 class LineDerivative : public AbstractDerivatives {
   public:
-    LineDerivative(int scheme, const Domain &dom, Options::BCType bc, Direct dir, hipStream_t st);
+    LineDerivative(int scheme, const Domain &dom, Options::BCType bc, Direct dir, hipStream_t st,
+                   const LineTune &shape = LineTune{});
     ~LineDerivative() override;
     void calc1stDerivField(const double *in, double *out) override;
     void calc1stDerivField_TPB(const double *in, double *out, const double *Nm2, const double *Nm1,
@@ -79,6 +82,7 @@ class LineDerivative : public AbstractDerivatives {
     double calcNeumann(const double *f) override;
     void calc1stDerivFields(int n, const double *const *in, double *const *out, bool accumulate) override;
     KernelTimer *timer = nullptr;
+    const mc3d_lineop *op() const { return op_; }
 
   protected:
     int scheme_;
@@ -88,16 +92,16 @@ class LineDerivative : public AbstractDerivatives {
 
 // src/Pade6.hpp:9-259, src/Penta10.hpp:9-197, src/CD2.hpp:9-83
 struct Pade6 : LineDerivative {
-    Pade6(const Domain &d, Options::BCType bc, Direct dir, hipStream_t st)
-        : LineDerivative(MC3D_PADE6, d, bc, dir, st) {}
+    Pade6(const Domain &d, Options::BCType bc, Direct dir, hipStream_t st, const LineTune &t = LineTune{})
+        : LineDerivative(MC3D_PADE6, d, bc, dir, st, t) {}
 };
 struct Penta10 : LineDerivative {
-    Penta10(const Domain &d, Options::BCType bc, Direct dir, hipStream_t st)
-        : LineDerivative(MC3D_PENTA10, d, bc, dir, st) {}
+    Penta10(const Domain &d, Options::BCType bc, Direct dir, hipStream_t st, const LineTune &t = LineTune{})
+        : LineDerivative(MC3D_PENTA10, d, bc, dir, st, t) {}
 };
 struct CD2 : LineDerivative {
-    CD2(const Domain &d, Options::BCType bc, Direct dir, hipStream_t st)
-        : LineDerivative(MC3D_CD2, d, bc, dir, st) {}
+    CD2(const Domain &d, Options::BCType bc, Direct dir, hipStream_t st, const LineTune &t = LineTune{})
+        : LineDerivative(MC3D_CD2, d, bc, dir, st, t) {}
 };
 
 // ---------------------------------------------------------------------------
@@ -117,11 +121,12 @@ class AbstractFilter {
 class LineFilter : public AbstractFilter {
   public:
     LineFilter(int kind, double alphaF, const Domain &dom, const int bc3[3],
-               AbstractDerivatives::Direct dir, hipStream_t st);
+               AbstractDerivatives::Direct dir, hipStream_t st, const LineTune &shape = LineTune{});
     ~LineFilter() override;
     void filterField(const double *in, double *out) override;
     void filterFields(int n, const double *const *in, double *const *out) override;
     KernelTimer *timer = nullptr;
+    const mc3d_lineop *op() const { return op_; }
 
   protected:
     mc3d_lineop *op_ = nullptr;
@@ -131,12 +136,14 @@ class LineFilter : public AbstractFilter {
 // src/Compact8Filter.hpp:9-139, src/Compact10Filter.hpp:9-308,
 // src/CommN5PadeTestFilter.hpp, src/CommN5ExpTestFilter.hpp
 struct Compact8Filter : LineFilter {
-    Compact8Filter(double af, const Domain &d, const int bc3[3], AbstractDerivatives::Direct dir, hipStream_t st)
-        : LineFilter(MC3D_COMPACT8, af, d, bc3, dir, st) {}
+    Compact8Filter(double af, const Domain &d, const int bc3[3], AbstractDerivatives::Direct dir, hipStream_t st,
+                   const LineTune &t = LineTune{})
+        : LineFilter(MC3D_COMPACT8, af, d, bc3, dir, st, t) {}
 };
 struct Compact10Filter : LineFilter {
-    Compact10Filter(double af, const Domain &d, const int bc3[3], AbstractDerivatives::Direct dir, hipStream_t st)
-        : LineFilter(MC3D_COMPACT10, af, d, bc3, dir, st) {}
+    Compact10Filter(double af, const Domain &d, const int bc3[3], AbstractDerivatives::Direct dir, hipStream_t st,
+                    const LineTune &t = LineTune{})
+        : LineFilter(MC3D_COMPACT10, af, d, bc3, dir, st, t) {}
 };
 struct CommN5PadeTestFilter : LineFilter {
     CommN5PadeTestFilter(const Domain &d, const int bc3[3], AbstractDerivatives::Direct dir, hipStream_t st)
@@ -225,40 +232,45 @@ class AbstractRK {
     virtual ~AbstractRK() = default;
     // runs full steps until the solver's end condition (maxTimeStep / maxTime)
     virtual void executeSolverLoop();
-    // one full time step (all stages)
-    virtual void step() = 0;
+    // one full time step: advance(), then the solver's checks, dumps and end test
+    void step();
+    // the launches of one full step (CFL and all stages) with their host
+    // bookkeeping, and nothing that waits for the device
+    void advance();
     // register update of the current stage (fused with the k2 assembly)
     virtual void updateConservedData() = 0;
     virtual int stages() const = 0;
+    // which of the scheme's step forms comes next (LSLDDRK4 alternates two)
+    virtual int variant() const { return 0; }
+    // host bookkeeping at the end of a step
+    virtual void finishStep() {}
 };
 
 class TVDRK3 : public AbstractRK {  // src/TVDRK3.hpp
   public:
     explicit TVDRK3(CurvilinearCSolver *c) { cs = c; }
-    void step() override;
     void updateConservedData() override;
     int stages() const override { return 3; }
 };
 class RK4 : public AbstractRK {  // src/RK4.hpp
   public:
     explicit RK4(CurvilinearCSolver *c) { cs = c; }
-    void step() override;
     void updateConservedData() override;
     int stages() const override { return 4; }
 };
 class KenRK4 : public AbstractRK {  // src/KenRK4.hpp
   public:
     explicit KenRK4(CurvilinearCSolver *c) { cs = c; }
-    void step() override;
     void updateConservedData() override;
     int stages() const override { return 5; }
 };
 class LSLDDRK4 : public AbstractRK {  // src/LSLDDRK4.hpp: alternating 5- and 6-stage steps
   public:
     explicit LSLDDRK4(CurvilinearCSolver *c) { cs = c; }
-    void step() override;
     void updateConservedData() override;
     int stages() const override { return rkDualStepStep == 1 ? 5 : 6; }
+    int variant() const override { return rkDualStepStep - 1; }
+    void finishStep() override { rkDualStepStep = rkDualStepStep == 1 ? 2 : 1; }
     int rkDualStepStep = 1;
 };
 
@@ -267,7 +279,11 @@ class LSLDDRK4 : public AbstractRK {  // src/LSLDDRK4.hpp: alternating 5- and 6-
 // ---------------------------------------------------------------------------
 class CurvilinearCSolver {
   public:
-    CurvilinearCSolver(const Options &opt, hipStream_t st);
+    // members > 1: that many boxes of the options' size stacked along z in
+    // every field, each with its own time step and, when mu_members is given,
+    // its own MU_REF (periodic boxes, no LES model or Vreman with NONE / LOCAL
+    // averaging)
+    CurvilinearCSolver(const Options &opt, hipStream_t st, int members = 1, const double *mu_members = nullptr);
     ~CurvilinearCSolver();
 
     Options opt;
@@ -342,6 +358,22 @@ class CurvilinearCSolver {
     const Cons &state() const { return rkStep == 1 ? r1 : rk_; }
     double *field_by_name(const std::string &name);
 
+    // stacked members
+    int members() const { return dom.members; }
+    void setMemberInitial(int b, const double *rho0, const double *U0, const double *V0, const double *W0,
+                          const double *p0);
+    void setMemberMetrics(int b, const double *const m[10]);
+    // member b's part of a full field (null for names that are not full fields)
+    double *memberField(int b, const std::string &name);
+    StepScalars memberScalars(int b);
+    // rebuild the derivative and filter operators with the given launch shapes
+    void setLineShapes(const LineTune deriv[3], const LineTune filt[3]);
+    // launch sequence of the next step: steps with equal keys issue the same
+    // launches with the same arguments (-1: the first step, which differs)
+    int stepKey() const;
+    // the host bookkeeping of advance() without its launches (for a replayed step)
+    void skipAdvance();
+
     // boundary handling (src/CurvilinearCSolver_BC.cpp, src/SpongeBC.cpp)
     void setupBoundaries();
     void wallInitialConditions();   // :63-322
@@ -369,6 +401,7 @@ class CurvilinearCSolver {
     void hitStats(double out[11]);
 
   private:
+    std::vector<double> mu_members_;  // per-member MU_REF when members > 1
     std::chrono::steady_clock::time_point last_check_ = std::chrono::steady_clock::now();
     bool dumps_ = false;
     std::string out_dir_;

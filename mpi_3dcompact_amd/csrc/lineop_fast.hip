@@ -125,12 +125,13 @@ __global__ __launch_bounds__(1024) void lineop_strided(OpDev op, LaunchArgs a) {
         base = (long)(qq / a.Nx) * a.Nx * a.Ny + qq % a.Nx;
         ps = a.Nx;
     } else {
-        base = qq;
-        ps = (long)a.Nx * a.Ny;
+        const int plane = a.Nx * a.Ny;
+        base = (long)(qq % plane) + (long)(qq / plane) * plane * a.Nz;
+        ps = plane;
     }
     const int cs = c * MM;
     // window loads through a buffer descriptor (fields below 2^28 points)
-    const unsigned fbytes = (unsigned)a.Nx * (unsigned)a.Ny * (unsigned)a.Nz * 8u;
+    const unsigned fbytes = (unsigned)a.Nx * (unsigned)a.Ny * (unsigned)a.Nz * (unsigned)a.nbatch * 8u;
     const __amdgpu_buffer_rsrc_t rin =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(in), (short)0, (int)fbytes, kRawBufferWord3);
     const unsigned boff = (unsigned)base * 8u, rowb = (unsigned)ps * 8u;

@@ -16,11 +16,14 @@
 
 namespace mc3d {
 
+// first point of line q; y-lines run over the members stacked along z as
+// over one taller box, z-lines of member b start b * Nx * Ny * Nz further
 template <int DIR>
-__device__ __forceinline__ long line_base(int q, int Nx, int Ny) {
-    if (DIR == 0) return (long)q * Nx;
-    if (DIR == 1) return (long)(q / Nx) * Nx * Ny + (q % Nx);
-    return (long)q;
+__device__ __forceinline__ long line_base(int q, const LaunchArgs &a) {
+    if (DIR == 0) return (long)q * a.Nx;
+    if (DIR == 1) return (long)(q / a.Nx) * a.Nx * a.Ny + (q % a.Nx);
+    const int plane = a.Nx * a.Ny;
+    return (long)(q % plane) + (long)(q / plane) * plane * a.Nz;
 }
 
 template <int DIR>
@@ -81,14 +84,14 @@ __global__ __launch_bounds__(256) void lineop_kernel(OpDev op, LaunchArgs a) {
         int p = e - H;
         double v;
         if (p >= 0 && p < N) {
-            v = in[line_base<DIR>(q, a.Nx, a.Ny) + p * ps];
+            v = in[line_base<DIR>(q, a) + p * ps];
         } else if (MODE == kModeTPB) {
             const int slot = (p < 0) ? 3 + p : 3 + (p - N);  // -1 -> Nm1, N -> Np1
             const double *w = a.wrap[slot];
             v = w ? w[q] : 0.0;
         } else {
             p = (p < 0) ? p + N : p - N;
-            v = in[line_base<DIR>(q, a.Nx, a.Ny) + p * ps];
+            v = in[line_base<DIR>(q, a) + p * ps];
         }
         tile[l * EXT + e] = v;
     }
@@ -182,7 +185,7 @@ __global__ __launch_bounds__(256) void lineop_kernel(OpDev op, LaunchArgs a) {
         else { ll = idx % L; p = idx / L; }
         const int qq = q0 + ll;
         if (qq >= a.nlines) continue;
-        const long g = line_base<DIR>(qq, a.Nx, a.Ny) + p * ps;
+        const long g = line_base<DIR>(qq, a) + p * ps;
         const double v = tile[ll * EXT + H + p];
         if (a.accumulate) out[g] += v;
         else out[g] = v;

@@ -13,7 +13,8 @@ struct LaunchArgs {
     const double *in[kMaxFields];
     double *out[kMaxFields];
     const double *wrap[6];  // TPB wrap values {Nm3, Nm2, Nm1, Np1, Np2, Np3}
-    int Nx, Ny, Nz;
+    int Nx, Ny, Nz;  // one member's box
+    int nbatch;      // members stacked along z (>= 1): z-lines of member b start b*Nx*Ny*Nz further
     int nlines;
     int accumulate;  // out += result instead of out = result
 };

@@ -8,7 +8,15 @@ namespace mc3d {
 // Ideal gas, src/IdealGas.hpp:35-55
 struct Gas {
     double gamma, Pr, R_gas, cp, T_ref, mu_ref;
+    // members stacked along z with their own mu_ref: mu_member[ip / member_pts]
+    // replaces mu_ref when mu_member is set
+    const double *mu_member;
+    unsigned member_pts;
 };
+
+__device__ __forceinline__ double member_mu_ref(const Gas &g, long ip) {
+    return g.mu_member ? g.mu_member[(unsigned)ip / g.member_pts] : g.mu_ref;
+}
 
 struct Metrics {
     const double *J, *J11, *J12, *J13, *J21, *J22, *J23, *J31, *J32, *J33;
@@ -59,7 +67,8 @@ struct RkArgs {
     Cons r1, rk, k2, r2;
     const double *div[5];  // summed flux divergences (cont, mom1..3, engy)
     const double *J;
-    const StepScalars *sc;
+    const StepScalars *sc;  // one per member when member_pts is set: sc[ip / member_pts]
+    unsigned member_pts;
     int op;
     double a, b, c;        // stage coefficients
     int eos;               // also update the non-conserved data from the new rk state
@@ -84,12 +93,13 @@ hipError_t launch_init_state(const double *rho0, const double *U0, const double 
 hipError_t launch_eos(Cons in, Prim pr, Gas g, long n, hipStream_t st);
 // Ucurv, Vcurv, Wcurv from U, V, W and the metrics (on request only)
 hipError_t launch_contravariant(Prim pr, Metrics m, long n, hipStream_t st);
-// CFL: partial maxima into partial[nblocks], then a single-block finish that
-// sets dt (const CFL) or cfl (const dt) and advances time.
+// CFL: for each of nmember members of n points (stacked), partial maxima
+// into partial[member * cfl_blocks(n) + block], then one finishing workgroup
+// per member that sets its dt (const CFL) or cfl (const dt) and advances its time.
 int cfl_blocks(long n);
 hipError_t launch_cfl(Prim pr, Metrics m, double dx, double dy, double dz, double *partial,
-                      long n, hipStream_t st);
-hipError_t launch_cfl_finish(const double *partial, int nblocks, StepScalars *sc, int const_cfl,
+                      long n, int nmember, hipStream_t st);
+hipError_t launch_cfl_finish(const double *partial, int nblocks, StepScalars *sc, int nmember, int const_cfl,
                              double cfl_target, int first_step, double time0, hipStream_t st);
 hipError_t launch_flux(const FluxArgs &a, Gas g, long n, hipStream_t st);
 hipError_t launch_rhs_rk(const RkArgs &a, long n, hipStream_t st);

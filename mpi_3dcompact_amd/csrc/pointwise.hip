@@ -18,8 +18,8 @@ constexpr int kBlock = 256;
 
 inline dim3 grid_for(long n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 
-__device__ __forceinline__ double solve_mu(const Gas &g, double T) {
-    return g.mu_ref * pow(T / g.T_ref, 0.76);
+__device__ __forceinline__ double solve_mu(const Gas &g, double T, long ip) {
+    return member_mu_ref(g, ip) * pow(T / g.T_ref, 0.76);
 }
 __device__ __forceinline__ double solve_p(const Gas &g, double rho, double rhoE, double U, double V,
                                           double W) {
@@ -48,7 +48,7 @@ __global__ void k_init_state(const double *__restrict__ rho0, const double *__re
     r1.q[4][ip] = p / (g.gamma - 1.0) + 0.5 * rho * (U * U + V * V + W * W);
     const double T = solve_T(g, rho, p);
     pr.T[ip] = T;
-    pr.mu[ip] = solve_mu(g, T);
+    pr.mu[ip] = solve_mu(g, T, ip);
     pr.sos[ip] = solve_sos(g, rho, p);
 }
 
@@ -75,7 +75,7 @@ __global__ void k_eos(Cons c, Prim pr, Gas g, long n) {
     pr.U[ip] = U; pr.V[ip] = V; pr.W[ip] = W;
     pr.p[ip] = p;
     pr.T[ip] = T;
-    pr.mu[ip] = solve_mu(g, T);
+    pr.mu[ip] = solve_mu(g, T, ip);
     pr.sos[ip] = solve_sos(g, rho, p);
 }
 
@@ -84,7 +84,8 @@ __global__ __launch_bounds__(kBlock) void k_cfl(Prim pr, Metrics m, double dx, d
                                                 double dz, double *partial, long n) {
     __shared__ double red[kBlock];
     double v = -100000.0;
-    for (long ip = (long)blockIdx.x * blockDim.x + threadIdx.x; ip < n;
+    const long first = (long)blockIdx.y * n;  // this member's points
+    for (long ip = first + (long)blockIdx.x * blockDim.x + threadIdx.x; ip < first + n;
          ip += (long)gridDim.x * blockDim.x) {
         const double s = pr.sos[ip];
         const double u = m.J11[ip] * (fabs(pr.U[ip]) + s) / dx + m.J22[ip] * (fabs(pr.V[ip]) + s) / dy +
@@ -97,14 +98,17 @@ __global__ __launch_bounds__(kBlock) void k_cfl(Prim pr, Metrics m, double dx, d
         if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+    if (threadIdx.x == 0) partial[(long)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
 }
 
+// one workgroup per member
 __global__ __launch_bounds__(kBlock) void k_cfl_finish(const double *partial, int nb,
-                                                       StepScalars *sc, int const_cfl,
+                                                       StepScalars *scs, int const_cfl,
                                                        double cfl_target, int first_step,
                                                        double time0) {
     __shared__ double red[kBlock];
+    StepScalars *sc = scs + blockIdx.x;
+    partial += (long)blockIdx.x * nb;
     double v = -100000.0;
     for (int i = threadIdx.x; i < nb; i += blockDim.x) v = fmax(v, partial[i]);
     red[threadIdx.x] = v;
@@ -232,7 +236,8 @@ __global__ __launch_bounds__(kBlock) void k_rhs_rk(RkArgs a, long n) {
     const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= n) return;
     const double J = a.J[ip];
-    const double dtJ = a.sc->dt * J;
+    const StepScalars *sc = a.member_pts ? a.sc + (unsigned)ip / a.member_pts : a.sc;
+    const double dtJ = sc->dt * J;
     double z[5];
     if (a.bc.sponge) {  // calcSpongeSource = sigma (avg - phi), src/CurvilinearCSolver.hpp:293-295
         const double sg = a.bc.sigma[ip];
@@ -261,7 +266,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_rk(RkArgs a, long n) {
 #pragma unroll
         for (int f = 0; f < 6; ++f)
             if (on[f] && a.bc.role[f] == kFaceWall) {
-                k2[0] = -a.sc->dt * J * a.bc.cont[f / 2][ip];
+                k2[0] = -sc->dt * J * a.bc.cont[f / 2][ip];
                 k2[1] = 0.0; k2[2] = 0.0; k2[3] = 0.0; k2[4] = 0.0;
             }
 #pragma unroll
@@ -286,7 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_rk(RkArgs a, long n) {
         pr.U[ip] = U; pr.V[ip] = V; pr.W[ip] = W;
         pr.p[ip] = p;
         pr.T[ip] = T;
-        pr.mu[ip] = solve_mu(g, T);
+        pr.mu[ip] = solve_mu(g, T, ip);
         pr.sos[ip] = solve_sos(g, rho, p);
     }
 }
@@ -443,14 +448,16 @@ int cfl_blocks(long n) {
 }
 
 hipError_t launch_cfl(Prim pr, Metrics m, double dx, double dy, double dz, double *partial, long n,
-                      hipStream_t st) {
-    hipLaunchKernelGGL(k_cfl, dim3(cfl_blocks(n)), dim3(kBlock), 0, st, pr, m, dx, dy, dz, partial, n);
+                      int nmember, hipStream_t st) {
+    if (nmember < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_cfl, dim3(cfl_blocks(n), nmember), dim3(kBlock), 0, st, pr, m, dx, dy, dz, partial, n);
     return hipGetLastError();
 }
 
-hipError_t launch_cfl_finish(const double *partial, int nblocks, StepScalars *sc, int const_cfl,
+hipError_t launch_cfl_finish(const double *partial, int nblocks, StepScalars *sc, int nmember, int const_cfl,
                              double cfl_target, int first_step, double time0, hipStream_t st) {
-    hipLaunchKernelGGL(k_cfl_finish, dim3(1), dim3(kBlock), 0, st, partial, nblocks, sc, const_cfl,
+    if (nmember < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_cfl_finish, dim3(nmember), dim3(kBlock), 0, st, partial, nblocks, sc, const_cfl,
                        cfl_target, first_step, time0);
     return hipGetLastError();
 }

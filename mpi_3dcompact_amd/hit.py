@@ -131,6 +131,19 @@ def options_text(N, fd="PADE6", filt="COMPACT10", alpha_f=0.49, rk="TVDRK3", cfl
     return "\n".join(lines) + "\n"
 
 
+def ensemble_members(count=64, seed=2024, sizes=(32, 48, 64, 96, 128), mt=(0.1, 0.4),
+                     mu=(1.0 / 3200.0, 1.0 / 800.0)):
+    """Members of a decaying-HIT ensemble: N drawn uniformly from sizes, M_t
+    uniform in mt, MU_REF uniform in mu, and an initial-field seed each, from
+    one seeded generator.  Returns a list of dicts N, Mt, mu, seed."""
+    r = np.random.default_rng(seed)
+    out = []
+    for m in range(count):
+        out.append(dict(N=int(r.choice(sizes)), Mt=float(r.uniform(*mt)), mu=float(r.uniform(*mu)),
+                        seed=int(r.integers(0, 2 ** 31 - 1))))
+    return out
+
+
 FACE_NAMES = ("X0", "X1", "Y0", "Y1", "Z0", "Z1")
 _FACE = {"PERIODIC": 0, "INTERNALLY_PERIODIC": 1, "SPONGE": 2, "ADIABATIC_WALL": 3, "CONST_T_WALL": 4,
          "MOVING_ADIABATIC_WALL": 5, "INLET": 6}
