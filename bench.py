@@ -20,6 +20,11 @@ configuration (256^3 by default).
 effective_bandwidth = the design-independent dataflow of the reference for
 this configuration (9592 B per point-step: every reference operator's distinct
 full-size FP64 inputs and outputs, KenRK4 + Vreman LOCAL) times the step rate.
+ensemble = a second workload of 64 HIT boxes of uneven sizes advanced together
+(members of equal size stacked, one HIP graph per step, tuned line-operator
+shapes): its point-step rate, the same with direct launches and with default
+shapes, the members run one after another as standalone solvers, and a
+parity sample of members against standalone runs.
 """
 import argparse
 import json
@@ -75,6 +80,126 @@ def cpu_baseline(n_cpu):
                       f"({dt:.1f} s), OpenMP over independent lines"}
 
 
+# ensemble workload: 64 decaying-HIT boxes, N drawn from {32, 48, 64, 96, 128},
+# M_t ~ U[0.1, 0.4], MU_REF ~ U[1/3200, 1/800] (hit.ensemble_members, seed 2024),
+# PADE6 + COMPACT10 (alpha_f 0.49) + RK4 at CFL 0.5, each box its own dt
+ENS_CASE = dict(fd="PADE6", filt="COMPACT10", alpha_f=0.49, rk="RK4", cfl=0.5)
+# reference dataflow per point-step of plain RK4: stage subtotal 1512 B x 4,
+# RK4 register updates 680 B, filter + CFL 312 B
+B_REF_RK4 = 7040.0
+
+
+def ensemble_leg(steps, warmup, check_members=3, cache=None, step_only=False):
+    """Ensemble throughput (graph replays, tuned shapes) with the same ensemble
+    without graphs and without tuning, the members run one after another as
+    standalone solvers, and a parity sample of members against standalone runs.
+    cache: tuning cache file; step_only: the tuned graph-replayed steps alone."""
+    import numpy as np
+    from mpi_3dcompact_amd import hit
+    from mpi_3dcompact_amd.ensemble import Ensemble
+    from mpi_3dcompact_amd.solver import Solver
+    members = hit.ensemble_members(64, seed=2024)
+    pts = sum(m["N"] ** 3 for m in members)
+    log(f"[bench] ensemble: {len(members)} members, {pts / 1e6:.1f} M points; generating initial fields")
+    ics = [hit.hit_initial_condition(m["N"], k0=4.0, Mt=m["Mt"], seed=m["seed"]) for m in members]
+    opts = hit.options_text(members[0]["N"], **ENS_CASE)
+
+    def make(tune):
+        e = Ensemble(opts, [(m["N"],) * 3 for m in members], [m["mu"] for m in members])
+        for i, m in enumerate(members):
+            e.set_metrics(i, hit.identity_metrics(m["N"]))
+            e.set_initial(i, ics[i])
+        t0 = time.perf_counter()
+        if tune:
+            e.tune(cache)
+        return e, time.perf_counter() - t0
+
+    def timed(e, k):
+        e.sync()
+        t0 = time.perf_counter()
+        e.step(k)
+        e.sync()
+        return time.perf_counter() - t0
+
+    res = {}
+    if step_only:
+        e, tune_s = make(True)
+        e.step(warmup)
+        wall = timed(e, steps)
+        e.close()
+        value = pts * steps / wall / 1e6
+        return {"points": pts, "value": round(value, 3), "unit": "Mpt-steps/s",
+                "ms_per_step": round(wall * 1e3 / steps, 3), "steps": steps, "warmup": warmup,
+                "tune_seconds": round(tune_s, 2)}
+    # default shapes, graphs
+    e, _ = make(False)
+    e.step(warmup)
+    res["untuned_graph_s"] = timed(e, steps)
+    e.close()
+    # tuned shapes: graphs, then direct launches on the same ensemble
+    e, tune_s = make(True)
+    info = e.info()
+    log(f"[bench] ensemble tuned in {tune_s:.1f} s")
+    e.step(warmup)
+    wall = timed(e, steps)
+    e.use_graphs(False)
+    res["tuned_direct_s"] = timed(e, steps)
+    total_steps = warmup + 2 * steps
+    # parity sample: the largest, smallest and a middle size against standalone runs
+    sizes = sorted({m["N"] for m in members})
+    pick = [next(i for i, m in enumerate(members) if m["N"] == n)
+            for n in (sizes[0], sizes[len(sizes) // 2], sizes[-1])][:check_members]
+    worst = 0.0
+    for i in pick:
+        m = members[i]
+        g = Solver(hit.options_text(m["N"], mu_ref=m["mu"], **ENS_CASE), (m["N"],) * 3)
+        g.set_metrics(hit.identity_metrics(m["N"]))
+        g.set_initial(ics[i])
+        g.step(total_steps)
+        for f in ("rho", "rhoU", "rhoV", "rhoW", "rhoE"):
+            a, b = e.field(i, f), g.field(f)
+            worst = max(worst, float(np.linalg.norm(a - b) / np.linalg.norm(b)))
+        g.close()
+    finite = all(np.isfinite(e.field(i, "rho")).all() for i in range(len(members)))
+    e.close()
+    if not finite:
+        raise SystemExit("non-finite density in the ensemble")
+    # the members one after another, each as a standalone solver
+    log("[bench] ensemble members as standalone solvers")
+    alone = 0.0
+    per = 3
+    for i, m in enumerate(members):
+        g = Solver(hit.options_text(m["N"], mu_ref=m["mu"], **ENS_CASE), (m["N"],) * 3)
+        g.set_metrics(hit.identity_metrics(m["N"]))
+        g.set_initial(ics[i])
+        g.step(2)
+        g.sync()
+        t0 = time.perf_counter()
+        g.step(per)
+        g.sync()
+        alone += (time.perf_counter() - t0) / per
+        g.close()
+    value = pts * steps / wall / 1e6
+    return {
+        "workload": "64 decaying-HIT boxes, N from {32,48,64,96,128}, PADE6 + COMPACT10 (alpha_f 0.49) + RK4 "
+                    "(CFL 0.5, own dt per box), M_t U[0.1,0.4], MU_REF U[1/3200,1/800], seed 2024",
+        "points": pts, "buckets": len(info["buckets"]),
+        "value": round(value, 3), "unit": "Mpt-steps/s", "ms_per_step": round(wall * 1e3 / steps, 3),
+        "steps": steps, "warmup": warmup,
+        "direct_launch_value": round(pts * steps / res["tuned_direct_s"] / 1e6, 3),
+        "untuned_value": round(pts * steps / res["untuned_graph_s"] / 1e6, 3),
+        "standalone_sequential_value": round(pts / alone / 1e6, 3),
+        "batched_over_standalone": round((pts * steps / wall) / (pts / alone), 3),
+        "tune_seconds": round(tune_s, 2),
+        "effective_bandwidth": {"bytes_per_pt_step": B_REF_RK4,
+                                "achieved": round(B_REF_RK4 * value * 1e6 / 1e9, 1), "unit": "GB/s",
+                                "frac": round(B_REF_RK4 * value * 1e6 / 1e9 / HBM_PEAK_GBS, 4)},
+        "parity_vs_standalone": {"members": pick, "steps": total_steps, "max_rel_l2": worst},
+        "shapes": [{"dims": b["dims"][0], "members": len(b["members"]), "deriv": b["deriv"], "filter": b["filter"]}
+                   for b in info["buckets"]],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +208,12 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="grid points per direction")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble workload")
+    ap.add_argument("--ensemble-only", action="store_true", help="run the ensemble workload alone")
+    ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--ensemble-cache", default=None, help="line-operator tuning cache file")
+    ap.add_argument("--ensemble-step-only", action="store_true",
+                    help="with --ensemble-only: the tuned, graph-replayed steps alone")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -92,6 +223,11 @@ def main():
     import numpy as np
     from mpi_3dcompact_amd import hit
     from mpi_3dcompact_amd.solver import Solver
+
+    if args.ensemble_only:
+        print(json.dumps({"ensemble": ensemble_leg(args.ensemble_steps, 4, cache=args.ensemble_cache,
+                                                   step_only=args.ensemble_step_only)}), flush=True)
+        return
 
     N = args.n
     log(f"[bench] generating {N}^3 HIT initial field")
@@ -163,6 +299,8 @@ def main():
     }
     if cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_leg(args.ensemble_steps, 4, cache=args.ensemble_cache)
     print(json.dumps(out), flush=True)
 
 

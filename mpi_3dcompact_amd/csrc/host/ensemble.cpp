@@ -223,6 +223,7 @@ void Ensemble::tune(const std::string &cache_path) {
         std::ifstream f(cache_path);
         std::string line;
         while (std::getline(f, line)) {
+            if (line.empty() || line[0] == '#') continue;
             const size_t eq = line.find(" = ");
             if (eq != std::string::npos) cache[line.substr(0, eq)] = line.substr(eq + 3);
         }

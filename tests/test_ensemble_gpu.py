@@ -149,6 +149,8 @@ def test_tuned_ensemble_within_1e12_of_standalone_after_200_steps(gpu, tmp_path)
     e2 = build(members, tune=True, tune_cache=cache, **C3)
     i2 = e2.info()
     assert all(t["cached"] for b in i2["buckets"] for t in b["tuned"])
+    text = cache.read_text().splitlines()
+    assert sum(ln.startswith("#") for ln in text) == 1 and len(text) == 1 + 6 * nb
     assert [b["deriv"] + b["filter"] for b in i2["buckets"]] == [b["deriv"] + b["filter"] for b in info["buckets"]]
     e2.step(5)
     e3 = build(members, tune=True, tune_cache=cache, graphs=False, **C3)
