@@ -170,7 +170,8 @@ int apply(const mc3d_lineop *op, const double *const *in, double *const *out, in
         return mc3d::fail(MC3D_EINVAL, "transformed-periodic derivatives take one member");
     a.Nx = op->dims[0]; a.Ny = op->dims[1]; a.Nz = op->dims[2];
     a.nbatch = op->nbatch;
-    a.nlines = (int)((long)a.Nx * a.Ny * a.Nz * op->nbatch / op->dims[op->dir]);
+    // z-lines: one member's lines per grid z layer; x- and y-lines: all members'
+    a.nlines = (int)((long)a.Nx * a.Ny * a.Nz * (op->dir == 2 ? 1 : op->nbatch) / op->dims[op->dir]);
     a.accumulate = accumulate;
     const hipStream_t st = static_cast<hipStream_t>(stream);
     hipError_t e = (op->fast && mode != mc3d::kModeTPB) ? mc3d::lineop_launch_fast(op->dev, a, nfields, st)

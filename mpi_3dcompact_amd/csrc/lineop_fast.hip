@@ -121,17 +121,20 @@ __global__ __launch_bounds__(1024) void lineop_strided(OpDev op, LaunchArgs a) {
     const double *__restrict__ in = a.in[blockIdx.y];
     double *__restrict__ out = a.out[blockIdx.y];
     long base, ps;
+    unsigned fbytes = (unsigned)a.Nx * (unsigned)a.Ny * (unsigned)a.Nz * 8u;  // one member
     if (op.dir == 1) {
         base = (long)(qq / a.Nx) * a.Nx * a.Ny + qq % a.Nx;
         ps = a.Nx;
+        fbytes *= (unsigned)a.nbatch;
     } else {
-        const int plane = a.Nx * a.Ny;
-        base = (long)(qq % plane) + (long)(qq / plane) * plane * a.Nz;
-        ps = plane;
+        const long member = (long)blockIdx.z * a.Nx * a.Ny * a.Nz;
+        in += member;
+        out += member;
+        base = qq;
+        ps = (long)a.Nx * a.Ny;
     }
     const int cs = c * MM;
     // window loads through a buffer descriptor (fields below 2^28 points)
-    const unsigned fbytes = (unsigned)a.Nx * (unsigned)a.Ny * (unsigned)a.Nz * (unsigned)a.nbatch * 8u;
     const __amdgpu_buffer_rsrc_t rin =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(in), (short)0, (int)fbytes, kRawBufferWord3);
     const unsigned boff = (unsigned)base * 8u, rowb = (unsigned)ps * 8u;
@@ -362,7 +365,7 @@ hipError_t launch_fast(const OpDev &op, const LaunchArgs &a, int nfields, hipStr
         dim3 grid((a.nlines + L - 1) / L, nfields), block(L, op.P);
         hipLaunchKernelGGL((lineop_contig<W, H, MM, PER>), grid, block, lineop_fast_lds_bytes(op), st, op, a);
     } else {
-        dim3 grid((a.nlines + 63) / 64, nfields), block(64, op.P);
+        dim3 grid((a.nlines + 63) / 64, nfields, op.dir == 2 ? a.nbatch : 1), block(64, op.P);
         const size_t lds = sizeof(double) * (size_t)64 * (op.NI + 1);
         hipLaunchKernelGGL((lineop_strided<W, H, MM, PER>), grid, block, lds, st, op, a);
     }

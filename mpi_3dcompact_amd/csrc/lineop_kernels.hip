@@ -16,14 +16,12 @@
 
 namespace mc3d {
 
-// first point of line q; y-lines run over the members stacked along z as
-// over one taller box, z-lines of member b start b * Nx * Ny * Nz further
+// first point of line q (z-lines: within the member of grid z)
 template <int DIR>
 __device__ __forceinline__ long line_base(int q, const LaunchArgs &a) {
     if (DIR == 0) return (long)q * a.Nx;
     if (DIR == 1) return (long)(q / a.Nx) * a.Nx * a.Ny + (q % a.Nx);
-    const int plane = a.Nx * a.Ny;
-    return (long)(q % plane) + (long)(q / plane) * plane * a.Nz;
+    return (long)q;
 }
 
 template <int DIR>
@@ -63,8 +61,9 @@ __global__ __launch_bounds__(256) void lineop_kernel(OpDev op, LaunchArgs a) {
     double *red = smem + (size_t)L * EXT;
 
     const int f = blockIdx.y;
-    const double *__restrict__ in = a.in[f];
-    double *__restrict__ out = a.out[f];
+    const long member = DIR == 2 ? (long)blockIdx.z * a.Nx * a.Ny * a.Nz : 0;
+    const double *__restrict__ in = a.in[f] + member;
+    double *__restrict__ out = a.out[f] + member;
     const int q0 = blockIdx.x * L;
     const int nthr = blockDim.x;
     const int tid = threadIdx.x;
@@ -230,7 +229,7 @@ size_t lineop_lds_bytes(const OpDev &op) {
 
 hipError_t lineop_launch(const OpDev &op, const LaunchArgs &a, int nfields, int mode, hipStream_t st) {
     const int nblk = (a.nlines + op.L - 1) / op.L;
-    dim3 grid(nblk, nfields);
+    dim3 grid(nblk, nfields, op.dir == 2 ? a.nbatch : 1);
     dim3 block(op.L * op.P);
     const size_t lds = lineop_lds_bytes(op);
     if (op.W == 0) return launch_dir<0>(op, a, mode, grid, block, lds, st);

@@ -14,7 +14,10 @@ struct LaunchArgs {
     double *out[kMaxFields];
     const double *wrap[6];  // TPB wrap values {Nm3, Nm2, Nm1, Np1, Np2, Np3}
     int Nx, Ny, Nz;  // one member's box
-    int nbatch;      // members stacked along z (>= 1): z-lines of member b start b*Nx*Ny*Nz further
+    // members stacked along z (>= 1).  x- and y-lines run over all of them as
+    // over one box nbatch*Nz tall (nlines counts them all); z-lines are
+    // launched with the member as grid z (nlines counts one member's lines)
+    int nbatch;
     int nlines;
     int accumulate;  // out += result instead of out = result
 };
