@@ -318,6 +318,7 @@ void Ensemble::tuneBucket(Bucket &b, std::map<std::string, std::string> &cache) 
                             }
                             mc3d_lineop_destroy(op);
                             ++best.candidates;
+                            best.trials.emplace_back(t, ms);
                             if (best.ms < 0.0 || ms < best.ms) {
                                 best.ms = ms;
                                 best.shape = t;
@@ -360,7 +361,12 @@ std::string Ensemble::info() const {
             const TunedShape &t = b.tuned[k];
             o << (k ? ", " : "") << "{\"key\": \"" << json_escape(t.key) << "\", \"mm\": " << t.shape.mm
               << ", \"family\": " << t.shape.family << ", \"lines\": " << t.shape.lines << ", \"ms\": " << t.ms
-              << ", \"candidates\": " << t.candidates << ", \"cached\": " << (t.cached ? "true" : "false") << "}";
+              << ", \"candidates\": " << t.candidates << ", \"cached\": " << (t.cached ? "true" : "false")
+              << ", \"trials\": [";
+            for (size_t r = 0; r < t.trials.size(); ++r)
+                o << (r ? ", " : "") << "[" << t.trials[r].first.mm << ", " << t.trials[r].first.family << ", "
+                  << t.trials[r].first.lines << ", " << t.trials[r].second << "]";
+            o << "]}";
         }
         o << "]}";
     }

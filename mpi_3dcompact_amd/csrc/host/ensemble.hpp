@@ -33,6 +33,7 @@ struct TunedShape {
     double ms = 0.0;   // measured time of the usage pattern with this shape (0: default, not measured)
     int candidates = 0;
     bool cached = false;
+    std::vector<std::pair<LineTune, double>> trials;  // every candidate measured (empty when cached)
 };
 
 class Ensemble {

@@ -14,6 +14,12 @@ struct Gas {
     unsigned member_pts;
 };
 
+// Streaming access of the pointwise kernels: each field value is touched once
+// per launch and the fields exceed every cache, so loads and stores are
+// non-temporal (measured: 5.5 -> 6.0 TB/s for kernels of 13-48 streams).
+__device__ __forceinline__ double ldnt(const double *p, long i) { return __builtin_nontemporal_load(p + i); }
+__device__ __forceinline__ void stnt(double *p, long i, double v) { __builtin_nontemporal_store(v, p + i); }
+
 __device__ __forceinline__ double member_mu_ref(const Gas &g, long ip) {
     return g.mu_member ? g.mu_member[(unsigned)ip / g.member_pts] : g.mu_ref;
 }

@@ -58,25 +58,25 @@ __global__ void k_init_state(const double *__restrict__ rho0, const double *__re
 __global__ void k_contravariant(Prim pr, Metrics m, long n) {
     const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= n) return;
-    const double U = pr.U[ip], V = pr.V[ip], W = pr.W[ip];
-    pr.Ucurv[ip] = m.J11[ip] * U + m.J12[ip] * V + m.J13[ip] * W;
-    pr.Vcurv[ip] = m.J21[ip] * U + m.J22[ip] * V + m.J23[ip] * W;
-    pr.Wcurv[ip] = m.J31[ip] * U + m.J32[ip] * V + m.J33[ip] * W;
+    const double U = ldnt(pr.U, ip), V = ldnt(pr.V, ip), W = ldnt(pr.W, ip);
+    stnt(pr.Ucurv, ip, ldnt(m.J11, ip) * U + ldnt(m.J12, ip) * V + ldnt(m.J13, ip) * W);
+    stnt(pr.Vcurv, ip, ldnt(m.J21, ip) * U + ldnt(m.J22, ip) * V + ldnt(m.J23, ip) * W);
+    stnt(pr.Wcurv, ip, ldnt(m.J31, ip) * U + ldnt(m.J32, ip) * V + ldnt(m.J33, ip) * W);
 }
 
 // updateNonConservedData, src/CurvilinearCSolver_Core.cpp:1158-1198
 __global__ void k_eos(Cons c, Prim pr, Gas g, long n) {
     const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= n) return;
-    const double rho = c.q[0][ip];
-    const double U = c.q[1][ip] / rho, V = c.q[2][ip] / rho, W = c.q[3][ip] / rho;
-    const double p = solve_p(g, rho, c.q[4][ip], U, V, W);
+    const double rho = ldnt(c.q[0], ip);
+    const double U = ldnt(c.q[1], ip) / rho, V = ldnt(c.q[2], ip) / rho, W = ldnt(c.q[3], ip) / rho;
+    const double p = solve_p(g, rho, ldnt(c.q[4], ip), U, V, W);
     const double T = solve_T(g, rho, p);
-    pr.U[ip] = U; pr.V[ip] = V; pr.W[ip] = W;
-    pr.p[ip] = p;
-    pr.T[ip] = T;
-    pr.mu[ip] = solve_mu(g, T, ip);
-    pr.sos[ip] = solve_sos(g, rho, p);
+    stnt(pr.U, ip, U); stnt(pr.V, ip, V); stnt(pr.W, ip, W);
+    stnt(pr.p, ip, p);
+    stnt(pr.T, ip, T);
+    stnt(pr.mu, ip, solve_mu(g, T, ip));
+    stnt(pr.sos, ip, solve_sos(g, rho, p));
 }
 
 // calcDtFromCFL wave-speed maximum, src/CurvilinearCSolver_Core.cpp:197-207
@@ -87,9 +87,10 @@ __global__ __launch_bounds__(kBlock) void k_cfl(Prim pr, Metrics m, double dx, d
     const long first = (long)blockIdx.y * n;  // this member's points
     for (long ip = first + (long)blockIdx.x * blockDim.x + threadIdx.x; ip < first + n;
          ip += (long)gridDim.x * blockDim.x) {
-        const double s = pr.sos[ip];
-        const double u = m.J11[ip] * (fabs(pr.U[ip]) + s) / dx + m.J22[ip] * (fabs(pr.V[ip]) + s) / dy +
-                         m.J33[ip] * (fabs(pr.W[ip]) + s) / dz;
+        const double s = ldnt(pr.sos, ip);
+        const double u = ldnt(m.J11, ip) * (fabs(ldnt(pr.U, ip)) + s) / dx +
+                         ldnt(m.J22, ip) * (fabs(ldnt(pr.V, ip)) + s) / dy +
+                         ldnt(m.J33, ip) * (fabs(ldnt(pr.W, ip)) + s) / dz;
         v = u > v ? u : v;
     }
     red[threadIdx.x] = v;
@@ -132,13 +133,13 @@ __global__ __launch_bounds__(kBlock) void k_flux(FluxArgs a, Gas g, long n) {
     const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= n) return;
     const Metrics &m = a.m;
-    const double J11 = m.J11[ip], J12 = m.J12[ip], J13 = m.J13[ip];
-    const double J21 = m.J21[ip], J22 = m.J22[ip], J23 = m.J23[ip];
-    const double J31 = m.J31[ip], J32 = m.J32[ip], J33 = m.J33[ip];
+    const double J11 = ldnt(m.J11, ip), J12 = ldnt(m.J12, ip), J13 = ldnt(m.J13, ip);
+    const double J21 = ldnt(m.J21, ip), J22 = ldnt(m.J22, ip), J23 = ldnt(m.J23, ip);
+    const double J31 = ldnt(m.J31, ip), J32 = ldnt(m.J32, ip), J33 = ldnt(m.J33, ip);
     // velocity gradient tensor, gradU[i][j] = d u_i / d x_j
-    const double dU1 = a.dU[0][ip], dU2 = a.dU[1][ip], dU3 = a.dU[2][ip];
-    const double dV1 = a.dV[0][ip], dV2 = a.dV[1][ip], dV3 = a.dV[2][ip];
-    const double dW1 = a.dW[0][ip], dW2 = a.dW[1][ip], dW3 = a.dW[2][ip];
+    const double dU1 = ldnt(a.dU[0], ip), dU2 = ldnt(a.dU[1], ip), dU3 = ldnt(a.dU[2], ip);
+    const double dV1 = ldnt(a.dV[0], ip), dV2 = ldnt(a.dV[1], ip), dV3 = ldnt(a.dV[2], ip);
+    const double dW1 = ldnt(a.dW[0], ip), dW2 = ldnt(a.dW[1], ip), dW3 = ldnt(a.dW[2], ip);
     const double dUdx = J11 * dU1 + J21 * dU2 + J31 * dU3;
     const double dUdy = J12 * dU1 + J22 * dU2 + J32 * dU3;
     const double dUdz = J13 * dU1 + J23 * dU2 + J33 * dU3;
@@ -152,12 +153,12 @@ __global__ __launch_bounds__(kBlock) void k_flux(FluxArgs a, Gas g, long n) {
     double T22 = -(2.0 / 3.0) * dUdx + (4.0 / 3.0) * dVdy - (2.0 / 3.0) * dWdz;
     double T33 = -(2.0 / 3.0) * dUdx - (2.0 / 3.0) * dVdy + (4.0 / 3.0) * dWdz;
     double T12 = dUdy + dVdx, T13 = dUdz + dWdx, T23 = dVdz + dWdy;
-    const double mu = a.mu[ip];
+    const double mu = ldnt(a.mu, ip);
     double mu_eff, k_eff;
     if (a.les) {
-        mu_eff = mu + a.mu_sgs[ip];
-        if (a.les == 2) k_eff = g.cp * mu / g.Pr + a.k_sgs[ip];
-        else k_eff = g.cp * (mu / g.Pr + a.mu_sgs[ip] / a.Pr_t);
+        mu_eff = mu + ldnt(a.mu_sgs, ip);
+        if (a.les == 2) k_eff = g.cp * mu / g.Pr + ldnt(a.k_sgs, ip);
+        else k_eff = g.cp * (mu / g.Pr + ldnt(a.mu_sgs, ip) / a.Pr_t);
         if (mu_eff < 0.0) mu_eff = 0.0;
         if (k_eff < 0.0) k_eff = 0.0;
     } else {
@@ -167,45 +168,45 @@ __global__ __launch_bounds__(kBlock) void k_flux(FluxArgs a, Gas g, long n) {
     T11 *= mu_eff; T22 *= mu_eff; T33 *= mu_eff;
     T12 *= mu_eff; T13 *= mu_eff; T23 *= mu_eff;
     const double T21 = T12, T31 = T13, T32 = T23;
-    const double dT1 = a.dT[0][ip], dT2 = a.dT[1][ip], dT3 = a.dT[2][ip];
+    const double dT1 = ldnt(a.dT[0], ip), dT2 = ldnt(a.dT[1], ip), dT3 = ldnt(a.dT[2], ip);
     const double q1 = k_eff * (J11 * dT1 + J21 * dT2 + J31 * dT3);
     const double q2 = k_eff * (J12 * dT1 + J22 * dT2 + J32 * dT3);
     const double q3 = k_eff * (J13 * dT1 + J23 * dT2 + J33 * dT3);
-    const double U = a.U[ip], V = a.V[ip], W = a.W[ip], p = a.p[ip];
+    const double U = ldnt(a.U, ip), V = ldnt(a.V, ip), W = ldnt(a.W, ip), p = ldnt(a.p, ip);
     const double b1 = U * T11 + V * T12 + W * T13 + q1;
     const double b2 = U * T21 + V * T22 + W * T23 + q2;
     const double b3 = U * T31 + V * T32 + W * T33 + q3;
-    const double iJ = 1.0 / m.J[ip];
+    const double iJ = 1.0 / ldnt(m.J, ip);
     // contravariant velocities (as updateNonConservedData forms them, :1190-1196)
     const double Uc = J11 * U + J12 * V + J13 * W;
     const double Vc = J21 * U + J22 * V + J23 * W;
     const double Wc = J31 * U + J32 * V + J33 * W;
-    const double rho = a.rho[ip], rhoU = a.rhoU[ip], rhoV = a.rhoV[ip], rhoW = a.rhoW[ip];
-    const double rhoE = a.rhoE[ip];
-    a.pre[0][ip] = iJ * (rho * Uc);
-    a.pre[5][ip] = iJ * (rho * Vc);
-    a.pre[10][ip] = iJ * (rho * Wc);
+    const double rho = ldnt(a.rho, ip), rhoU = ldnt(a.rhoU, ip), rhoV = ldnt(a.rhoV, ip), rhoW = ldnt(a.rhoW, ip);
+    const double rhoE = ldnt(a.rhoE, ip);
+    stnt(a.pre[0], ip, iJ * (rho * Uc));
+    stnt(a.pre[5], ip, iJ * (rho * Vc));
+    stnt(a.pre[10], ip, iJ * (rho * Wc));
     double F1, F2, F3, G1, G2, G3;
     F1 = rhoU * Uc + J11 * p; F2 = rhoU * Vc + J21 * p; F3 = rhoU * Wc + J31 * p;
     G1 = J11 * T11 + J12 * T21 + J13 * T31;
     G2 = J21 * T11 + J22 * T21 + J23 * T31;
     G3 = J31 * T11 + J32 * T21 + J33 * T31;
-    a.pre[1][ip] = iJ * (F1 - G1); a.pre[6][ip] = iJ * (F2 - G2); a.pre[11][ip] = iJ * (F3 - G3);
+    stnt(a.pre[1], ip, iJ * (F1 - G1)); stnt(a.pre[6], ip, iJ * (F2 - G2)); stnt(a.pre[11], ip, iJ * (F3 - G3));
     F1 = rhoV * Uc + J12 * p; F2 = rhoV * Vc + J22 * p; F3 = rhoV * Wc + J32 * p;
     G1 = J11 * T12 + J12 * T22 + J13 * T32;
     G2 = J21 * T12 + J22 * T22 + J23 * T32;
     G3 = J31 * T12 + J32 * T22 + J33 * T32;
-    a.pre[2][ip] = iJ * (F1 - G1); a.pre[7][ip] = iJ * (F2 - G2); a.pre[12][ip] = iJ * (F3 - G3);
+    stnt(a.pre[2], ip, iJ * (F1 - G1)); stnt(a.pre[7], ip, iJ * (F2 - G2)); stnt(a.pre[12], ip, iJ * (F3 - G3));
     F1 = rhoW * Uc + J13 * p; F2 = rhoW * Vc + J23 * p; F3 = rhoW * Wc + J33 * p;
     G1 = J11 * T13 + J12 * T23 + J13 * T33;
     G2 = J21 * T13 + J22 * T23 + J23 * T33;
     G3 = J31 * T13 + J32 * T23 + J33 * T33;
-    a.pre[3][ip] = iJ * (F1 - G1); a.pre[8][ip] = iJ * (F2 - G2); a.pre[13][ip] = iJ * (F3 - G3);
+    stnt(a.pre[3], ip, iJ * (F1 - G1)); stnt(a.pre[8], ip, iJ * (F2 - G2)); stnt(a.pre[13], ip, iJ * (F3 - G3));
     F1 = (rhoE + p) * Uc; F2 = (rhoE + p) * Vc; F3 = (rhoE + p) * Wc;
     G1 = J11 * b1 + J12 * b2 + J13 * b3;
     G2 = J21 * b1 + J22 * b2 + J23 * b3;
     G3 = J31 * b1 + J32 * b2 + J33 * b3;
-    a.pre[4][ip] = iJ * (F1 - G1); a.pre[9][ip] = iJ * (F2 - G2); a.pre[14][ip] = iJ * (F3 - G3);
+    stnt(a.pre[4], ip, iJ * (F1 - G1)); stnt(a.pre[9], ip, iJ * (F2 - G2)); stnt(a.pre[14], ip, iJ * (F3 - G3));
 }
 
 // one conserved variable: k2 and the register update of stage op; returns the
@@ -214,15 +215,18 @@ __device__ __forceinline__ double rk_update(const RkArgs &a, long ip, int q, dou
     double *r1 = a.r1.q[q], *rk = a.rk.q[q], *r2 = a.r2.q[q];
     double n = 0.0;
     switch (a.op) {
-        case kTvd1: n = r1[ip] + a.c * k2; rk[ip] = n; break;                        // TVDRK3.hpp:62-80
-        case kTvd2: n = a.a * r1[ip] + a.b * rk[ip] + a.c * k2; rk[ip] = n; break;   // :82-102
-        case kTvd3: r2[ip] = a.a * r1[ip] + a.b * rk[ip] + a.c * k2; break;          // :104-124
-        case kRk4First: r2[ip] = r1[ip] + a.a * k2; n = r1[ip] + a.b * k2; rk[ip] = n; break;  // RK4.hpp:53-70
-        case kRk4Mid: r2[ip] += a.a * k2; n = r1[ip] + a.b * k2; rk[ip] = n; break;            // :72-106
-        case kRk4Last: r2[ip] += a.a * k2; break;                                               // :108-118
-        case kLsFirst: n = r1[ip] + a.b * k2; rk[ip] = n; r1[ip] = k2; break;                   // KenRK4.hpp:58-74
-        case kLsMid: { double v = k2 + a.a * r1[ip]; r1[ip] = v; n = rk[ip] + a.b * v; rk[ip] = n; break; }  // :76-127
-        case kLsLast: { double v = k2 + a.a * r1[ip]; r1[ip] = v; r2[ip] = rk[ip] + a.b * v; break; }  // :128-145
+        case kTvd1: n = ldnt(r1, ip) + a.c * k2; stnt(rk, ip, n); break;                         // TVDRK3.hpp:62-80
+        case kTvd2: n = a.a * ldnt(r1, ip) + a.b * ldnt(rk, ip) + a.c * k2; stnt(rk, ip, n); break;  // :82-102
+        case kTvd3: stnt(r2, ip, a.a * ldnt(r1, ip) + a.b * ldnt(rk, ip) + a.c * k2); break;         // :104-124
+        case kRk4First: {  // RK4.hpp:53-70
+            const double v = ldnt(r1, ip);
+            stnt(r2, ip, v + a.a * k2); n = v + a.b * k2; stnt(rk, ip, n); break;
+        }
+        case kRk4Mid: stnt(r2, ip, ldnt(r2, ip) + a.a * k2); n = ldnt(r1, ip) + a.b * k2; stnt(rk, ip, n); break;  // :72-106
+        case kRk4Last: stnt(r2, ip, ldnt(r2, ip) + a.a * k2); break;                                // :108-118
+        case kLsFirst: n = ldnt(r1, ip) + a.b * k2; stnt(rk, ip, n); stnt(r1, ip, k2); break;      // KenRK4.hpp:58-74
+        case kLsMid: { double v = k2 + a.a * ldnt(r1, ip); stnt(r1, ip, v); n = ldnt(rk, ip) + a.b * v; stnt(rk, ip, n); break; }  // :76-127
+        case kLsLast: { double v = k2 + a.a * ldnt(r1, ip); stnt(r1, ip, v); stnt(r2, ip, ldnt(rk, ip) + a.b * v); break; }  // :128-145
     }
     return n;
 }
@@ -235,28 +239,28 @@ __device__ __forceinline__ double rk_update(const RkArgs &a, long ip, int q, dou
 __global__ __launch_bounds__(kBlock) void k_rhs_rk(RkArgs a, long n) {
     const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= n) return;
-    const double J = a.J[ip];
+    const double J = ldnt(a.J, ip);
     const StepScalars *sc = a.member_pts ? a.sc + (unsigned)ip / a.member_pts : a.sc;
     const double dtJ = sc->dt * J;
     double z[5];
     if (a.bc.sponge) {  // calcSpongeSource = sigma (avg - phi), src/CurvilinearCSolver.hpp:293-295
-        const double sg = a.bc.sigma[ip];
+        const double sg = ldnt(a.bc.sigma, ip);
 #pragma unroll
-        for (int q = 0; q < 5; ++q) z[q] = (sg * (a.bc.avg[q][ip] - a.bc.qs.q[q][ip]) + 0.0) / J;
+        for (int q = 0; q < 5; ++q) z[q] = (sg * (ldnt(a.bc.avg[q], ip) - ldnt(a.bc.qs.q[q], ip)) + 0.0) / J;
     } else {
         const double z0 = (0.0 + 0.0) / J;
 #pragma unroll
         for (int q = 0; q < 5; ++q) z[q] = z0;
     }
     double k2[5];
-    k2[0] = dtJ * (-a.div[0][ip] + z[0]);
+    k2[0] = dtJ * (-ldnt(a.div[0], ip) + z[0]);
 #pragma unroll
     for (int q = 1; q < 4; ++q) {
-        double v = a.k2.q[q][ip] + (-a.div[q][ip] + z[q]);
+        double v = ldnt(a.k2.q[q], ip) + (-ldnt(a.div[q], ip) + z[q]);
         k2[q] = v * dtJ;
     }
     {
-        double v = -a.div[4][ip] + z[4];
+        double v = -ldnt(a.div[4], ip) + z[4];
         k2[4] = v * dtJ;
     }
     if (a.bc.active) {  // postStepBCHandling, src/CurvilinearCSolver_BC.cpp:702-842
@@ -266,7 +270,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_rk(RkArgs a, long n) {
 #pragma unroll
         for (int f = 0; f < 6; ++f)
             if (on[f] && a.bc.role[f] == kFaceWall) {
-                k2[0] = -sc->dt * J * a.bc.cont[f / 2][ip];
+                k2[0] = -sc->dt * J * ldnt(a.bc.cont[f / 2], ip);
                 k2[1] = 0.0; k2[2] = 0.0; k2[3] = 0.0; k2[4] = 0.0;
             }
 #pragma unroll
@@ -278,7 +282,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_rk(RkArgs a, long n) {
     double nq[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
-        a.k2.q[q][ip] = k2[q];
+        stnt(a.k2.q[q], ip, k2[q]);
         nq[q] = rk_update(a, ip, q, k2[q]);
     }
     if (a.eos) {  // as k_eos
@@ -288,11 +292,11 @@ __global__ __launch_bounds__(kBlock) void k_rhs_rk(RkArgs a, long n) {
         const double U = nq[1] / rho, V = nq[2] / rho, W = nq[3] / rho;
         const double p = solve_p(g, rho, nq[4], U, V, W);
         const double T = solve_T(g, rho, p);
-        pr.U[ip] = U; pr.V[ip] = V; pr.W[ip] = W;
-        pr.p[ip] = p;
-        pr.T[ip] = T;
-        pr.mu[ip] = solve_mu(g, T, ip);
-        pr.sos[ip] = solve_sos(g, rho, p);
+        stnt(pr.U, ip, U); stnt(pr.V, ip, V); stnt(pr.W, ip, W);
+        stnt(pr.p, ip, p);
+        stnt(pr.T, ip, T);
+        stnt(pr.mu, ip, solve_mu(g, T, ip));
+        stnt(pr.sos, ip, solve_sos(g, rho, p));
     }
 }
 
@@ -306,12 +310,12 @@ __global__ __launch_bounds__(kBlock) void k_vreman(const double *dU0, const doub
                                                    double dx, double dy, double dz, long n) {
     const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= n) return;
-    const double J11 = m.J11[ip], J12 = m.J12[ip], J13 = m.J13[ip];
-    const double J21 = m.J21[ip], J22 = m.J22[ip], J23 = m.J23[ip];
-    const double J31 = m.J31[ip], J32 = m.J32[ip], J33 = m.J33[ip];
-    const double u1 = dU0[ip], u2 = dU1[ip], u3 = dU2[ip];
-    const double v1 = dV0[ip], v2 = dV1[ip], v3 = dV2[ip];
-    const double w1 = dW0[ip], w2 = dW1[ip], w3 = dW2[ip];
+    const double J11 = ldnt(m.J11, ip), J12 = ldnt(m.J12, ip), J13 = ldnt(m.J13, ip);
+    const double J21 = ldnt(m.J21, ip), J22 = ldnt(m.J22, ip), J23 = ldnt(m.J23, ip);
+    const double J31 = ldnt(m.J31, ip), J32 = ldnt(m.J32, ip), J33 = ldnt(m.J33, ip);
+    const double u1 = ldnt(dU0, ip), u2 = ldnt(dU1, ip), u3 = ldnt(dU2, ip);
+    const double v1 = ldnt(dV0, ip), v2 = ldnt(dV1, ip), v3 = ldnt(dV2, ip);
+    const double w1 = ldnt(dW0, ip), w2 = ldnt(dW1, ip), w3 = ldnt(dW2, ip);
     const double d1v1 = J11 * u1 + J21 * u2 + J31 * u3;
     const double d2v1 = J12 * u1 + J22 * u2 + J32 * u3;
     const double d3v1 = J13 * u1 + J23 * u2 + J33 * u3;
@@ -321,7 +325,7 @@ __global__ __launch_bounds__(kBlock) void k_vreman(const double *dU0, const doub
     const double d1v3 = J11 * w1 + J21 * w2 + J31 * w3;
     const double d2v3 = J12 * w1 + J22 * w2 + J32 * w3;
     const double d3v3 = J13 * w1 + J23 * w2 + J33 * w3;
-    const double vol = dx * dy * dz / m.J[ip];
+    const double vol = dx * dy * dz / ldnt(m.J, ip);
     const double del = pow(vol, 1.0 / 3.0);
     const double d1 = del * del, d2 = d1, d3 = d1;
     const double b11 = d1 * d1v1 * d1v1 + d2 * d2v1 * d2v1 + d3 * d3v1 * d3v1;
@@ -337,9 +341,9 @@ __global__ __launch_bounds__(kBlock) void k_vreman(const double *dU0, const doub
     if (bbeta < eps) bbeta = eps;
     const double ss = sqrt(bbeta / (abeta + eps));
     const double c = 2.5 * 0.17 * 0.17;
-    const double ms = c * rho[ip] * ss;
+    const double ms = c * ldnt(rho, ip) * ss;
     const double cap = 0.05 * d1 * sqrt(2.0 * abeta);
-    mu_sgs[ip] = ms < cap ? ms : cap;
+    stnt(mu_sgs, ip, ms < cap ? ms : cap);
 }
 
 __global__ void k_fill(double *p, double v, long n) {
