@@ -97,8 +97,10 @@ __device__ __forceinline__ double bload(__amdgpu_buffer_rsrc_t r, unsigned voff,
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
 
+// results are written once and read by a later launch (from HBM at these sizes):
+// non-temporal stores (cache-policy bit 1)
 __device__ __forceinline__ void bstore(double v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 2);
 }
 
 __device__ __forceinline__ bool pass_line(const OpDev &op, int dir, int q, int Nx, int Ny) {
@@ -251,7 +253,7 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
 #pragma unroll
             for (int k = 0; k < kBatch; ++k) {
                 const int q = q0 + u.ll;
-                v[k] = q < a.nlines ? in[(long)q * N + u.p] : 0.0;
+                v[k] = q < a.nlines ? __builtin_nontemporal_load(in + (long)q * N + u.p) : 0.0;
                 u.next(qd, rd, N);
             }
 #pragma unroll
@@ -349,8 +351,7 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
                 const int qq = q0 + t.ll;
                 if (qq < a.nlines) {
                     double *g = out + (long)qq * N + t.p;
-                    if (a.accumulate) *g += v[k];
-                    else *g = v[k];
+                    __builtin_nontemporal_store(a.accumulate ? __builtin_nontemporal_load(g) + v[k] : v[k], g);
                 }
                 t.next(qd, rd, N);
             }
