@@ -77,8 +77,17 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
     auto *op = new mc3d_lineop();
     std::string err;
     int rc = 0;
-    // 32-row chunks only pay off in the fast kernels; otherwise plan 16-row chunks
-    const int mms[2] = {tune.mm, tune.mm ? tune.mm : 16};
+    // 32-row chunks only pay off in the fast kernels; otherwise plan 16-row chunks.
+    // Contiguous (x) lines are staged whole in LDS, which bounds the waves per CU;
+    // 16-row chunks double them and measured faster for tridiagonal operators
+    // at every size and for pentadiagonal ones below 256 points.
+    int first = tune.mm;
+    if (!first && dir == 0) {
+        const int w = cls == mc3d::kDeriv ? (kind == MC3D_PENTA10 ? 2 : kind == MC3D_PADE6 ? 1 : 0)
+                                          : (kind == MC3D_COMMN5PADE ? 2 : 1);
+        if (w == 1 || (w == 2 && dims[0] < 256)) first = 16;
+    }
+    const int mms[2] = {first, first ? first : 16};
     for (int mm : mms) {
         mc3d::PlanRequest rq{cls, kind, bc, dims[dir], dd, alphaF, mm};
         op->plan = mc3d::Plan();
@@ -112,7 +121,7 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
         const bool offsets_ok = npts * 8 < 2147483648L;
         op->fast = p.uniform && rows_ok && offsets_ok && mc3d::lineop_fast_supported(d, (size_t)lds_limit);
         if (tune.family == 2) op->fast = false;
-        if (op->fast || p.MM == 16 || tune.mm) break;
+        if (op->fast || p.MM == 16 || first) break;
     }
     const mc3d::Plan &p = op->plan;
     if (tune.family == 1 && !op->fast) {
