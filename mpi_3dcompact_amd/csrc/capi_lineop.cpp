@@ -79,14 +79,10 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
     int rc = 0;
     // 32-row chunks only pay off in the fast kernels; otherwise plan 16-row chunks.
     // Contiguous (x) lines are staged whole in LDS, which bounds the waves per CU;
-    // 16-row chunks double them and measured faster for tridiagonal operators
-    // at every size and for pentadiagonal ones below 256 points.
+    // 16-row chunks double them, and measured faster for every x operator but the
+    // Penta10 derivative on lines of 256 points or more.
     int first = tune.mm;
-    if (!first && dir == 0) {
-        const int w = cls == mc3d::kDeriv ? (kind == MC3D_PENTA10 ? 2 : kind == MC3D_PADE6 ? 1 : 0)
-                                          : (kind == MC3D_COMMN5PADE ? 2 : 1);
-        if (w == 1 || (w == 2 && dims[0] < 256)) first = 16;
-    }
+    if (!first && dir == 0) first = (cls == mc3d::kDeriv && kind == MC3D_PENTA10 && dims[0] >= 256) ? 0 : 16;
     const int mms[2] = {first, first ? first : 16};
     for (int mm : mms) {
         mc3d::PlanRequest rq{cls, kind, bc, dims[dir], dd, alphaF, mm};

@@ -21,9 +21,9 @@ inline dim3 grid_for(long n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)
 // gradU[i][j] = d u_i / d x_j from d u_i / d xi_k, src/CurvilinearCSolver_Core.cpp:548-561
 __device__ __forceinline__ void cart(const Metrics &m, long ip, double d0, double d1, double d2,
                                      double &gx, double &gy, double &gz) {
-    gx = m.J11[ip] * d0 + m.J21[ip] * d1 + m.J31[ip] * d2;
-    gy = m.J12[ip] * d0 + m.J22[ip] * d1 + m.J32[ip] * d2;
-    gz = m.J13[ip] * d0 + m.J23[ip] * d1 + m.J33[ip] * d2;
+    gx = ldnt(m.J11, ip) * d0 + ldnt(m.J21, ip) * d1 + ldnt(m.J31, ip) * d2;
+    gy = ldnt(m.J12, ip) * d0 + ldnt(m.J22, ip) * d1 + ldnt(m.J32, ip) * d2;
+    gz = ldnt(m.J13, ip) * d0 + ldnt(m.J23, ip) * d1 + ldnt(m.J33, ip) * d2;
 }
 
 // src/DSMSGS.hpp:127-168
@@ -31,63 +31,63 @@ __global__ __launch_bounds__(kBlock) void k_dsm_strain(DsmStrainArgs a, long n) 
     const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= n) return;
     double g00, g01, g02, g10, g11, g12, g20, g21, g22;
-    cart(a.m, ip, a.dU[0][ip], a.dU[1][ip], a.dU[2][ip], g00, g01, g02);
-    cart(a.m, ip, a.dV[0][ip], a.dV[1][ip], a.dV[2][ip], g10, g11, g12);
-    cart(a.m, ip, a.dW[0][ip], a.dW[1][ip], a.dW[2][ip], g20, g21, g22);
+    cart(a.m, ip, ldnt(a.dU[0], ip), ldnt(a.dU[1], ip), ldnt(a.dU[2], ip), g00, g01, g02);
+    cart(a.m, ip, ldnt(a.dV[0], ip), ldnt(a.dV[1], ip), ldnt(a.dV[2], ip), g10, g11, g12);
+    cart(a.m, ip, ldnt(a.dW[0], ip), ldnt(a.dW[1], ip), ldnt(a.dW[2], ip), g20, g21, g22);
     const double S00 = g00, S11 = g11, S22 = g22;
     const double S01 = 0.5 * (g01 + g10);
     const double S02 = 0.5 * (g02 + g20);
     const double S12 = 0.5 * (g12 + g21);
     const double Smag = sqrt(2.0 * (S00 * S00 + S11 * S11 + S22 * S22) +
                              4.0 * (S01 * S01 + S02 * S02 + S12 * S12));
-    a.S[0][ip] = S00; a.S[1][ip] = S01; a.S[2][ip] = S02;
-    a.S[3][ip] = S11; a.S[4][ip] = S12; a.S[5][ip] = S22;
-    a.Smag[ip] = Smag;
-    const double rho = a.rho[ip];
-    a.M2[0][ip] = 2.0 * rho * Smag * (S00 - (1.0 / 3.0) * (S00 + S11 + S22));
-    a.M2[3][ip] = 2.0 * rho * Smag * (S11 - (1.0 / 3.0) * (S00 + S11 + S22));
-    a.M2[5][ip] = 2.0 * rho * Smag * (S22 - (1.0 / 3.0) * (S00 + S11 + S22));
-    a.M2[1][ip] = 2.0 * rho * Smag * S01;
-    a.M2[2][ip] = 2.0 * rho * Smag * S02;
-    a.M2[4][ip] = 2.0 * rho * Smag * S12;
+    stnt(a.S[0], ip, S00); stnt(a.S[1], ip, S01); stnt(a.S[2], ip, S02);
+    stnt(a.S[3], ip, S11); stnt(a.S[4], ip, S12); stnt(a.S[5], ip, S22);
+    stnt(a.Smag, ip, Smag);
+    const double rho = ldnt(a.rho, ip);
+    stnt(a.M2[0], ip, 2.0 * rho * Smag * (S00 - (1.0 / 3.0) * (S00 + S11 + S22)));
+    stnt(a.M2[3], ip, 2.0 * rho * Smag * (S11 - (1.0 / 3.0) * (S00 + S11 + S22)));
+    stnt(a.M2[5], ip, 2.0 * rho * Smag * (S22 - (1.0 / 3.0) * (S00 + S11 + S22)));
+    stnt(a.M2[1], ip, 2.0 * rho * Smag * S01);
+    stnt(a.M2[2], ip, 2.0 * rho * Smag * S02);
+    stnt(a.M2[4], ip, 2.0 * rho * Smag * S12);
 }
 
 // src/DSMSGS.hpp:198-238
 __global__ __launch_bounds__(kBlock) void k_dsm_ml(DsmMLArgs a, long n) {
     const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= n) return;
-    const double rh = a.rho_hat[ip], Sh = a.Smag_hat[ip];
-    const double s00 = a.S_hat[0][ip], s01 = a.S_hat[1][ip], s02 = a.S_hat[2][ip];
-    const double s11 = a.S_hat[3][ip], s12 = a.S_hat[4][ip], s22 = a.S_hat[5][ip];
-    a.M[0][ip] = a.M[0][ip] - 2.0 * kTf2 * rh * Sh * (s00 - (1.0 / 3.0) * (s00 + s11 + s22));
-    a.M[3][ip] = a.M[3][ip] - 2.0 * kTf2 * rh * Sh * (s11 - (1.0 / 3.0) * (s00 + s11 + s22));
-    a.M[5][ip] = a.M[5][ip] - 2.0 * kTf2 * rh * Sh * (s22 - (1.0 / 3.0) * (s00 + s11 + s22));
-    a.M[1][ip] = a.M[1][ip] - 2.0 * kTf2 * rh * Sh * s01;
-    a.M[2][ip] = a.M[2][ip] - 2.0 * kTf2 * rh * Sh * s02;
-    a.M[4][ip] = a.M[4][ip] - 2.0 * kTf2 * rh * Sh * s12;
-    const double rho = a.rho[ip], ru = a.rhoU[ip], rv = a.rhoV[ip], rw = a.rhoW[ip];
-    a.Lt[0][ip] = ru * ru / rho;
-    a.Lt[1][ip] = ru * rv / rho;
-    a.Lt[2][ip] = ru * rw / rho;
-    a.Lt[3][ip] = rv * rv / rho;
-    a.Lt[4][ip] = rv * rw / rho;
-    a.Lt[5][ip] = rw * rw / rho;
+    const double rh = ldnt(a.rho_hat, ip), Sh = ldnt(a.Smag_hat, ip);
+    const double s00 = ldnt(a.S_hat[0], ip), s01 = ldnt(a.S_hat[1], ip), s02 = ldnt(a.S_hat[2], ip);
+    const double s11 = ldnt(a.S_hat[3], ip), s12 = ldnt(a.S_hat[4], ip), s22 = ldnt(a.S_hat[5], ip);
+    stnt(a.M[0], ip, ldnt(a.M[0], ip) - 2.0 * kTf2 * rh * Sh * (s00 - (1.0 / 3.0) * (s00 + s11 + s22)));
+    stnt(a.M[3], ip, ldnt(a.M[3], ip) - 2.0 * kTf2 * rh * Sh * (s11 - (1.0 / 3.0) * (s00 + s11 + s22)));
+    stnt(a.M[5], ip, ldnt(a.M[5], ip) - 2.0 * kTf2 * rh * Sh * (s22 - (1.0 / 3.0) * (s00 + s11 + s22)));
+    stnt(a.M[1], ip, ldnt(a.M[1], ip) - 2.0 * kTf2 * rh * Sh * s01);
+    stnt(a.M[2], ip, ldnt(a.M[2], ip) - 2.0 * kTf2 * rh * Sh * s02);
+    stnt(a.M[4], ip, ldnt(a.M[4], ip) - 2.0 * kTf2 * rh * Sh * s12);
+    const double rho = ldnt(a.rho, ip), ru = ldnt(a.rhoU, ip), rv = ldnt(a.rhoV, ip), rw = ldnt(a.rhoW, ip);
+    stnt(a.Lt[0], ip, ru * ru / rho);
+    stnt(a.Lt[1], ip, ru * rv / rho);
+    stnt(a.Lt[2], ip, ru * rw / rho);
+    stnt(a.Lt[3], ip, rv * rv / rho);
+    stnt(a.Lt[4], ip, rv * rw / rho);
+    stnt(a.Lt[5], ip, rw * rw / rho);
 }
 
 // src/DSMSGS.hpp:250-270
 __global__ __launch_bounds__(kBlock) void k_dsm_l(DsmLArgs a, long n) {
     const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= n) return;
-    const double uh = a.rhoU_hat[ip], vh = a.rhoV_hat[ip], wh = a.rhoW_hat[ip], rh = a.rho_hat[ip];
-    a.L[0][ip] = a.L[0][ip] - uh * uh / rh;
-    a.L[1][ip] = a.L[1][ip] - uh * vh / rh;
-    a.L[2][ip] = a.L[2][ip] - uh * wh / rh;
-    a.L[3][ip] = a.L[3][ip] - vh * vh / rh;
-    a.L[4][ip] = a.L[4][ip] - vh * wh / rh;
-    a.L[5][ip] = a.L[5][ip] - wh * wh / rh;
+    const double uh = ldnt(a.rhoU_hat, ip), vh = ldnt(a.rhoV_hat, ip), wh = ldnt(a.rhoW_hat, ip), rh = ldnt(a.rho_hat, ip);
+    stnt(a.L[0], ip, ldnt(a.L[0], ip) - uh * uh / rh);
+    stnt(a.L[1], ip, ldnt(a.L[1], ip) - uh * vh / rh);
+    stnt(a.L[2], ip, ldnt(a.L[2], ip) - uh * wh / rh);
+    stnt(a.L[3], ip, ldnt(a.L[3], ip) - vh * vh / rh);
+    stnt(a.L[4], ip, ldnt(a.L[4], ip) - vh * wh / rh);
+    stnt(a.L[5], ip, ldnt(a.L[5], ip) - wh * wh / rh);
     if (a.CIdenom) {
-        const double Smag = a.Smag[ip];
-        a.CIdenom[ip] = 2.0 * a.rho[ip] * Smag * Smag;
+        const double Smag = ldnt(a.Smag, ip);
+        stnt(a.CIdenom, ip, 2.0 * ldnt(a.rho, ip) * Smag * Smag);
     }
 }
 
@@ -96,34 +96,34 @@ __global__ __launch_bounds__(kBlock) void k_dsm_contract(DsmContractArgs a, long
     const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= n) return;
     if (a.alpha_hat) {
-        const double Sh = a.Smag_hat[ip];
-        a.CIdenom[ip] = 2.0 * kTf2 * a.rho_hat[ip] * Sh * Sh - a.alpha_hat[ip];
+        const double Sh = ldnt(a.Smag_hat, ip);
+        stnt(a.CIdenom, ip, 2.0 * kTf2 * ldnt(a.rho_hat, ip) * Sh * Sh - ldnt(a.alpha_hat, ip));
     }
-    const double L00 = a.L[0][ip], L01 = a.L[1][ip], L02 = a.L[2][ip];
-    const double L11 = a.L[3][ip], L12 = a.L[4][ip], L22 = a.L[5][ip];
-    const double M00 = a.M[0][ip], M01 = a.M[1][ip], M02 = a.M[2][ip];
-    const double M11 = a.M[3][ip], M12 = a.M[4][ip], M22 = a.M[5][ip];
+    const double L00 = ldnt(a.L[0], ip), L01 = ldnt(a.L[1], ip), L02 = ldnt(a.L[2], ip);
+    const double L11 = ldnt(a.L[3], ip), L12 = ldnt(a.L[4], ip), L22 = ldnt(a.L[5], ip);
+    const double M00 = ldnt(a.M[0], ip), M01 = ldnt(a.M[1], ip), M02 = ldnt(a.M[2], ip);
+    const double M11 = ldnt(a.M[3], ip), M12 = ldnt(a.M[4], ip), M22 = ldnt(a.M[5], ip);
     const double Lkk = L00 + L11 + L22;
-    a.Lkk[ip] = Lkk;
-    a.LijMij[ip] = (L00 - (1.0 / 3.0) * Lkk) * M00 + (L11 - (1.0 / 3.0) * Lkk) * M11 +
-                   (L22 - (1.0 / 3.0) * Lkk) * M22 + 2.0 * L01 * M01 + 2.0 * L02 * M02 +
-                   2.0 * L12 * M12;
-    a.MijMij[ip] = M00 * M00 + M11 * M11 + M22 * M22 + 2.0 * M01 * M01 + 2.0 * M02 * M02 +
-                   2.0 * M12 * M12;
+    stnt(a.Lkk, ip, Lkk);
+    stnt(a.LijMij, ip, (L00 - (1.0 / 3.0) * Lkk) * M00 + (L11 - (1.0 / 3.0) * Lkk) * M11 +
+                           (L22 - (1.0 / 3.0) * Lkk) * M22 + 2.0 * L01 * M01 + 2.0 * L02 * M02 +
+                           2.0 * L12 * M12);
+    stnt(a.MijMij, ip, M00 * M00 + M11 * M11 + M22 * M22 + 2.0 * M01 * M01 + 2.0 * M02 * M02 +
+                           2.0 * M12 * M12);
 }
 
 // src/DSMSGS.hpp:314-341
 __global__ __launch_bounds__(kBlock) void k_dsm_mu(DsmMuArgs a, long n) {
     const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= n) return;
-    const double rho = a.rho[ip], Smag = a.Smag[ip];
-    a.mu_sgs[ip] = rho * Smag * a.LijMij_avg[ip] / a.MijMij_avg[ip];
-    a.taukk[ip] = a.use_taukk ? 2.0 * rho * Smag * Smag * a.Lkk_avg[ip] / a.CIdenom_avg[ip] : 0.0;
+    const double rho = ldnt(a.rho, ip), Smag = ldnt(a.Smag, ip);
+    stnt(a.mu_sgs, ip, rho * Smag * ldnt(a.LijMij_avg, ip) / ldnt(a.MijMij_avg, ip));
+    stnt(a.taukk, ip, a.use_taukk ? 2.0 * rho * Smag * Smag * ldnt(a.Lkk_avg, ip) / ldnt(a.CIdenom_avg, ip) : 0.0);
     if (a.C) {
-        const double vol = a.dx * a.dy * a.dz / a.J[ip];
+        const double vol = a.dx * a.dy * a.dz / ldnt(a.J, ip);
         const double del = pow(vol, 1.0 / 3.0);
-        a.C[ip] = a.LijMij_avg[ip] / (del * del * a.MijMij_avg[ip]);
-        a.CI[ip] = a.use_taukk ? a.Lkk_avg[ip] / (del * del * a.CIdenom_avg[ip]) : 0.0;
+        stnt(a.C, ip, ldnt(a.LijMij_avg, ip) / (del * del * ldnt(a.MijMij_avg, ip)));
+        stnt(a.CI, ip, a.use_taukk ? ldnt(a.Lkk_avg, ip) / (del * del * ldnt(a.CIdenom_avg, ip)) : 0.0);
     }
 }
 
@@ -132,13 +132,13 @@ __global__ __launch_bounds__(kBlock) void k_dsm_taukk_eos(const double *rho, con
                                                           Prim pr, Gas g, long n) {
     const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= n) return;
-    const double r = rho[ip];
-    const double p = pr.p[ip] - 0.5 * taukk[ip];
+    const double r = ldnt(rho, ip);
+    const double p = ldnt(pr.p, ip) - 0.5 * ldnt(taukk, ip);
     const double T = p / (r * g.R_gas);
-    pr.p[ip] = p;
-    pr.T[ip] = T;
-    pr.mu[ip] = member_mu_ref(g, ip) * pow(T / g.T_ref, 0.76);
-    pr.sos[ip] = sqrt(g.gamma * p / r);
+    stnt(pr.p, ip, p);
+    stnt(pr.T, ip, T);
+    stnt(pr.mu, ip, member_mu_ref(g, ip) * pow(T / g.T_ref, 0.76));
+    stnt(pr.sos, ip, sqrt(g.gamma * p / r));
 }
 
 // src/DSMSGS.hpp:345-385
@@ -146,44 +146,44 @@ __global__ __launch_bounds__(kBlock) void k_dsm_nk(DsmNKArgs a, long n) {
     const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= n) return;
     double gx, gy, gz;
-    cart(a.m, ip, a.dT[0][ip], a.dT[1][ip], a.dT[2][ip], gx, gy, gz);
-    a.gT[0][ip] = gx; a.gT[1][ip] = gy; a.gT[2][ip] = gz;
-    const double rho = a.rho[ip], Smag = a.Smag[ip], T = a.T[ip];
-    a.N[0][ip] = rho * Smag * gx;
-    a.N[1][ip] = rho * Smag * gy;
-    a.N[2][ip] = rho * Smag * gz;
-    a.K[0][ip] = a.rhoU[ip] * T;
-    a.K[1][ip] = a.rhoV[ip] * T;
-    a.K[2][ip] = a.rhoW[ip] * T;
-    a.rhoT[ip] = rho * T;
+    cart(a.m, ip, ldnt(a.dT[0], ip), ldnt(a.dT[1], ip), ldnt(a.dT[2], ip), gx, gy, gz);
+    stnt(a.gT[0], ip, gx); stnt(a.gT[1], ip, gy); stnt(a.gT[2], ip, gz);
+    const double rho = ldnt(a.rho, ip), Smag = ldnt(a.Smag, ip), T = ldnt(a.T, ip);
+    stnt(a.N[0], ip, rho * Smag * gx);
+    stnt(a.N[1], ip, rho * Smag * gy);
+    stnt(a.N[2], ip, rho * Smag * gz);
+    stnt(a.K[0], ip, ldnt(a.rhoU, ip) * T);
+    stnt(a.K[1], ip, ldnt(a.rhoV, ip) * T);
+    stnt(a.K[2], ip, ldnt(a.rhoW, ip) * T);
+    stnt(a.rhoT, ip, rho * T);
 }
 
 // src/DSMSGS.hpp:401-425; K_i divides by rho, not rho_hat, as the reference does
 __global__ __launch_bounds__(kBlock) void k_dsm_nn(DsmNNArgs a, long n) {
     const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= n) return;
-    const double rh = a.rho_hat[ip], Sh = a.Smag_hat[ip], rTh = a.rhoT_hat[ip], rho = a.rho[ip];
-    const double N0 = kTf2 * rh * Sh * a.g_hat[0][ip] - a.NRHS[0][ip];
-    const double N1 = kTf2 * rh * Sh * a.g_hat[1][ip] - a.NRHS[1][ip];
-    const double N2 = kTf2 * rh * Sh * a.g_hat[2][ip] - a.NRHS[2][ip];
-    const double K0 = -a.rhoU_hat[ip] * rTh / rho + a.KRHS[0][ip];
-    const double K1 = -a.rhoV_hat[ip] * rTh / rho + a.KRHS[1][ip];
-    const double K2 = -a.rhoW_hat[ip] * rTh / rho + a.KRHS[2][ip];
-    a.NiNi[ip] = N0 * N0 + N1 * N1 + N2 * N2;
-    a.NiKi[ip] = K0 * N0 + K1 * N1 + K2 * N2;
+    const double rh = ldnt(a.rho_hat, ip), Sh = ldnt(a.Smag_hat, ip), rTh = ldnt(a.rhoT_hat, ip), rho = ldnt(a.rho, ip);
+    const double N0 = kTf2 * rh * Sh * ldnt(a.g_hat[0], ip) - ldnt(a.NRHS[0], ip);
+    const double N1 = kTf2 * rh * Sh * ldnt(a.g_hat[1], ip) - ldnt(a.NRHS[1], ip);
+    const double N2 = kTf2 * rh * Sh * ldnt(a.g_hat[2], ip) - ldnt(a.NRHS[2], ip);
+    const double K0 = -ldnt(a.rhoU_hat, ip) * rTh / rho + ldnt(a.KRHS[0], ip);
+    const double K1 = -ldnt(a.rhoV_hat, ip) * rTh / rho + ldnt(a.KRHS[1], ip);
+    const double K2 = -ldnt(a.rhoW_hat, ip) * rTh / rho + ldnt(a.KRHS[2], ip);
+    stnt(a.NiNi, ip, N0 * N0 + N1 * N1 + N2 * N2);
+    stnt(a.NiKi, ip, K0 * N0 + K1 * N1 + K2 * N2);
 }
 
 // src/DSMSGS.hpp:427-444
 __global__ __launch_bounds__(kBlock) void k_dsm_ksgs(DsmKArgs a, long n) {
     const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= n) return;
-    const double NiNi = a.NiNi_avg[ip], NiKi = a.NiKi_avg[ip];
+    const double NiNi = ldnt(a.NiNi_avg, ip), NiKi = ldnt(a.NiKi_avg, ip);
     const double C_over_Prt = NiKi / NiNi;
-    a.k_sgs[ip] = -a.cp * C_over_Prt * a.rho[ip] * a.Smag[ip];
+    stnt(a.k_sgs, ip, -a.cp * C_over_Prt * ldnt(a.rho, ip) * ldnt(a.Smag, ip));
     if (a.Prt) {
-        const double vol = a.dx * a.dy * a.dz / a.J[ip];
+        const double vol = a.dx * a.dy * a.dz / ldnt(a.J, ip);
         const double del = pow(vol, 1.0 / 3.0);
-        a.Prt[ip] = -a.C[ip] * del * del * NiNi / NiKi;
+        stnt(a.Prt, ip, -ldnt(a.C, ip) * del * del * NiNi / NiKi);
     }
 }
 

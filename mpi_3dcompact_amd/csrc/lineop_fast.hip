@@ -22,6 +22,8 @@
 // through LDS.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "lineop.h"
 #include "lineop_launch.h"
 
@@ -235,8 +237,13 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
     constexpr int L = contig_lines<PER>();
     constexpr int LS = L + 1;  // row stride of the transposed tile
     const int N = op.N, P = op.P, NI = op.NI;
-    double *tile = smem;                            // [N + 2H][LS], row e holds phi[e - H]
-    double *red = smem + (size_t)(N + 2 * H) * LS;  // [L][NI+1]
+    double *tile = smem;  // [N + 2H][LS], row e holds phi[e - H]
+    // [L][NI+1] interface values.  Pentadiagonal operators keep them inside the
+    // tile, which is dead from the window reads to the write-back: their
+    // interface is twice as large, and sharing lets two workgroups fit a CU
+    // (two more barriers); tridiagonal ones fit anyway and append it.
+    constexpr bool kShare = W == 2;
+    double *red = kShare ? smem : smem + (size_t)(N + 2 * H) * LS;
     const int tid = threadIdx.y * L + threadIdx.x;
     const int nthr = L * P;                         // L * N = nthr * MM elements
     const int qd = nthr / N, rd = nthr - qd * N;
@@ -283,6 +290,7 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
     const double *trow = tile + (size_t)cs * LS + l;  // row cs holds phi[cs - H]
 #pragma unroll
     for (int j = 0; j < MM + 2 * H; ++j) w[j] = trow[j * LS];
+    if (kShare) __syncthreads();  // every window is read before the interface values overwrite the tile
     double y[MM];
     chunk_rhs<H, MM, PER>(op, w, y, c == 0, c == P - 1);
     if (W > 0) {
@@ -327,9 +335,14 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
             for (int j = 0; j < 2 * W; ++j) x -= sp[i * 2 * W + j] * s[j];
             y[i] = x;
         }
+        if (kShare) __syncthreads();  // every interface value is read before the results overwrite the tile
     }
     const bool pass = live && pass_line(op, 0, q, a.Nx, a.Ny);
-    if (live && !pass) {
+    if (kShare && pass) {  // the tile no longer holds the input: reload it (nothing is stored yet)
+#pragma unroll
+        for (int i = 0; i < MM; ++i) y[i] = in[(long)q * N + cs + i];
+    }
+    if (live && (kShare || !pass)) {
 #pragma unroll
         for (int i = 0; i < MM; ++i) tile[(cs + H + i) * LS + l] = y[i];
     }
@@ -388,9 +401,10 @@ hipError_t launch_fast_mm(const OpDev &op, const LaunchArgs &a, int nfields, hip
 }  // namespace
 
 size_t lineop_fast_lds_bytes(const OpDev &op) {
-    if (op.dir == 0) {
+    if (op.dir == 0) {  // the tile, with the interface values inside it (W = 2) or after it
         const size_t L = op.periodic ? contig_lines<true>() : contig_lines<false>();
-        return sizeof(double) * ((size_t)(op.N + 2 * op.H) * (L + 1) + L * (op.NI + 1));
+        const size_t tile = (size_t)(op.N + 2 * op.H) * (L + 1), red = L * (op.NI + 1);
+        return sizeof(double) * (op.W == 2 ? std::max(tile, red) : tile + red);
     }
     return sizeof(double) * (size_t)64 * (op.NI + 1);
 }
