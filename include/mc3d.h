@@ -185,6 +185,13 @@ int mc3d_solver_set_output(mc3d_solver *s, const char *dir, const double *x, con
  * Overwrites the velocity-derivative work fields.  Synchronous. */
 int mc3d_solver_hit_stats(mc3d_solver *s, double stats[11]);
 int mc3d_solver_field_device(mc3d_solver *s, const char *name, double **dev_ptr);
+/* Running statistics ([STATS] VELOCITYSTATS / THERMOSTATS, src/Stats.hpp):
+ * their accumulated weights (the time they cover); the fields are read with
+ * mc3d_solver_get_field as UAVG VAVG WAVG URMS VRMS WRMS UVREY UWREY VWREY and
+ * RHOAVG RHORMS PAVG PRMS TAVG TRMS.  Every SolutionDump is accompanied by
+ * VelocityStats.<step> / ThermoStats.<step> (weight, then those fields) and,
+ * with SPONGE faces, SpongeAvgDump.<step> (the five sponge averages). */
+int mc3d_solver_stats_weights(mc3d_solver *s, double *velocity, double *thermo);
 int mc3d_solver_scalars(mc3d_solver *s, double *time, double *dt, int *timestep);
 /* the solver's HIP stream (as void*), for timing with events */
 int mc3d_solver_stream(mc3d_solver *s, void **stream);

@@ -284,6 +284,14 @@ int mc3d_solver_timing_report(mc3d_solver *s, char *buf, int buflen, int *needed
     return MC3D_OK;
 }
 
+int mc3d_solver_stats_weights(mc3d_solver *s, double *velocity, double *thermo) {
+    if (!s) return mc3d::fail(MC3D_EINVAL, "null solver");
+    const mc3d::Stats *st = s->cs->stats.get();
+    if (velocity) *velocity = st ? st->velocityStatsWeight : 0.0;
+    if (thermo) *thermo = st ? st->thermoStatsWeight : 0.0;
+    return MC3D_OK;
+}
+
 int mc3d_solver_set_closure(mc3d_solver *s, const mc3d_closure *c) {
     if (!s || !c) return mc3d::fail(MC3D_EINVAL, "null argument");
     auto *l = dynamic_cast<mc3d::LearnedSGS *>(s->cs->les.get());

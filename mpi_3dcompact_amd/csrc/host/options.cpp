@@ -208,13 +208,23 @@ int Options::parse(const std::string &text, Options &o, std::string &err) {
     }
     r.req("RESTART.SPONGEAVGFROMRESTART", o.spongeFromRestart);
     if (o.spongeFromRestart) r.req("RESTART.SPONGEAVGFILENAME", o.sponge_filename);
+    // src/Options.hpp:394-417
     r.opt("STATS.VELOCITYSTATS", o.velocityStats);
+    if (o.velocityStats) {
+        r.req("STATS.VELOCITYSTATSFROMRESTART", o.velocityStatsFromRestart);
+        if (o.velocityStatsFromRestart) r.req("STATS.VELOCITYSTATSFILENAME", o.velocityStatsFilename);
+    }
     r.opt("STATS.THERMOSTATS", o.thermoStats);
+    if (o.thermoStats) {
+        r.req("STATS.THERMOSTATSFROMRESTART", o.thermoStatsFromRestart);
+        if (o.thermoStatsFromRestart) r.req("STATS.THERMOSTATSFILENAME", o.thermoStatsFilename);
+    }
     const std::initializer_list<std::pair<const char *, StatsAvgType>> av = {
         {"NONE", O::NONE}, {"XI1_AVG", O::XI1_AVG}, {"XI2_AVG", O::XI2_AVG},
         {"XI3_AVG", O::XI3_AVG}, {"LOCAL", O::LOCAL}, {"GLOBAL", O::GLOBAL}};
     r.enm("STATS.STATSAVGTYPE", o.statsAvgType, av);
     r.opt("STATS.INTERVAL", o.stats_interval);
+    if ((o.velocityStats || o.thermoStats) && o.stats_interval <= 0) r.fail("STATS.INTERVAL must be positive");
     // NONE is accepted as an alias of NOMODEL (the reference README spells it so,
     // its parser does not: src/Options.cpp:6-15 vs README.md:47-49)
     r.enm("LES.LESMODEL", o.lesModel, {{"NOMODEL", O::NOMODEL}, {"NONE", O::NOMODEL},

@@ -54,6 +54,8 @@ struct Options {
     // [STATS]
     StatsAvgType statsAvgType = NONE;
     bool velocityStats = false, thermoStats = false;
+    bool velocityStatsFromRestart = false, thermoStatsFromRestart = false;
+    std::string velocityStatsFilename, thermoStatsFilename;
     int stats_interval = 5;
     // [LES]
     StatsAvgType musgsAvgType = NONE;

@@ -7,6 +7,7 @@
 #include <vector>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 
 #include "../dsm.h"
 #include "../stats.h"
@@ -418,6 +419,8 @@ CurvilinearCSolver::CurvilinearCSolver(const Options &o, hipStream_t s, int memb
             opt.musgsAvgType != Options::LOCAL)
             throw Error(MC3D_ECONFIG, "stacked members take LESAVERAGING NONE or LOCAL");
         if (N > 268435455L) throw Error(MC3D_EINVAL, "stacked members above 2^28 points per field");
+        if (opt.velocityStats || opt.thermoStats)
+            throw Error(MC3D_ECONFIG, "stacked members keep no running statistics");
     }
     timeStep = opt.timeStep;
     // src/IdealGas.hpp:35-48
@@ -496,6 +499,7 @@ CurvilinearCSolver::CurvilinearCSolver(const Options &o, hipStream_t s, int memb
     if (opt.lesModel == Options::VREMAN) les.reset(new VremanSGS(this));
     else if (opt.lesModel == Options::DSM) les.reset(new DSMSGS(this));
     else if (opt.lesModel == Options::LEARNED) les.reset(new LearnedSGS(this));
+    if (opt.velocityStats || opt.thermoStats) stats.reset(new Stats(this));
     // src/MPI_3DCompact.cpp:164-176
     switch (opt.rkType) {
         case Options::TVDRK3: rk.reset(new TVDRK3(this)); break;
@@ -738,6 +742,25 @@ void CurvilinearCSolver::dumpSolution(const std::string &path, const double *con
     std::string name = path;
     if (name.empty()) name = (out_dir_.empty() ? std::string() : out_dir_ + "/") + dump_name(timeStep);
     write_dump(name, d, fields);
+    // the sponge averages and the statistics go beside it (src/CurvilinearCSolver_Core.cpp:1413-1419)
+    const size_t slash = name.find_last_of('/');
+    const std::string dir = slash == std::string::npos ? std::string() : name.substr(0, slash);
+    if (sponge_) dumpSpongeAvg(dir);
+    if (stats) stats->dumpStatsFields(dir, timeStep);
+}
+
+void CurvilinearCSolver::dumpSpongeAvg(const std::string &dir) {
+    const std::string name = (dir.empty() ? std::string() : dir + "/") + "SpongeAvgDump." + std::to_string(timeStep);
+    std::ofstream out(name, std::ios::binary | std::ios::trunc);
+    if (!out) throw Error(MC3D_EINVAL, "cannot write " + name);
+    std::vector<double> buf((size_t)N);
+    for (int q = 0; q < 5; ++q) {
+        hip_check(hipMemcpyAsync(buf.data(), spgAvg_[q], sizeof(double) * N, hipMemcpyDeviceToHost, st),
+                  "sponge read-back");
+        hip_check(hipStreamSynchronize(st), "sponge read-back");
+        out.write(reinterpret_cast<const char *>(buf.data()), (std::streamsize)(sizeof(double) * N));
+    }
+    if (!out) throw Error(MC3D_EINVAL, "write failed: " + name);
 }
 
 void CurvilinearCSolver::set_output(const std::string &dir, const double *const mesh[3]) {
@@ -1055,11 +1078,26 @@ void CurvilinearCSolver::checkSolution() {
             }
         }
     }
+    // src/CurvilinearCSolver_Core.cpp:1295-1316
+    if (stats) {
+        if (stats->velocityStats)
+            for (const char *f : {"UAVG", "URMS", "VAVG", "VRMS", "WAVG", "WRMS", "UVREY", "UWREY", "VWREY"})
+                getRange(stats->field(f), f);
+        if (stats->thermoStats)
+            for (const char *f : {"RHOAVG", "RHORMS", "PAVG", "PRMS", "TAVG", "TRMS"}) getRange(stats->field(f), f);
+    }
     std::fflush(stdout);
 }
 
 void CurvilinearCSolver::postStep() {
     checkSolution();
+    // src/CurvilinearCSolver.hpp:362-364
+    if (stats && timeStep % opt.stats_interval == 0) {
+        StepScalars h;
+        hip_check(hipMemcpyAsync(&h, sc, sizeof(h), hipMemcpyDeviceToHost, st), "scalar read");
+        hip_check(hipStreamSynchronize(st), "scalar read");
+        stats->updateStatsFields(h.time);
+    }
     const double *mesh[3] = {mesh_[0].empty() ? nullptr : mesh_[0].data(),
                              mesh_[1].empty() ? nullptr : mesh_[1].data(),
                              mesh_[2].empty() ? nullptr : mesh_[2].data()};
@@ -1108,6 +1146,8 @@ double *CurvilinearCSolver::field_by_name(const std::string &n) {
         return n[1] == 'U' ? dU[d] : n[1] == 'V' ? dV[d] : n[1] == 'W' ? dW[d] : dT[d];
     }
     if (n == "sigma") return sigma_;
+    if (stats)
+        if (double *p = stats->field(n)) return p;
     if (n.size() > 3 && n.compare(n.size() - 3, 3, "Avg") == 0)  // sponge averages rhoAvg .. rhoEAvg
         for (int q = 0; q < 5; ++q)
             if (n == std::string(cons[q]) + "Avg") return spgAvg_[q];

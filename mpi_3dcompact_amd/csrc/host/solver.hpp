@@ -18,6 +18,7 @@
 #include "../pointwise.h"
 #include "mc3d.h"
 #include "options.hpp"
+#include "runstats.hpp"
 #include "timer.hpp"
 
 namespace mc3d {
@@ -299,6 +300,7 @@ class CurvilinearCSolver {
     std::unique_ptr<AbstractFilter> filtX, filtY, filtZ;
     std::unique_ptr<AbstractSGS> les;
     std::unique_ptr<AbstractRK> rk;
+    std::unique_ptr<Stats> stats;  // STATS.VELOCITYSTATS / THERMOSTATS (src/CurvilinearCSolver.hpp:224-229)
 
     // device fields (one slab)
     Metrics msh{};
@@ -387,6 +389,9 @@ class CurvilinearCSolver {
     bool hasSponge() const { return sponge_; }
     double *sigmaField() const { return sigma_; }
     double *spongeAvg(int q) const { return spgAvg_[q]; }
+    // SpongeAvgDump.<timeStep> in dir: the five sponge averages (SpongeBC::dumpSpongeAvg,
+    // src/SpongeBC.cpp:363-420)
+    void dumpSpongeAvg(const std::string &dir);
     // learned closure: training pairs (features, DSM coefficients) of the
     // current state; the DSM runs on it and the fields its tau_kk update
     // changes are restored afterwards

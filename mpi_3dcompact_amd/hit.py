@@ -81,10 +81,11 @@ def options_text(N, fd="PADE6", filt="COMPACT10", alpha_f=0.49, rk="TVDRK3", cfl
                  filter_step=1, substep_filtering=False, dsm_filter=1, use_taukk=True,
                  coeff_range=False, ts_type="CONST_CFL", dt=None, restart_file=None,
                  dump_step=1000000, disp=None, closure_file=None, closure_fallback=True,
-                 faces=None, sponge=None, closure_fallback_fraction=None):
+                 faces=None, sponge=None, closure_fallback_fraction=None, stats=None):
     """MPI_3DCompact.in text for a periodic HIT box of N^3 points (restart_file:
     start from that SolutionDump, which also sets the grid; disp: 3x3
-    PERIODICDISP rows, default the 2 pi box)."""
+    PERIODICDISP rows, default the 2 pi box; stats: dict velocity, thermo (bools),
+    avg (STATSAVGTYPE name), interval)."""
     fds = fd if isinstance(fd, (list, tuple)) else (fd, fd, fd)
     lines = ["[RESTART]"]
     if restart_file:
@@ -99,7 +100,14 @@ def options_text(N, fd="PADE6", filt="COMPACT10", alpha_f=0.49, rk="TVDRK3", cfl
         "MAXTIME = 1000000.0", f"MAXTIMESTEP = {max_steps}", f"FILTERSTEP = {filter_step}",
         f"SUBSTEPFILTERING = {'TRUE' if substep_filtering else 'FALSE'}",
         "CHECKSTEP = 1000000", f"DUMPSTEP = {dump_step}",
-        "[STATS]", "STATSAVGTYPE = NONE", "VELOCITYSTATS = FALSE", "THERMOSTATS = FALSE",
+        "[STATS]", f"STATSAVGTYPE = {(stats or {}).get('avg', 'NONE')}",
+        f"VELOCITYSTATS = {'TRUE' if (stats or {}).get('velocity') else 'FALSE'}",
+        f"THERMOSTATS = {'TRUE' if (stats or {}).get('thermo') else 'FALSE'}",
+        "VELOCITYSTATSFROMRESTART = " + ("TRUE" if (stats or {}).get("velocity_file") else "FALSE"),
+        "THERMOSTATSFROMRESTART = " + ("TRUE" if (stats or {}).get("thermo_file") else "FALSE"),
+    ] + ([f"VELOCITYSTATSFILENAME = {stats['velocity_file']}"] if (stats or {}).get("velocity_file") else []) + (
+        [f"THERMOSTATSFILENAME = {stats['thermo_file']}"] if (stats or {}).get("thermo_file") else []) + [
+        f"INTERVAL = {int((stats or {}).get('interval', 5))}",
         "[BC]"]
     for d, a in enumerate("XYZ"):
         per = fk[2 * d] == "PERIODIC"
@@ -177,7 +185,7 @@ _AVG = {"NONE": 0, "XI1_AVG": 1, "XI2_AVG": 2, "XI3_AVG": 3, "LOCAL": 4, "GLOBAL
 def oracle_config(N, fd="PADE6", filt="COMPACT10", alpha_f=0.49, rk="TVDRK3", cfl=0.5,
                   mu_ref=1.0 / 1600.0, les="NOMODEL", les_avg="NONE", filter_step=1,
                   substep_filtering=False, dsm_filter=1, use_taukk=True, coeff_range=False,
-                  ts_type="CONST_CFL", dt=None, faces=None, **_):
+                  ts_type="CONST_CFL", dt=None, faces=None, stats=None, **_):
     """The same case as options_text(), as the field dict of the CPU oracle's config."""
     fds = fd if isinstance(fd, (list, tuple)) else (fd, fd, fd)
     fk = face_kinds(faces)
@@ -190,4 +198,8 @@ def oracle_config(N, fd="PADE6", filt="COMPACT10", alpha_f=0.49, rk="TVDRK3", cf
                 ts_type=1 if ts_type == "CONST_CFL" else 0, cfl=cfl if cfl else 0.0,
                 dt=dt if dt else 0.0, filter_step=filter_step,
                 substep_filtering=int(substep_filtering), dx=sp[0], dy=sp[1], dz=sp[2],
-                timestep0=0, time0=0.0)
+                timestep0=0, time0=0.0,
+                velocity_stats=int(bool((stats or {}).get("velocity"))),
+                thermo_stats=int(bool((stats or {}).get("thermo"))),
+                stats_avg=_AVG[(stats or {}).get("avg", "NONE")],
+                stats_interval=int((stats or {}).get("interval", 5)))

@@ -30,6 +30,7 @@ _SIGS = [
     ("mc3d_solver_sample_closure", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_long, ctypes.c_long, ctypes.c_long]),
     ("mc3d_solver_eval_sgs", ctypes.c_int, [_VP]),
     ("mc3d_solver_hit_stats", ctypes.c_int, [_VP, _D]),
+    ("mc3d_solver_stats_weights", ctypes.c_int, [_VP, _D, _D]),
     ("mc3d_solver_closure_stats", ctypes.c_int, [_VP] + [ctypes.POINTER(ctypes.c_long)] * 4),
     ("mc3d_solver_timing_report", ctypes.c_int, [_VP, ctypes.c_char_p, ctypes.c_int,
                                                  ctypes.POINTER(ctypes.c_int)]),
@@ -106,6 +107,12 @@ class Solver:
         out = (ctypes.c_double * 11)()
         check(self._L.mc3d_solver_hit_stats(self._h, out))
         return dict(zip(self.HIT_STATS, out[:]))
+
+    def stats_weights(self):
+        """Accumulated weights (time covered) of the velocity and thermo statistics."""
+        v, t = ctypes.c_double(), ctypes.c_double()
+        check(self._L.mc3d_solver_stats_weights(self._h, ctypes.byref(v), ctypes.byref(t)))
+        return v.value, t.value
 
     def closure_stats(self):
         v = [ctypes.c_long() for _ in range(4)]

@@ -58,6 +58,88 @@ struct orc_solver {
     // sponge (src/SpongeBC.hpp:38-45)
     bool spongeFlag = false;
     Vec sigma, spgAvg[5];
+    // running statistics (src/Stats.hpp:19-40): UAVG VAVG WAVG URMS VRMS WRMS UVREY
+    // UWREY VWREY, then RHOAVG RHORMS PAVG PRMS TAVG TRMS
+    Vec vstat[9], tstat[6];
+    double velWeight = 0.0, thermoWeight = 0.0, statsPrevTime = 0.0;
+
+    // Stats::doAveraging (src/Stats.cpp:248-345): XI1 / XI3 replace phi by its
+    // average along the line, summing phi / N in line order; the XI2 branch
+    // averages a scratch array and leaves phi as it is (:274-297); the other
+    // types do nothing
+    void statsAveraging(Vec &phi) {
+        const int Nx = c.Nx, Ny = c.Ny, Nz = c.Nz;
+        if (c.stats_avg == ORC_AVG_XI1) {
+            for (int k = 0; k < Nz; ++k)
+                for (int j = 0; j < Ny; ++j) {
+                    double a = 0.0;
+                    for (int i = 0; i < Nx; ++i) a += phi[idx(i, j, k)] / (double)Nx;
+                    for (int i = 0; i < Nx; ++i) phi[idx(i, j, k)] = a;
+                }
+        } else if (c.stats_avg == ORC_AVG_XI3) {
+            for (int j = 0; j < Ny; ++j)
+                for (int i = 0; i < Nx; ++i) {
+                    double a = 0.0;
+                    for (int k = 0; k < Nz; ++k) a += phi[idx(i, j, k)] / (double)Nz;
+                    for (int k = 0; k < Nz; ++k) phi[idx(i, j, k)] = a;
+                }
+        }
+    }
+
+    // Stats::updateStatsFields (src/Stats.cpp:121-246): time-weighted running
+    // means, RMS and Reynolds stresses of U V W (the current velocities) and of
+    // rho2 (the stage register before the filter), p and T
+    void updateStats() {
+        const double dts = time - statsPrevTime;
+        if (c.velocity_stats && dts != 0.0) {
+            const double oW = velWeight, tW = oW + dts;
+            Vec &UA = vstat[0], &VA = vstat[1], &WA = vstat[2], &UR = vstat[3], &VR = vstat[4], &WR = vstat[5];
+            Vec &UV = vstat[6], &UW = vstat[7], &VW = vstat[8];
+            for (long ip = 0; ip < n; ++ip) {
+                UR[ip] = (oW * (UR[ip] * UR[ip] + UA[ip] * UA[ip]) + dts * U[ip] * U[ip]) / tW;
+                VR[ip] = (oW * (VR[ip] * VR[ip] + VA[ip] * VA[ip]) + dts * V[ip] * V[ip]) / tW;
+                WR[ip] = (oW * (WR[ip] * WR[ip] + WA[ip] * WA[ip]) + dts * W[ip] * W[ip]) / tW;
+                UV[ip] = (oW * (UV[ip] + UA[ip] * VA[ip]) + dts * U[ip] * V[ip]) / tW;
+                UW[ip] = (oW * (UW[ip] + UA[ip] * WA[ip]) + dts * U[ip] * W[ip]) / tW;
+                VW[ip] = (oW * (VW[ip] + VA[ip] * WA[ip]) + dts * V[ip] * W[ip]) / tW;
+                UA[ip] = (oW * UA[ip] + dts * U[ip]) / tW;
+                VA[ip] = (oW * VA[ip] + dts * V[ip]) / tW;
+                WA[ip] = (oW * WA[ip] + dts * W[ip]) / tW;
+            }
+            const int order[9] = {3, 4, 5, 6, 7, 8, 0, 1, 2};
+            for (int q : order) statsAveraging(vstat[q]);
+            for (long ip = 0; ip < n; ++ip) {
+                UR[ip] = std::sqrt(std::fabs(UR[ip] - UA[ip] * UA[ip]));
+                VR[ip] = std::sqrt(std::fabs(VR[ip] - VA[ip] * VA[ip]));
+                WR[ip] = std::sqrt(std::fabs(WR[ip] - WA[ip] * WA[ip]));
+                UV[ip] = UV[ip] - UA[ip] * VA[ip];
+                UW[ip] = UW[ip] - UA[ip] * WA[ip];
+                VW[ip] = VW[ip] - VA[ip] * WA[ip];
+            }
+            velWeight = tW;
+        }
+        if (c.thermo_stats && dts != 0.0) {
+            const double oW = thermoWeight, tW = oW + dts;
+            Vec &RA = tstat[0], &RR = tstat[1], &PA = tstat[2], &PR = tstat[3], &TA = tstat[4], &TR = tstat[5];
+            for (long ip = 0; ip < n; ++ip) {
+                RR[ip] = (oW * (RR[ip] * RR[ip] + RA[ip] * RA[ip]) + dts * rho2[ip] * rho2[ip]) / tW;
+                PR[ip] = (oW * (PR[ip] * PR[ip] + PA[ip] * PA[ip]) + dts * p[ip] * p[ip]) / tW;
+                TR[ip] = (oW * (TR[ip] * TR[ip] + TA[ip] * TA[ip]) + dts * T[ip] * T[ip]) / tW;
+                RA[ip] = (oW * RA[ip] + dts * rho2[ip]) / tW;
+                PA[ip] = (oW * PA[ip] + dts * p[ip]) / tW;
+                TA[ip] = (oW * TA[ip] + dts * T[ip]) / tW;
+            }
+            const int order[6] = {1, 3, 5, 0, 2, 4};
+            for (int q : order) statsAveraging(tstat[q]);
+            for (long ip = 0; ip < n; ++ip) {
+                RR[ip] = std::sqrt(std::fabs(RR[ip] - RA[ip] * RA[ip]));
+                PR[ip] = std::sqrt(std::fabs(PR[ip] - PA[ip] * PA[ip]));
+                TR[ip] = std::sqrt(std::fabs(TR[ip] - TA[ip] * TA[ip]));
+            }
+            thermoWeight = tW;
+        }
+        statsPrevTime = time;
+    }
 
     // ---------------- boundary faces ----------------
     long idx(int i, int j, int k) const { return ((long)k * c.Ny + j) * c.Nx + i; }
@@ -748,6 +830,9 @@ struct orc_solver {
             updateNonConservedData();
         }
         if (c.rk == ORC_LSLDDRK4) dualStep = dualStep == 1 ? 2 : 1;
+        // postStep, src/CurvilinearCSolver.hpp:355-365
+        if ((c.velocity_stats || c.thermo_stats) && c.stats_interval > 0 && timeStep % c.stats_interval == 0)
+            updateStats();
     }
 };
 
@@ -802,6 +887,12 @@ orc_solver *orc_solver_create(const orc_config *cfg, const double *const metrics
     s->time = cfg->time0;
     s->dt = cfg->dt;
     s->cfl = cfg->cfl;
+    // src/Stats.hpp:42-250: zero fields and weights, prev_time = the current time
+    if (cfg->velocity_stats)
+        for (auto &v : s->vstat) v.assign(n, 0.0);
+    if (cfg->thermo_stats)
+        for (auto &v : s->tstat) v.assign(n, 0.0);
+    s->statsPrevTime = cfg->time0;
     return s;
 }
 
@@ -896,7 +987,13 @@ const double *orc_solver_field(const orc_solver *s, int id) {
         case ORC_F_RHOVK2: return s->rhoVk2.data();
         case ORC_F_RHOWK2: return s->rhoWk2.data();
         case ORC_F_RHOEK2: return s->rhoEk2.data();
-        default: return nullptr;
+        case ORC_F_RHO2: return s->rho2.data();
+        default:
+            if (id >= ORC_F_UAVG && id <= ORC_F_VWREY)
+                return s->c.velocity_stats ? s->vstat[id - ORC_F_UAVG].data() : nullptr;
+            if (id >= ORC_F_RHOAVG && id <= ORC_F_TRMS)
+                return s->c.thermo_stats ? s->tstat[id - ORC_F_RHOAVG].data() : nullptr;
+            return nullptr;
     }
 }
 
@@ -933,6 +1030,7 @@ double orc_rk_amplification(int rk, int dual_step, double z) {
 }
 
 double orc_solver_time(const orc_solver *s) { return s->time; }
+double orc_solver_stats_weight(const orc_solver *s, int which) { return which ? s->thermoWeight : s->velWeight; }
 double orc_solver_dt(const orc_solver *s) { return s->dt; }
 int orc_solver_timestep(const orc_solver *s) { return s->timeStep; }
 void orc_solver_destroy(orc_solver *s) { delete s; }

@@ -41,6 +41,9 @@ typedef struct {
     int timestep0;
     double time0;
     int face[6];            /* face kinds X0 X1 Y0 Y1 Z0 Z1, ORC_FACE_* (0: periodic) */
+    /* running statistics (src/Stats.hpp, src/Stats.cpp): STATS.VELOCITYSTATS,
+     * THERMOSTATS, STATSAVGTYPE (ORC_AVG_*), INTERVAL (0: none) */
+    int velocity_stats, thermo_stats, stats_avg, stats_interval;
 } orc_config;
 
 /* face kinds, in the order of src/Options.hpp BCKind */
@@ -55,6 +58,10 @@ enum {
     ORC_F_MUSGS, ORC_F_TAUKK, ORC_F_KSGS, ORC_F_C, ORC_F_CI, ORC_F_PRT,
     ORC_F_RHOK, ORC_F_RHOUK, ORC_F_RHOVK, ORC_F_RHOWK, ORC_F_RHOEK,
     ORC_F_RHOK2, ORC_F_RHOUK2, ORC_F_RHOVK2, ORC_F_RHOWK2, ORC_F_RHOEK2,
+    ORC_F_UAVG, ORC_F_VAVG, ORC_F_WAVG, ORC_F_URMS, ORC_F_VRMS, ORC_F_WRMS,
+    ORC_F_UVREY, ORC_F_UWREY, ORC_F_VWREY,
+    ORC_F_RHOAVG, ORC_F_RHORMS, ORC_F_PAVG, ORC_F_PRMS, ORC_F_TAVG, ORC_F_TRMS,
+    ORC_F_RHO2,
     ORC_F_COUNT
 };
 
@@ -69,6 +76,8 @@ orc_solver *orc_solver_create(const orc_config *cfg, const double *const metrics
 int orc_solver_steps(orc_solver *s, int n);
 const double *orc_solver_field(const orc_solver *s, int id);
 double orc_solver_time(const orc_solver *s);
+/* accumulated statistics weights (sum of the time increments): which 0 velocity, 1 thermo */
+double orc_solver_stats_weight(const orc_solver *s, int which);
 double orc_solver_dt(const orc_solver *s);
 int orc_solver_timestep(const orc_solver *s);
 void orc_solver_destroy(orc_solver *s);

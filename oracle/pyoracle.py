@@ -191,7 +191,9 @@ CONST_DT, CONST_CFL = 0, 1
 FIELD_IDS = {name: i for i, name in enumerate(
     ["rho", "rhoU", "rhoV", "rhoW", "rhoE", "U", "V", "W", "p", "T", "mu", "sos",
      "mu_sgs", "taukk", "k_sgs", "C", "CI", "Prt",
-     "rhok", "rhoUk", "rhoVk", "rhoWk", "rhoEk", "rhok2", "rhoUk2", "rhoVk2", "rhoWk2", "rhoEk2"])}
+     "rhok", "rhoUk", "rhoVk", "rhoWk", "rhoEk", "rhok2", "rhoUk2", "rhoVk2", "rhoWk2", "rhoEk2",
+     "UAVG", "VAVG", "WAVG", "URMS", "VRMS", "WRMS", "UVREY", "UWREY", "VWREY",
+     "RHOAVG", "RHORMS", "PAVG", "PRMS", "TAVG", "TRMS", "rho2"])}
 
 
 class OrcConfig(ctypes.Structure):
@@ -204,7 +206,9 @@ class OrcConfig(ctypes.Structure):
                 ("dt", ctypes.c_double), ("filter_step", ctypes.c_int),
                 ("substep_filtering", ctypes.c_int), ("dx", ctypes.c_double),
                 ("dy", ctypes.c_double), ("dz", ctypes.c_double), ("timestep0", ctypes.c_int),
-                ("time0", ctypes.c_double), ("face", ctypes.c_int * 6)]
+                ("time0", ctypes.c_double), ("face", ctypes.c_int * 6),
+                ("velocity_stats", ctypes.c_int), ("thermo_stats", ctypes.c_int),
+                ("stats_avg", ctypes.c_int), ("stats_interval", ctypes.c_int)]
 
 
 # face kinds (src/Options.hpp BCKind order)
@@ -241,6 +245,8 @@ def _solver_lib():
         for nm in ("orc_solver_time", "orc_solver_dt"):
             getattr(L, nm).restype = ctypes.c_double
             getattr(L, nm).argtypes = [ctypes.c_void_p]
+        L.orc_solver_stats_weight.restype = ctypes.c_double
+        L.orc_solver_stats_weight.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_solver_timestep.restype = ctypes.c_int
         L.orc_solver_timestep.argtypes = [ctypes.c_void_p]
         L.orc_solver_destroy.restype = None
@@ -288,6 +294,8 @@ class Solver:
 
     def field(self, name):
         ptr = self._L.orc_solver_field(self._h, FIELD_IDS[name])
+        if not ptr:
+            raise KeyError(f"{name} is not kept by this configuration")
         n = self.shape[0] * self.shape[1] * self.shape[2]
         return np.ctypeslib.as_array(ptr, shape=(n,)).reshape(self.shape).copy()
 
@@ -297,6 +305,10 @@ class Solver:
         arr = (_D * 5)(*[_p(o) for o in outs])
         self._L.orc_solver_rhs(self._h, arr)
         return [o.reshape(self.shape) for o in outs]
+
+    def stats_weight(self, which):
+        """accumulated statistics weight: 0 velocity, 1 thermo"""
+        return self._L.orc_solver_stats_weight(self._h, which)
 
     @property
     def time(self):
