@@ -530,9 +530,116 @@ __device__ __forceinline__ void features(const double A[3][3], double vol_J, flo
     f[6] = (float)(log(vol_J) / 3.0);
 }
 
-__global__ __launch_bounds__(256) void k_learned_sgs(ClosureArgs a, long n) {
-    InferW W;
-    load_infer(a.params, W);
+// The solver's closure keeps the network in LDS and loads a wave's weight
+// operands into registers only around the products of each 64-point group:
+// the FP64 feature phase and the network phase then do not hold their
+// registers at the same time (4 waves per SIMD instead of 1).  Per-lane
+// operands are stored [operand][lane] as 16-byte entries (w1 t = 0..3, w2 t,b,
+// w3 t,b, w4 b), biases by neuron.
+constexpr int kWOps = 4 + 8 + 8 + 2;
+struct InferLds {
+    s16x8 w[kWOps][64];
+    float b[3][kMlpHid];
+    float b4[4];
+};
+
+// the per-lane operands of load_infer, written entry by entry by the whole
+// workgroup (no register copy of the network)
+__device__ void stage_infer(const float *__restrict__ P, InferLds &s) {
+    for (int idx = threadIdx.x; idx < kWOps * 64 * 8; idx += blockDim.x) {
+        const int e = idx & 7, l = (idx >> 3) & 63, op = idx >> 9;
+        const int i = l & 15, g = l >> 4;
+        float v;
+        if (op < 4) {
+            const int k = 8 * g + e;
+            v = k < kMlpIn ? P[kMlpOffW1 + (16 * op + i) * kMlpIn + k] : 0.0f;
+        } else if (op < 20) {
+            const int q = op < 12 ? op - 4 : op - 12, t = q >> 1, b = q & 1;
+            v = P[(op < 12 ? kMlpOffW2 : kMlpOffW3) + (16 * t + i) * kMlpHid + kappa(b, g, e)];
+        } else {
+            v = i < kMlpOut ? P[kMlpOffW4 + i * kMlpHid + kappa(op - 20, g, e)] : 0.0f;
+        }
+        reinterpret_cast<short *>(&s.w[op][l])[e] = (short)f2bf(v);
+    }
+    for (int k = threadIdx.x; k < 3 * kMlpHid; k += blockDim.x) {
+        const int layer = k / kMlpHid, j = k % kMlpHid;
+        s.b[layer][j] = P[(layer == 0 ? kMlpOffB1 : layer == 1 ? kMlpOffB2 : kMlpOffB3) + j];
+    }
+    if (threadIdx.x < 4) s.b4[threadIdx.x] = threadIdx.x < kMlpOut ? P[kMlpOffB4 + threadIdx.x] : 0.0f;
+    __syncthreads();
+}
+
+// FP32 -> BF16 by the hardware conversion (round to nearest even, as f2bf for
+// finite values)
+__device__ __forceinline__ short bf16_bits(float v) { return __builtin_bit_cast(short, (__bf16)v); }
+
+__device__ __forceinline__ void activate_lds(const f32x4 acc[4], const float *bias, int g, s16x8 h[2]) {
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int t = 2 * b + (e >> 2), r = e & 3;
+            h[b][e] = bf16_bits(gelu(acc[t][r] + bias[16 * t + 4 * g + r]));
+        }
+}
+
+// The network on the 64 points of a wave (lane L: standardised inputs x[7]),
+// layer by layer over the wave's four 16-point blocks: only one layer's weight
+// operands are in registers at a time, loaded from LDS right before its
+// products.  Returns this lane's two outputs.
+__device__ __forceinline__ void infer64_lds(const InferLds &s, const float x[kMlpIn], float &y0, float &y1) {
+    const int l = lane_id(), g = l >> 4;
+    const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+    // compiler-only barrier: the operand loads below stay here instead of being
+    // hoisted out of the caller's loop (which would keep them live through the
+    // feature phase)
+    __asm__ volatile("" ::: "memory");
+    s16x8 h[4][2];
+    {  // layer 1: block b's inputs are the features of points 16 b .. 16 b + 15
+        s16x8 w[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) w[t] = s.w[t][l];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            s16x8 op;
+            const int src = 16 * b + (l & 15);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const float v = e < kMlpIn ? __shfl(x[e < kMlpIn ? e : 0], src) : 0.0f;
+                op[e] = (g == 0 && e < kMlpIn) ? bf16_bits(v) : (short)0;
+            }
+            f32x4 acc[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t] = mfma(w[t], op, zero);
+            activate_lds(acc, s.b[0], g, h[b]);
+        }
+    }
+#pragma unroll
+    for (int layer = 0; layer < 2; ++layer) {  // layers 2 and 3
+        s16x8 w[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) w[k] = s.w[4 + 8 * layer + k][l];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            f32x4 acc[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[t] = mfma(w[2 * t + 1], h[b][1], mfma(w[2 * t], h[b][0], zero));
+            activate_lds(acc, s.b[1 + layer], g, h[b]);
+        }
+    }
+    const s16x8 w40 = s.w[20][l], w41 = s.w[21][l];
+    const float b4 = g == 0 ? s.b4[0] : 0.0f, b4b = g == 0 ? s.b4[1] : 0.0f;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const f32x4 y = mfma(w41, h[b][1], mfma(w40, h[b][0], zero));
+        const float o0 = __shfl(y[0] + b4, l & 15), o1 = __shfl(y[1] + b4b, l & 15);
+        if (g == b) { y0 = o0; y1 = o1; }
+    }
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_learned_sgs(ClosureArgs a, long n) {
+    __shared__ InferLds net;
+    stage_infer(a.params, net);
     __shared__ int wcount[4];
     const int l = lane_id(), w = threadIdx.x >> 6;
     const long nwaves = (long)gridDim.x * 4;
@@ -567,7 +674,7 @@ __global__ __launch_bounds__(256) void k_learned_sgs(ClosureArgs a, long n) {
             for (int k = 0; k < kMlpIn; ++k) x[k] = 0.0f;
         }
         float y0 = 0.0f, y1 = 0.0f;
-        infer64(W, x, y0, y1);
+        infer64_lds(net, x, y0, y1);
         if (live) {
             const float C = y0 * a.sc.out_std[0] + a.sc.out_mean[0];
             const float D = y1 * a.sc.out_std[1] + a.sc.out_mean[1];
