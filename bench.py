@@ -25,6 +25,9 @@ ensemble = a second workload of 64 HIT boxes of uneven sizes advanced together
 shapes): its point-step rate, the same with direct launches and with default
 shapes, the members run one after another as standalone solvers, and a
 parity sample of members against standalone runs.
+learned_closure = the learned-vs-DSM half of the metric as the committed
+closure studies measured it (speedup and KE / dissipation errors at 256^3 and
+64^3), not re-run here.
 """
 import argparse
 import json
@@ -62,6 +65,27 @@ def measured_traffic(kernel, n):
     if t.get("kernel") != kernel or n != 256:
         return None
     return t.get("traffic_bytes_per_launch")
+
+
+def learned_closure_study():
+    """The learned-closure half of the metric, from the committed studies
+    (python -m mpi_3dcompact_amd.closure_study --N <n> --avg XI1_AVG: training and
+    1000-step runs take minutes, so they are not repeated here)."""
+    out = {}
+    for n in (256, 64):
+        path = os.path.join(ROOT, "profiles", f"closure_study_{n}_xi1.json")
+        try:
+            with open(path) as f:
+                d = json.load(f)
+        except (OSError, ValueError):
+            continue
+        out[f"{n}^3"] = {"speedup_vs_dsm": round(d["speedup_vs_dsm"], 3),
+                          "ms_per_step": {k: round(v, 3) for k, v in d["ms_per_step"].items()},
+                          "ke_max_rel": d["learned_vs_dsm"]["ke_max_rel"],
+                          "dissipation_max_rel": d["learned_vs_dsm"]["dissipation_max_rel"],
+                          "constant_coefficients": d["constant_vs_dsm"],
+                          "source": f"profiles/closure_study_{n}_xi1.json"}
+    return out or None
 
 
 def cpu_baseline(n_cpu):
@@ -299,6 +323,7 @@ def main():
     }
     if cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    out["learned_closure"] = learned_closure_study()
     if not args.no_ensemble:
         out["ensemble"] = ensemble_leg(args.ensemble_steps, 4, cache=args.ensemble_cache)
     print(json.dumps(out), flush=True)

@@ -2,10 +2,11 @@
 
     python -m mpi_3dcompact_amd.closure_study [--N 256] [--online-steps 1000] ...
 
-1. A DSM LES (PENTA10, COMPACT8, KENRK4, DSMTESTFILTER 1, LOCAL averaging,
+1. A DSM LES (PENTA10, COMPACT8, KENRK4, DSMTESTFILTER 1, LESAVERAGING --avg,
    DSMUSETAUKK, DSMCOEFFRANGE) of decaying HIT from a training seed; at evenly
-   spaced snapshots the DSM's coefficients (C, C / Pr_t) and the closure
-   features of strided points are sampled on the GPU.
+   spaced snapshots from step 0 to --gen-steps the DSM's coefficients
+   (C, C / Pr_t) and the closure features of strided points are sampled on the
+   GPU, the features averaged as the DSM averages its contractions.
 2. Standardisation from the samples (targets winsorised at the 0.1 / 99.9
    percentiles), the trusted input range = the sampled range widened by 10%
    of its span, then Adam training with the library's BF16-MFMA kernels.
@@ -51,22 +52,26 @@ def generate(args, torch):
     N = args.N
     npts = N ** 3
     g = make_solver(N, "DSM", args.mu, args.train_seed, args.k0, args.Mt, args.avg)
-    per = int(args.samples // args.snapshots)
+    # snapshots at steps 0, every, ..., gen_steps: the pairs span the decay the
+    # closure is used over (the online runs start from another realisation)
+    nsnap = args.snapshots + 1
+    per = int(args.samples // nsnap)
     stride = max(1, npts // per)
     per = min(per, (npts - 1) // stride + 1)
-    X = torch.empty((per * args.snapshots, 7), dtype=torch.float32, device="cuda")
-    T = torch.empty((per * args.snapshots, 2), dtype=torch.float32, device="cuda")
+    X = torch.empty((per * nsnap, 7), dtype=torch.float32, device="cuda")
+    T = torch.empty((per * nsnap, 2), dtype=torch.float32, device="cuda")
     rng = np.random.default_rng(args.train_seed)
     every = max(1, args.gen_steps // args.snapshots)
     t0 = time.time()
-    for s in range(args.snapshots):
-        g.step(every)
+    for s in range(nsnap):
+        if s:
+            g.step(every)
         off = int(rng.integers(0, npts - (per - 1) * stride))
         st = g.hit_stats()
         if not all(np.isfinite(v) for v in st.values()):
-            raise RuntimeError(f"the DSM run is not finite at step {(s + 1) * every}")
+            raise RuntimeError(f"the DSM run is not finite at step {s * every}")
         g.sample_closure(X[s * per:], T[s * per:], per, off, stride)
-        _say(f"[closure] DSM step {(s + 1) * every}: sampled {per} points (stride {stride}), "
+        _say(f"[closure] DSM step {s * every}: sampled {per} points (stride {stride}), "
              f"KE {0.5 * st['rho_uu']:.5e}, {time.time() - t0:.1f} s")
     del g
     return X, T
@@ -146,8 +151,9 @@ def main(argv=None):
                     "LOCAL-averaged DSM diverges after a few hundred steps at 256^3 (XI1_AVG runs 1000)")
     ap.add_argument("--train-seed", type=int, default=1)
     ap.add_argument("--eval-seed", type=int, default=0)
-    ap.add_argument("--gen-steps", type=int, default=400)
-    ap.add_argument("--snapshots", type=int, default=4)
+    ap.add_argument("--gen-steps", type=int, default=1000, help="DSM steps of the training run (sampled at 0, "
+                    "gen_steps / snapshots, ..., gen_steps)")
+    ap.add_argument("--snapshots", type=int, default=8)
     ap.add_argument("--samples", type=float, default=2e7)
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--batch", type=int, default=8192)

@@ -51,6 +51,18 @@ hipError_t launch_mlp_adam(float *params, float *m, float *v, const float *grad,
 // inference path the solver uses)
 hipError_t launch_mlp_infer(const float *params, const float *X, float *Y, long n, hipStream_t st);
 
+// The seven raw features of every point (FP32 values stored as FP64 fields),
+// for averaging before the network: the closure then sees its inputs averaged
+// as the DSM averages L_ij M_ij and M_ij M_ij (LESAVERAGING).
+struct FeatureArgs {
+    const double *dU[3], *dV[3], *dW[3];
+    Metrics m;
+    double vol;
+    double *feat[kMlpIn];
+    double *smag, *delta2;  // |S| and Delta^2 of each point (null: not written)
+};
+hipError_t launch_closure_features(const FeatureArgs &a, long n, hipStream_t st);
+
 // Per-point closure on the solver's fields: Cartesian gradient from the
 // computational derivatives and metrics (src/CurvilinearCSolver_Core.cpp:548-561),
 // features, network, then
@@ -70,9 +82,40 @@ struct ClosureArgs {
     double *mu_sgs, *k_sgs;
     unsigned char *flag;
     int *blockcount;
+    const double *feat[kMlpIn];  // averaged raw features; null: formed at each point
+    const double *smag, *delta2;  // with feat: |S| and Delta^2 of each point (FeatureArgs)
 };
 int closure_blocks(long n);  // workgroups launch_learned_sgs uses (size of blockcount)
 hipError_t launch_learned_sgs(const ClosureArgs &a, long n, hipStream_t st);
+// With LESAVERAGING XI1 / XI2 / XI3 the averaged inputs, and so C and C/Pr_t,
+// are constant along each line: the network runs once per line on the line
+// means of the features (feat[k][q], nlines lines), writing C[q], D[q] = C/Pr_t
+// and the out-of-range flag of each line; blockcount[b] = flagged lines of
+// workgroup b times the line length.  The same numerics as launch_learned_sgs.
+struct LineClosureArgs {
+    const double *feat[kMlpIn];
+    const float *params;
+    MlpScaling sc;
+    long nlines;
+    int line_len;
+    float *C, *D;
+    unsigned char *lflag;
+    int *blockcount;
+};
+int line_closure_blocks(long nlines);  // workgroups launch_line_closure uses
+hipError_t launch_line_closure(const LineClosureArgs &a, hipStream_t st);
+// every point: mu_sgs = (rho Delta^2) |S| C[line], k_sgs = (cp rho Delta^2 |S|) D[line],
+// flag = the line's flag (lines of direction dir numbered as xi_line_of numbers them)
+struct ClosureApplyArgs {
+    const double *rho, *smag, *delta2;
+    const float *C, *D;
+    const unsigned char *lflag;
+    double cp;
+    int dir, Nx, Ny;
+    double *mu_sgs, *k_sgs;
+    unsigned char *flag;
+};
+hipError_t launch_closure_apply(const ClosureApplyArgs &a, long n, hipStream_t st);
 // total = sum of blockcount[0..nb) into count[0]
 hipError_t launch_count_sum(const int *blockcount, int nb, int *count, hipStream_t st);
 // where flag[p]: mu_sgs, k_sgs = the DSM's values at that point
@@ -93,6 +136,7 @@ struct SampleArgs {
     double vol;
     long offset, stride, count;
     float *X, *T;
+    const double *feat[kMlpIn];  // averaged raw features; null: formed at each point
 };
 hipError_t launch_closure_samples(const SampleArgs &a, hipStream_t st);
 

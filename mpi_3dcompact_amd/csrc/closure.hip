@@ -637,6 +637,25 @@ __device__ __forceinline__ void infer64_lds(const InferLds &s, const float x[kMl
     }
 }
 
+__global__ __launch_bounds__(256) void k_closure_features(FeatureArgs a, long n) {
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const GradIn gi{{a.dU[0], a.dU[1], a.dU[2]}, {a.dV[0], a.dV[1], a.dV[2]}, {a.dW[0], a.dW[1], a.dW[2]}, a.m};
+    double A[3][3];
+    cart_grad(gi, p, A);
+    float f[kMlpIn];
+    double Smag;
+    const double vol_J = a.vol / a.m.J[p];
+    features(A, vol_J, f, Smag);
+#pragma unroll
+    for (int k = 0; k < kMlpIn; ++k) __builtin_nontemporal_store((double)f[k], a.feat[k] + p);
+    if (a.smag) {
+        const double d = cbrt(vol_J);
+        __builtin_nontemporal_store(Smag, a.smag + p);
+        __builtin_nontemporal_store(d * d, a.delta2 + p);
+    }
+}
+
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_learned_sgs(ClosureArgs a, long n) {
     __shared__ InferLds net;
     stage_infer(a.params, net);
@@ -653,12 +672,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         double base = 0.0;
         bool out = false;
         if (live) {
-            double A[3][3];
-            cart_grad(gi, p, A);
-            const double vol_J = a.vol / a.m.J[p];
             float f[kMlpIn];
-            double Smag;
-            features(A, vol_J, f, Smag);
+            double Smag, d2;
+            if (a.feat[0]) {  // averaged inputs; |S| and Delta^2 of the point from the feature pass
+#pragma unroll
+                for (int k = 0; k < kMlpIn; ++k) f[k] = (float)a.feat[k][p];
+                Smag = a.smag[p];
+                d2 = a.delta2[p];
+            } else {
+                double A[3][3];
+                cart_grad(gi, p, A);
+                const double vol_J = a.vol / a.m.J[p];
+                features(A, vol_J, f, Smag);
+                const double d = cbrt(vol_J);
+                d2 = d * d;
+            }
 #pragma unroll
             for (int k = 0; k < kMlpIn; ++k) {
                 const bool in = f[k] >= a.sc.lo[k] && f[k] <= a.sc.hi[k];
@@ -667,8 +695,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                 const float fc = in ? f[k] : (f[k] < a.sc.lo[k] ? a.sc.lo[k] : a.sc.hi[k]);
                 x[k] = (fc - a.sc.in_mean[k]) / a.sc.in_std[k];
             }
-            const double d = cbrt(vol_J);
-            base = (a.rho[p] * (d * d)) * Smag;
+            base = (a.rho[p] * d2) * Smag;
         } else {
 #pragma unroll
             for (int k = 0; k < kMlpIn; ++k) x[k] = 0.0f;
@@ -687,6 +714,53 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     if (l == 0) wcount[w] = count;
     __syncthreads();
     if (threadIdx.x == 0) a.blockcount[blockIdx.x] = ((wcount[0] + wcount[1]) + wcount[2]) + wcount[3];
+}
+
+// the network on lines (LineClosureArgs); the clamp, standardisation and
+// de-standardisation of k_learned_sgs
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_line_closure(LineClosureArgs a) {
+    __shared__ InferLds net;
+    stage_infer(a.params, net);
+    __shared__ int wcount[4];
+    const int l = lane_id(), w = threadIdx.x >> 6;
+    const long nwaves = (long)gridDim.x * 4;
+    const long ngroups = (a.nlines + 63) / 64;
+    int count = 0;
+    for (long grp = (long)blockIdx.x * 4 + w; grp < ngroups; grp += nwaves) {
+        const long q = grp * 64 + l;
+        const bool live = q < a.nlines;
+        float x[kMlpIn];
+        bool out = false;
+#pragma unroll
+        for (int k = 0; k < kMlpIn; ++k) {
+            const float f = live ? (float)a.feat[k][q] : 0.0f;
+            const bool in = f >= a.sc.lo[k] && f <= a.sc.hi[k];
+            out = out || !in;
+            const float fc = in ? f : (f < a.sc.lo[k] ? a.sc.lo[k] : a.sc.hi[k]);
+            x[k] = live ? (fc - a.sc.in_mean[k]) / a.sc.in_std[k] : 0.0f;
+        }
+        float y0 = 0.0f, y1 = 0.0f;
+        infer64_lds(net, x, y0, y1);
+        if (live) {
+            a.C[q] = y0 * a.sc.out_std[0] + a.sc.out_mean[0];
+            a.D[q] = y1 * a.sc.out_std[1] + a.sc.out_mean[1];
+            a.lflag[q] = out ? 1 : 0;
+        }
+        count += __popcll(__ballot(live && out));
+    }
+    if (l == 0) wcount[w] = count;
+    __syncthreads();
+    if (threadIdx.x == 0) a.blockcount[blockIdx.x] = (((wcount[0] + wcount[1]) + wcount[2]) + wcount[3]) * a.line_len;
+}
+
+__global__ __launch_bounds__(256) void k_closure_apply(ClosureApplyArgs a, long n) {
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const long q = xi_line_of(p, a.dir, a.Nx, a.Ny);
+    const double base = (ldnt(a.rho, p) * ldnt(a.delta2, p)) * ldnt(a.smag, p);
+    stnt(a.mu_sgs, p, base * (double)a.C[q]);
+    stnt(a.k_sgs, p, (a.cp * base) * (double)a.D[q]);
+    a.flag[p] = a.lflag[q];
 }
 
 __global__ void k_count_sum(const int *__restrict__ bc, int nb, int *__restrict__ count) {
@@ -714,11 +788,16 @@ __global__ void k_closure_samples(SampleArgs a) {
     if (i >= a.count) return;
     const long p = a.offset + i * a.stride;
     const GradIn gi{{a.dU[0], a.dU[1], a.dU[2]}, {a.dV[0], a.dV[1], a.dV[2]}, {a.dW[0], a.dW[1], a.dW[2]}, a.m};
-    double A[3][3];
-    cart_grad(gi, p, A);
     float f[kMlpIn];
-    double Smag;
-    features(A, a.vol / a.m.J[p], f, Smag);
+    if (a.feat[0]) {
+#pragma unroll
+        for (int k = 0; k < kMlpIn; ++k) f[k] = (float)a.feat[k][p];
+    } else {
+        double A[3][3];
+        cart_grad(gi, p, A);
+        double Smag;
+        features(A, a.vol / a.m.J[p], f, Smag);
+    }
 #pragma unroll
     for (int k = 0; k < kMlpIn; ++k) a.X[i * kMlpIn + k] = f[k];
     const double C = a.C[p];
@@ -727,6 +806,11 @@ __global__ void k_closure_samples(SampleArgs a) {
 }
 
 }  // namespace
+
+hipError_t launch_closure_features(const FeatureArgs &a, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_closure_features, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_mlp_grad_partials(const float *params, const float *X, const float *T, const int *idx, long n,
                                     long n_total, float *partial, float *lossp, hipStream_t st) {
@@ -761,6 +845,19 @@ hipError_t launch_mlp_infer(const float *params, const float *X, float *Y, long 
 int closure_blocks(long n) {
     const long groups = (n + 63) / 64;
     return (int)std::max<long>(1, std::min<long>((groups + 3) / 4, 2048));
+}
+
+int line_closure_blocks(long nlines) { return closure_blocks(nlines); }
+
+hipError_t launch_line_closure(const LineClosureArgs &a, hipStream_t st) {
+    if (a.nlines < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_line_closure, dim3((unsigned)line_closure_blocks(a.nlines)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_closure_apply(const ClosureApplyArgs &a, long n, hipStream_t st) {
+    hipLaunchKernelGGL(k_closure_apply, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_learned_sgs(const ClosureArgs &a, long n, hipStream_t st) {

@@ -181,11 +181,42 @@ void Closure::load(const std::string &path) {
 }
 
 // ---------------------------------------------------------------------------
+// The network's inputs are averaged as the DSM it replaces averages L_ij M_ij
+// and M_ij M_ij (LESAVERAGING, src/DSMSGS.hpp:245-300 with src/AbstractSGS.cpp:13-164):
+// line means, the global mean, or the test filter for LOCAL.  A network is to
+// be used with the averaging its training pairs were sampled under.
 LearnedSGS::LearnedSGS(CurvilinearCSolver *c) {
     cs = c;
     fallback = c->opt.closureFallback;
+    musgsAvgType = c->opt.musgsAvgType;
     mu_sgs = c->alloc_field();
     k_sgs = c->alloc_field();
+    if (musgsAvgType != Options::NONE) {
+        for (auto &f : feat_) f = c->alloc_field();
+        smag_ = c->alloc_field();
+        delta2_ = c->alloc_field();
+    }
+    if (musgsAvgType == Options::XI1_AVG || musgsAvgType == Options::XI2_AVG || musgsAvgType == Options::XI3_AVG) {
+        line_dir_ = musgsAvgType == Options::XI1_AVG ? 0 : musgsAvgType == Options::XI2_AVG ? 1 : 2;
+        const Domain &d = c->dom;
+        nlines_ = (long)d.gNx * d.gNy * d.gNz / (line_dir_ == 0 ? d.gNx : line_dir_ == 1 ? d.gNy : d.gNz);
+        for (auto &f : lines_) f = dev_alloc<double>((size_t)nlines_, "closure line means");
+        line_C_ = dev_alloc<float>((size_t)nlines_, "closure line coefficients");
+        line_D_ = dev_alloc<float>((size_t)nlines_, "closure line coefficients");
+        line_flag_ = dev_alloc<unsigned char>((size_t)nlines_, "closure line flags");
+    }
+    if (musgsAvgType == Options::LOCAL) {
+        const int bc3[3] = {c->opt.bcXType == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET,
+                            c->opt.bcYType == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET,
+                            c->opt.bcZType == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET};
+        std::unique_ptr<AbstractFilter> *fv[3] = {&filtX, &filtY, &filtZ};
+        for (int d = 0; d < 3; ++d) {
+            const auto dir = (AbstractDerivatives::Direct)d;
+            if (c->opt.testFilterType == 1) fv[d]->reset(new CommN5PadeTestFilter(c->dom, bc3, dir, c->st));
+            else fv[d]->reset(new CommN5ExpTestFilter(c->dom, bc3, dir, c->st));
+            static_cast<LineFilter *>(fv[d]->get())->timer = &c->timer;
+        }
+    }
     params_ = dev_alloc<float>(kMlpParams, "closure parameters");
     flag_ = dev_alloc<unsigned char>((size_t)c->N, "closure flags");
     blockcount_ = dev_alloc<int>((size_t)closure_blocks(c->N), "closure counts");
@@ -200,7 +231,10 @@ LearnedSGS::LearnedSGS(CurvilinearCSolver *c) {
 
 LearnedSGS::~LearnedSGS() {
     (void)hipStreamSynchronize(cs->st);
-    for (void *p : {(void *)params_, (void *)flag_, (void *)blockcount_, (void *)count_})
+    for (void *p : {(void *)params_, (void *)flag_, (void *)blockcount_, (void *)count_, (void *)line_C_,
+                    (void *)line_D_, (void *)line_flag_})
+        if (p) (void)hipFree(p);
+    for (double *p : lines_)
         if (p) (void)hipFree(p);
 }
 
@@ -229,11 +263,73 @@ void LearnedSGS::getSGSViscosity(const double *const dU[3], const double *const 
     a.k_sgs = k_sgs;
     a.flag = flag_;
     a.blockcount = blockcount_;
-    // read: 9 derivatives, 10 metrics, rho; written: mu_sgs, k_sgs (+ one byte of flags)
-    cs->timed("k_learned_sgs", 22.0, [&] { hip_check(launch_learned_sgs(a, cs->N, cs->st), "learned closure"); });
+    int count_blocks = 0;  // workgroup counts the flagged-point total is summed over
+    if (feat_[0]) {
+        FeatureArgs fa{};
+        for (int d = 0; d < 3; ++d) { fa.dU[d] = dU[d]; fa.dV[d] = dV[d]; fa.dW[d] = dW[d]; }
+        fa.m = cs->msh;
+        fa.vol = a.vol;
+        for (int k = 0; k < kMlpIn; ++k) fa.feat[k] = feat_[k];
+        fa.smag = smag_;
+        fa.delta2 = delta2_;
+        // read: 9 derivatives, 10 metrics; written: 7 features, |S|, Delta^2
+        cs->timed("k_closure_features", 28.0,
+                  [&] { hip_check(launch_closure_features(fa, cs->N, cs->st), "closure features"); });
+        if (line_dir_ >= 0) {
+            // line means of the features, the network once per line, then every point
+            const Domain &d = cs->dom;
+            const int len = line_dir_ == 0 ? d.gNx : line_dir_ == 1 ? d.gNy : d.gNz;
+            cs->timed("k_xi_average_fields", 7.0, [&] {
+                hip_check(launch_xi_average_fields(feat_, nullptr, kMlpIn, line_dir_, d.gNx, d.gNy, d.gNz, cs->st,
+                                                   lines_),
+                          "closure feature average");
+            });
+            LineClosureArgs la{};
+            for (int k = 0; k < kMlpIn; ++k) la.feat[k] = lines_[k];
+            la.params = params_;
+            la.sc = sc_;
+            la.nlines = nlines_;
+            la.line_len = len;
+            la.C = line_C_;
+            la.D = line_D_;
+            la.lflag = line_flag_;
+            la.blockcount = blockcount_;
+            cs->timed("k_line_closure", 0.0, [&] { hip_check(launch_line_closure(la, cs->st), "line closure"); });
+            ClosureApplyArgs ap{};
+            ap.rho = rho;
+            ap.smag = smag_;
+            ap.delta2 = delta2_;
+            ap.C = line_C_;
+            ap.D = line_D_;
+            ap.lflag = line_flag_;
+            ap.cp = cs->ig.cp;
+            ap.dir = line_dir_;
+            ap.Nx = d.gNx;
+            ap.Ny = d.gNy;
+            ap.mu_sgs = mu_sgs;
+            ap.k_sgs = k_sgs;
+            ap.flag = flag_;
+            // read: rho, |S|, Delta^2; written: mu_sgs, k_sgs
+            cs->timed("k_closure_apply", 5.0,
+                      [&] { hip_check(launch_closure_apply(ap, cs->N, cs->st), "closure apply"); });
+            count_blocks = line_closure_blocks(nlines_);
+        } else {
+            for (int k = 0; k < kMlpIn; ++k) doAveraging(feat_[k], feat_[k]);
+            for (int k = 0; k < kMlpIn; ++k) a.feat[k] = feat_[k];
+            a.smag = smag_;
+            a.delta2 = delta2_;
+        }
+    }
+    if (count_blocks == 0) {
+        // read: 9 derivatives, 10 metrics, rho (or 7 averaged features, |S|, Delta^2, rho);
+        // written: mu_sgs, k_sgs (+ one byte of flags)
+        cs->timed("k_learned_sgs", feat_[0] ? 12.0 : 22.0,
+                  [&] { hip_check(launch_learned_sgs(a, cs->N, cs->st), "learned closure"); });
+        count_blocks = closure_blocks(cs->N);
+    }
     ++evaluations;
     if (!fallback) return;
-    hip_check(launch_count_sum(blockcount_, closure_blocks(cs->N), count_, cs->st), "closure count");
+    hip_check(launch_count_sum(blockcount_, count_blocks, count_, cs->st), "closure count");
     int n_out = 0;
     hip_check(hipMemcpyAsync(&n_out, count_, sizeof(int), hipMemcpyDeviceToHost, cs->st), "closure count");
     hip_check(hipStreamSynchronize(cs->st), "closure count");

@@ -77,6 +77,15 @@ class LearnedSGS : public AbstractSGS {
     unsigned char *flag_ = nullptr;
     int *blockcount_ = nullptr, *count_ = nullptr;
     std::unique_ptr<DSMSGS> dsm_;
+    // averaged raw features (LESAVERAGING other than NONE), with |S| and Delta^2 per point
+    double *feat_[kMlpIn] = {};
+    double *smag_ = nullptr, *delta2_ = nullptr;
+    // XI1 / XI2 / XI3: the network per line (line means, coefficients, flags)
+    int line_dir_ = -1;
+    long nlines_ = 0;
+    double *lines_[kMlpIn] = {};
+    float *line_C_ = nullptr, *line_D_ = nullptr;
+    unsigned char *line_flag_ = nullptr;
 };
 
 }  // namespace mc3d

@@ -821,6 +821,18 @@ void CurvilinearCSolver::sampleClosure(float *X, float *T, long count, long offs
     sa.C = les->C;
     sa.Prt = les->Prt;
     sa.vol = dom.dx * dom.dy * dom.dz;
+    if (opt.musgsAvgType != Options::NONE) {
+        // the features averaged as the DSM averaged its contractions, in DSM work fields
+        DSMSGS *d = static_cast<DSMSGS *>(les.get());
+        FeatureArgs fa{};
+        for (int k = 0; k < 3; ++k) { fa.dU[k] = dU[k]; fa.dV[k] = dV[k]; fa.dW[k] = dW[k]; }
+        fa.m = msh;
+        fa.vol = sa.vol;
+        for (int k = 0; k < kMlpIn; ++k) fa.feat[k] = d->work[1 + k];
+        hip_check(launch_closure_features(fa, N, st), "closure features");
+        for (int k = 0; k < kMlpIn; ++k) les->doAveraging(fa.feat[k], fa.feat[k]);
+        for (int k = 0; k < kMlpIn; ++k) sa.feat[k] = fa.feat[k];
+    }
     sa.offset = offset; sa.stride = stride; sa.count = count;
     sa.X = X; sa.T = T;
     hip_check(launch_closure_samples(sa, st), "closure samples");

@@ -116,6 +116,19 @@ hipError_t launch_fill(double *p, double v, long n, hipStream_t st);
 // doAveraging XI1/XI2/XI3: average along direction dir (in place allowed)
 hipError_t launch_xi_average(const double *phi, double *phiF, int dir, int Nx, int Ny, int Nz,
                              hipStream_t st);
+// the same for nf <= 8 fields in one launch (phi[f] -> phiF[f], in place allowed);
+// with line set, the means go to line[f][q] (one value per line q, numbered as
+// xi_line_of numbers them) and phiF is not written
+hipError_t launch_xi_average_fields(const double *const *phi, double *const *phiF, int nf, int dir, int Nx,
+                                    int Ny, int Nz, hipStream_t st, double *const *line = nullptr);
+
+// line number of point p for direction dir (lines of direction dir, as the
+// line averages number them)
+__device__ __forceinline__ long xi_line_of(long p, int dir, int Nx, int Ny) {
+    if (dir == 0) return p / Nx;
+    if (dir == 1) return (p / ((long)Nx * Ny)) * Nx + p % Nx;
+    return p % ((long)Nx * Ny);
+}
 // per-block {min, max} pairs of phi into partial[2 * nblocks] (getRange)
 hipError_t launch_minmax_partial(const double *phi, double *partial, int nblocks, long n, hipStream_t st);
 // doAveraging GLOBAL: phiF = mean(phi); partial needs cfl_blocks(n) + 1 doubles

@@ -354,7 +354,8 @@ __global__ void k_fill(double *p, double v, long n) {
 // src/AbstractSGS.cpp:15-131: each term divided by the line length, summed in
 // index order, then broadcast back along the line
 template <int DIR>
-__global__ void k_xi_average(const double *phi, double *phiF, int Nx, int Ny, int Nz) {
+__device__ __forceinline__ void xi_average_line(const double *phi, double *phiF, int Nx, int Ny, int Nz,
+                                                double *line = nullptr) {
     const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int nl = DIR == 0 ? Nx : DIR == 1 ? Ny : Nz;
     const long nlines = (long)Nx * Ny * Nz / nl;
@@ -365,7 +366,61 @@ __global__ void k_xi_average(const double *phi, double *phiF, int Nx, int Ny, in
     if (DIR == 2) { base = q; stride = (long)Nx * Ny; }
     double acc = 0.0;
     for (int i = 0; i < nl; ++i) acc += phi[base + i * stride] / (double)nl;
+    if (line) {
+        line[q] = acc;
+        return;
+    }
     for (int i = 0; i < nl; ++i) phiF[base + i * stride] = acc;
+}
+
+template <int DIR>
+__global__ void k_xi_average(const double *phi, double *phiF, int Nx, int Ny, int Nz) {
+    xi_average_line<DIR>(phi, phiF, Nx, Ny, Nz);
+}
+
+struct XiFields {
+    const double *phi[8];
+    double *phiF[8];
+    double *line[8];  // per-line means instead of phiF (null: write phiF)
+};
+
+// x-lines (contiguous): a workgroup stages L whole lines through LDS with
+// coalesced loads, one thread per line sums f / N in line order (the same
+// rounding as xi_average_line), and the mean is written back coalesced
+constexpr int kXiThreads = 256;
+inline int xi1_lines(int Nx) { return Nx >= 8192 ? 1 : (8192 / Nx < 32 ? 8192 / Nx : 32); }
+
+__global__ __launch_bounds__(kXiThreads) void k_xi1_average_tiled(XiFields f, int Nx, long nlines, int L) {
+    extern __shared__ double tile[];  // [L][Nx + 1]
+    const double *phi = f.phi[blockIdx.y];
+    double *phiF = f.phiF[blockIdx.y];
+    const long q0 = (long)blockIdx.x * L;
+    const int nl = (int)(nlines - q0 < L ? nlines - q0 : L);
+    const int S = Nx + 1;
+    for (int e = threadIdx.x; e < nl * Nx; e += blockDim.x) {
+        const int l = e / Nx, i = e - l * Nx;
+        tile[l * S + i] = phi[q0 * Nx + e];
+    }
+    __syncthreads();
+    double *line = f.line[blockIdx.y];
+    if ((int)threadIdx.x < nl) {
+        double acc = 0.0;
+        const double *row = tile + threadIdx.x * S;
+        for (int i = 0; i < Nx; ++i) acc += row[i] / (double)Nx;
+        tile[threadIdx.x * S + Nx] = acc;  // the padding column holds the mean
+        if (line) line[q0 + threadIdx.x] = acc;
+    }
+    if (line) return;  // uniform: the whole workgroup leaves
+    __syncthreads();
+    for (int e = threadIdx.x; e < nl * Nx; e += blockDim.x) {
+        const int l = e / Nx;
+        phiF[q0 * Nx + e] = tile[l * S + Nx];
+    }
+}
+
+template <int DIR>
+__global__ void k_xi_average_fields(XiFields f, int Nx, int Ny, int Nz) {
+    xi_average_line<DIR>(f.phi[blockIdx.y], f.phiF[blockIdx.y], Nx, Ny, Nz, f.line[blockIdx.y]);
 }
 
 __global__ __launch_bounds__(kBlock) void k_sum_partial(const double *phi, double *partial, long n) {
@@ -494,9 +549,36 @@ hipError_t launch_xi_average(const double *phi, double *phiF, int dir, int Nx, i
     if (dir < 0 || dir > 2) return hipErrorInvalidValue;
     const int nl = dir == 0 ? Nx : dir == 1 ? Ny : Nz;
     const long nlines = (long)Nx * Ny * Nz / nl;
-    if (dir == 0) hipLaunchKernelGGL(k_xi_average<0>, grid_for(nlines), dim3(kBlock), 0, st, phi, phiF, Nx, Ny, Nz);
+    if (dir == 0) {
+        const double *p1[1] = {phi};
+        double *o1[1] = {phiF};
+        return launch_xi_average_fields(p1, o1, 1, 0, Nx, Ny, Nz, st);
+    }
     if (dir == 1) hipLaunchKernelGGL(k_xi_average<1>, grid_for(nlines), dim3(kBlock), 0, st, phi, phiF, Nx, Ny, Nz);
     if (dir == 2) hipLaunchKernelGGL(k_xi_average<2>, grid_for(nlines), dim3(kBlock), 0, st, phi, phiF, Nx, Ny, Nz);
+    return hipGetLastError();
+}
+
+hipError_t launch_xi_average_fields(const double *const *phi, double *const *phiF, int nf, int dir, int Nx,
+                                    int Ny, int Nz, hipStream_t st, double *const *line) {
+    if (nf < 1 || nf > 8 || dir < 0 || dir > 2) return hipErrorInvalidValue;
+    XiFields f{};
+    for (int k = 0; k < nf; ++k) {
+        f.phi[k] = phi[k];
+        f.phiF[k] = phiF ? phiF[k] : nullptr;
+        f.line[k] = line ? line[k] : nullptr;
+    }
+    const long nlines = (long)Nx * Ny * Nz / (dir == 0 ? Nx : dir == 1 ? Ny : Nz);
+    const dim3 grid((unsigned)((nlines + kBlock - 1) / kBlock), nf);
+    if (dir == 0) {
+        const int L = xi1_lines(Nx);
+        const size_t lds = sizeof(double) * (size_t)L * (Nx + 1);
+        hipLaunchKernelGGL(k_xi1_average_tiled, dim3((unsigned)((nlines + L - 1) / L), nf), dim3(kXiThreads), lds, st,
+                           f, Nx, nlines, L);
+        return hipGetLastError();
+    }
+    if (dir == 1) hipLaunchKernelGGL(k_xi_average_fields<1>, grid, dim3(kBlock), 0, st, f, Nx, Ny, Nz);
+    if (dir == 2) hipLaunchKernelGGL(k_xi_average_fields<2>, grid, dim3(kBlock), 0, st, f, Nx, Ny, Nz);
     return hipGetLastError();
 }
 

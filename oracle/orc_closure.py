@@ -175,6 +175,27 @@ def features(A, vol_J):
     return f
 
 
+def line_average(f, axis):
+    """Mean of f along one grid direction, summed as the solver sums it: f / N
+    added in line order (AbstractSGS::doAveraging, src/AbstractSGS.cpp:13-164)."""
+    N = f.shape[axis]
+    acc = np.zeros_like(np.take(f, 0, axis=axis))
+    for i in range(N):
+        acc = acc + np.take(f, i, axis=axis) / N
+    return np.broadcast_to(np.expand_dims(acc, axis), f.shape)
+
+
+def averaged_features(A, vol_J, avg):
+    """Closure inputs under LESAVERAGING avg: the features rounded to FP32 (as
+    the solver stores them), then line means along x, y or z (XI1_AVG, XI2_AVG,
+    XI3_AVG) or the global mean (GLOBAL) of (Nz, Ny, Nx) grids."""
+    f = features(A, vol_J).astype(np.float32).astype(np.float64)
+    if avg == "GLOBAL":
+        return np.broadcast_to(f.reshape(-1, f.shape[-1]).mean(axis=0), f.shape)
+    axis = {"XI1_AVG": 2, "XI2_AVG": 1, "XI3_AVG": 0}[avg]
+    return np.stack([line_average(f[..., k], axis) for k in range(f.shape[-1])], axis=-1)
+
+
 def smag(A):
     """|S| = sqrt(2 S_ij S_ij)."""
     S = 0.5 * (A + np.swapaxes(A, -1, -2))

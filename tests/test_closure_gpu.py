@@ -171,6 +171,38 @@ def test_solver_closure_matches_oracle(torch):
         assert np.quantile(err, 0.99) <= 1e-5 and err.max() <= 1e-2
 
 
+@pytest.mark.parametrize("avg", ["XI1_AVG", "XI3_AVG"])
+def test_solver_closure_with_averaged_inputs_matches_oracle(torch, avg):
+    """LESAVERAGING applies to the network's inputs as the DSM applies it to its
+    contractions: the seven features are averaged along the lines before the
+    network, |S| stays local."""
+    from mpi_3dcompact_amd.solver import Solver
+    N = 32
+    c, p, (im, isd, om, osd) = _scaled_closure(22)
+    g = Solver(hit.options_text(N, fd="PADE6", filt="COMPACT10", rk="RK4", les="LEARNED", les_avg=avg,
+                                closure_fallback=False), (N, N, N))
+    g.set_metrics(hit.identity_metrics(N))
+    g.set_initial(hit.hit_initial_condition(N, k0=4.0, Mt=0.3, seed=4))
+    g.set_closure(c)
+    g.eval_sgs()
+    A = _cart(g, (N, N, N))
+    vol_J = (1.0 / N) ** 3 / g.field("J")
+    f = C.averaged_features(A, vol_J, avg)
+    x = C.standardise(f.reshape(-1, 7), im, isd)
+    y = C.forward(p, x)
+    cp = 1.4 / 0.4 * (1.0 / 1.4)
+    mu, k = C.closure_outputs(y, om, osd, g.field("rho").ravel(), C.smag(A).ravel(), vol_J.ravel(), cp)
+    for name, ref in (("mu_sgs", mu), ("k_sgs", k)):
+        got = g.field(name).ravel()
+        err = np.abs(got - ref) / np.max(np.abs(ref))
+        _say(f"   {avg} {name}: max {err.max():.2e} p99 {np.quantile(err, 0.99):.2e}")
+        assert np.quantile(err, 0.99) <= 1e-5 and err.max() <= 1e-2
+    # a coefficient constant along the lines: mu_sgs / (rho Delta^2 |S|) is
+    axis = 2 if avg == "XI1_AVG" else 0
+    coef = (g.field("mu_sgs") / (g.field("rho") * np.cbrt(vol_J) ** 2 * C.smag(A)))
+    assert np.allclose(coef, np.take(coef, [0], axis=axis), rtol=1e-12, atol=0)
+
+
 def test_solver_closure_steps_and_falls_back(torch):
     """A narrow trusted range sends the stages to the DSM fallback; with the
     range wide open no stage falls back; with a fallback fraction of 1 the
@@ -194,12 +226,14 @@ def test_solver_closure_steps_and_falls_back(torch):
             assert np.all(np.isfinite(g.field(name)))
 
 
-def test_dsm_sampling_matches_its_fields(torch):
+@pytest.mark.parametrize("avg", ["GLOBAL", "XI1_AVG"])
+def test_dsm_sampling_matches_its_fields(torch, avg):
     """Training pairs of a DSM state: features equal the oracle's on the DSM's
-    own gradients, targets equal its C and C / Pr_t, and the state is unchanged."""
+    own gradients, averaged as the DSM averages (LESAVERAGING), targets equal
+    its C and C / Pr_t, and the state is unchanged."""
     from mpi_3dcompact_amd.solver import Solver
     N = 32
-    g = Solver(hit.options_text(N, fd="PENTA10", filt="COMPACT8", rk="KENRK4", les="DSM", les_avg="GLOBAL",
+    g = Solver(hit.options_text(N, fd="PENTA10", filt="COMPACT8", rk="KENRK4", les="DSM", les_avg=avg,
                                 coeff_range=True), (N, N, N))
     g.set_metrics(hit.identity_metrics(N))
     g.set_initial(hit.hit_initial_condition(N, k0=4.0, Mt=0.3, seed=5))
@@ -211,10 +245,11 @@ def test_dsm_sampling_matches_its_fields(torch):
     g.sample_closure(X, T, count, off, stride)
     for k, v in before.items():
         assert np.array_equal(g.field(k), v), k
-    A = _cart(g, (N, N, N)).reshape(-1, 3, 3)
-    vol_J = ((1.0 / N) ** 3 / g.field("J")).ravel()
+    A = _cart(g, (N, N, N))
+    vol_J = (1.0 / N) ** 3 / g.field("J")
     pts = off + stride * np.arange(count)
-    f = C.features(A[pts], vol_J[pts]).astype(np.float32)
+    # the global mean is summed in another order on the GPU (block partial sums)
+    f = C.averaged_features(A, vol_J, avg).reshape(-1, 7)[pts].astype(np.float32)
     Xg = X.cpu().numpy()
     assert np.max(np.abs(Xg - f) / np.maximum(np.abs(f), 1e-3)) <= 1e-6
     Cf, Pf = g.field("C").ravel()[pts], g.field("Prt").ravel()[pts]
