@@ -81,11 +81,12 @@ def options_text(N, fd="PADE6", filt="COMPACT10", alpha_f=0.49, rk="TVDRK3", cfl
                  filter_step=1, substep_filtering=False, dsm_filter=1, use_taukk=True,
                  coeff_range=False, ts_type="CONST_CFL", dt=None, restart_file=None,
                  dump_step=1000000, disp=None, closure_file=None, closure_fallback=True,
-                 faces=None, sponge=None, closure_fallback_fraction=None, stats=None):
+                 faces=None, sponge=None, closure_fallback_fraction=None, stats=None, dims=None):
     """MPI_3DCompact.in text for a periodic HIT box of N^3 points (restart_file:
     start from that SolutionDump, which also sets the grid; disp: 3x3
     PERIODICDISP rows, default the 2 pi box; stats: dict velocity, thermo (bools),
-    avg (STATSAVGTYPE name), interval)."""
+    avg (STATSAVGTYPE name), interval; dims: (Nx, Ny, Nz) instead of N^3)."""
+    Nx, Ny, Nz = dims if dims is not None else (N, N, N)
     fds = fd if isinstance(fd, (list, tuple)) else (fd, fd, fd)
     lines = ["[RESTART]"]
     if restart_file:
@@ -110,7 +111,7 @@ def options_text(N, fd="PADE6", filt="COMPACT10", alpha_f=0.49, rk="TVDRK3", cfl
         f"INTERVAL = {int((stats or {}).get('interval', 5))}",
         "[BC]"]
     for d, a in enumerate("XYZ"):
-        per = fk[2 * d] == "PERIODIC"
+        per = fk[2 * d] in _PERIODIC
         lines.append(f"BC{a}TYPE = {'PERIODIC_SOLVE' if per else 'DIRICHLET_SOLVE'}")
         lines += [f"BC{a}0 = {fk[2 * d]}", f"BC{a}1 = {fk[2 * d + 1]}"]
     if sponge is not None:
@@ -125,7 +126,7 @@ def options_text(N, fd="PADE6", filt="COMPACT10", alpha_f=0.49, rk="TVDRK3", cfl
         for j, b in enumerate("XYZ"):
             lines.append(f"PERIODICDISP{a}_{b} = {float(disp[i][j])!r}")
     lines += [
-        "[DOMAIN]"] + ([] if restart_file else [f"NX = {N}", f"NY = {N}", f"NZ = {N}"]) + [
+        "[DOMAIN]"] + ([] if restart_file else [f"NX = {Nx}", f"NY = {Ny}", f"NZ = {Nz}"]) + [
         "[PENCILDECOMP]", "PROW = 0", "PCOL = 0",
         "[SOLVER]", f"ALPHAF = {alpha_f!r}", f"MU_REF = {mu_ref!r}", "USETIMING = FALSE",
         f"XFDTYPE = {fds[0]}", f"YFDTYPE = {fds[1]}", f"ZFDTYPE = {fds[2]}",
@@ -153,6 +154,7 @@ def ensemble_members(count=64, seed=2024, sizes=(32, 48, 64, 96, 128), mt=(0.1, 
 
 
 FACE_NAMES = ("X0", "X1", "Y0", "Y1", "Z0", "Z1")
+_PERIODIC = ("PERIODIC", "INTERNALLY_PERIODIC")  # faces of a periodic (PERIODIC_SOLVE) direction
 _FACE = {"PERIODIC": 0, "INTERNALLY_PERIODIC": 1, "SPONGE": 2, "ADIABATIC_WALL": 3, "CONST_T_WALL": 4,
          "MOVING_ADIABATIC_WALL": 5, "INLET": 6}
 
@@ -162,7 +164,7 @@ def face_kinds(faces):
     faces = faces or {}
     out = [faces.get(f, "PERIODIC") for f in FACE_NAMES]
     for d in range(3):
-        if (out[2 * d] == "PERIODIC") != (out[2 * d + 1] == "PERIODIC"):
+        if (out[2 * d] in _PERIODIC) != (out[2 * d + 1] in _PERIODIC):
             raise ValueError("both faces of a direction must be periodic or neither")
     return out
 
@@ -170,7 +172,7 @@ def face_kinds(faces):
 def box_mesh(N, faces=None, L=2 * np.pi):
     """Coordinates x = L xi of the HIT box, xi in [0, 1) periodic or [0, 1] otherwise ((N, N, N) arrays)."""
     fk = face_kinds(faces)
-    axes = [np.arange(N) / (N if fk[2 * d] == "PERIODIC" else N - 1) * L for d in range(3)]
+    axes = [np.arange(N) / (N if fk[2 * d] in _PERIODIC else N - 1) * L for d in range(3)]
     z, y, x = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
     return x, y, z
 
@@ -185,13 +187,14 @@ _AVG = {"NONE": 0, "XI1_AVG": 1, "XI2_AVG": 2, "XI3_AVG": 3, "LOCAL": 4, "GLOBAL
 def oracle_config(N, fd="PADE6", filt="COMPACT10", alpha_f=0.49, rk="TVDRK3", cfl=0.5,
                   mu_ref=1.0 / 1600.0, les="NOMODEL", les_avg="NONE", filter_step=1,
                   substep_filtering=False, dsm_filter=1, use_taukk=True, coeff_range=False,
-                  ts_type="CONST_CFL", dt=None, faces=None, stats=None, **_):
+                  ts_type="CONST_CFL", dt=None, faces=None, stats=None, dims=None, **_):
     """The same case as options_text(), as the field dict of the CPU oracle's config."""
     fds = fd if isinstance(fd, (list, tuple)) else (fd, fd, fd)
     fk = face_kinds(faces)
-    bc = tuple(0 if fk[2 * d] == "PERIODIC" else 1 for d in range(3))
-    sp = [1.0 / N if b == 0 else 1.0 / (N - 1) for b in bc]
-    return dict(Nx=N, Ny=N, Nz=N, bc=bc, face=tuple(_FACE[k] for k in fk), fd=tuple(_FD[f] for f in fds),
+    bc = tuple(0 if fk[2 * d] in _PERIODIC else 1 for d in range(3))
+    n3 = dims if dims is not None else (N, N, N)
+    sp = [1.0 / n3[d] if bc[d] == 0 else 1.0 / (n3[d] - 1) for d in range(3)]
+    return dict(Nx=n3[0], Ny=n3[1], Nz=n3[2], bc=bc, face=tuple(_FACE[k] for k in fk), fd=tuple(_FD[f] for f in fds),
                 filter=_FILT[filt], alphaF=alpha_f, rk=_RK[rk], les=_LES[les],
                 les_avg=_AVG[les_avg], dsm_test_filter=dsm_filter, dsm_use_taukk=int(use_taukk),
                 dsm_coeff_range=int(coeff_range), mu_ref=mu_ref,
