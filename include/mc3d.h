@@ -178,6 +178,28 @@ int mc3d_solver_write_dump(mc3d_solver *s, const char *path, const double *x, co
  * into dir (NULL: working directory), with the given host mesh (copied). */
 int mc3d_solver_set_output(mc3d_solver *s, const char *dir, const double *x, const double *y,
                            const double *z);
+/* PNG image output of a field plane (PngWriter, src/PngWriter.hpp:83-190;
+ * CurvilinearCSolver::addImageOutput, src/CurvilinearCSolver_Core.cpp:1438):
+ * every `interval` steps (0: the first step only) the index plane nearest to
+ * `fraction` of the grid along `plane` (0, 1, 2) is gathered on the device
+ * and drawn as width x height RGB pixels, colour range [vmin, vmax] when
+ * has_range is nonzero, else the plane's extremes; colormap 0 GREYSCALE,
+ * 1 RAINBOW, 2 BWR.  Written as <var_name>.<step, 6 digits>.png into the
+ * output directory (mc3d_solver_set_output; default the working directory).
+ * The reference interpolates the curvilinear mesh onto a Cartesian pixel
+ * grid; here pixels sample the structured index plane (the same picture for
+ * a Cartesian mesh). */
+int mc3d_solver_add_image(mc3d_solver *s, int interval, int width, int height, const char *field,
+                          const char *var_name, int plane, double fraction, int has_range, double vmin,
+                          double vmax, int colormap);
+/* The index plane `index` normal to `dir` of a field, into host memory of
+ * n1 * n2 doubles, a1 fastest: dir 0 -> (a1, a2) = (j, k), 1 -> (k, i),
+ * 2 -> (i, j).  Synchronous. */
+int mc3d_solver_slice(mc3d_solver *s, const char *field, int dir, int index, double *host_out);
+/* Render an (n2, n1) plane (a1 fastest) as the image outputs do and write it
+ * as a PNG file; host only (no device needed). */
+int mc3d_image_write(const char *path, const double *plane, int n1, int n2, int width, int height,
+                     int has_range, double vmin, double vmax, int colormap);
 /* Decaying-HIT statistics of the current state, the turbdata.out hook of
  * tests/hit/Samtaney_D4/MPI_3DCompact.cpp:380-556 (device reductions):
  * stats = {time, tau, Re_lambda, M_t, epsilon = <2 mu S*_ij du_i/dx_j> / <rho>,

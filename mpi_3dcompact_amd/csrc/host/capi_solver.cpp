@@ -229,6 +229,50 @@ int mc3d_solver_get_field(mc3d_solver *s, const char *name, double *host_out) {
     });
 }
 
+int mc3d_solver_add_image(mc3d_solver *s, int interval, int width, int height, const char *field,
+                          const char *var_name, int plane, double fraction, int has_range, double vmin,
+                          double vmax, int colormap) {
+    if (!s || !field || !var_name) return mc3d::fail(MC3D_EINVAL, "null argument");
+    return guarded([&] {
+        mc3d::ImageOutput img;
+        img.interval = interval;
+        img.width = width;
+        img.height = height;
+        img.field = field;
+        img.var_name = var_name;
+        img.plane = plane;
+        img.fraction = fraction;
+        img.has_range = has_range != 0;
+        img.vmin = vmin;
+        img.vmax = vmax;
+        img.colormap = colormap;
+        s->cs->addImageOutput(img);
+    });
+}
+
+int mc3d_solver_slice(mc3d_solver *s, const char *field, int dir, int index, double *host_out) {
+    if (!s || !field || !host_out) return mc3d::fail(MC3D_EINVAL, "null argument");
+    return guarded([&] { s->cs->slice(field, dir, index, host_out); });
+}
+
+int mc3d_image_write(const char *path, const double *plane, int n1, int n2, int width, int height,
+                     int has_range, double vmin, double vmax, int colormap) {
+    if (!path || !plane) return mc3d::fail(MC3D_EINVAL, "null argument");
+    if (n1 < 1 || n2 < 1 || width < 1 || height < 1) return mc3d::fail(MC3D_EINVAL, "sizes must be positive");
+    if (colormap < mc3d::GREYSCALE || colormap > mc3d::BWR) return mc3d::fail(MC3D_EINVAL, "unknown colormap");
+    mc3d::ImageOutput img;
+    img.width = width;
+    img.height = height;
+    img.has_range = has_range != 0;
+    img.vmin = vmin;
+    img.vmax = vmax;
+    img.colormap = colormap;
+    const std::vector<unsigned char> rgb = mc3d::render_plane(plane, n1, n2, img);
+    if (!mc3d::write_png(path, rgb.data(), width, height))
+        return mc3d::fail(MC3D_EINVAL, std::string("cannot write ") + path);
+    return MC3D_OK;
+}
+
 int mc3d_solver_scalars(mc3d_solver *s, double *time, double *dt, int *timestep) {
     if (!s) return mc3d::fail(MC3D_EINVAL, "null solver");
     return guarded([&] {

@@ -15,6 +15,7 @@
 #include "mesh.hpp"
 #include "dump.hpp"
 #include "../lineop_api.h"
+#include "../slice.h"
 
 namespace mc3d {
 
@@ -514,6 +515,7 @@ CurvilinearCSolver::~CurvilinearCSolver() {
     (void)hipStreamSynchronize(st);
     for (void *p : extra_) (void)hipFree(p);
     if (slab_) (void)hipFree(slab_);
+    if (slice_buf_) (void)hipFree(slice_buf_);
 }
 
 double *CurvilinearCSolver::alloc_field() {
@@ -773,7 +775,64 @@ void CurvilinearCSolver::set_output(const std::string &dir, const double *const 
 }
 
 // calcDtFromCFL, src/CurvilinearCSolver_Core.cpp:189-237
+void CurvilinearCSolver::addImageOutput(const ImageOutput &img) {
+    if (img.width < 1 || img.height < 1) throw Error(MC3D_EINVAL, "image size must be positive");
+    if (img.plane < 0 || img.plane > 2) throw Error(MC3D_EINVAL, "image plane must be 0, 1 or 2");
+    if (img.interval < 0) throw Error(MC3D_EINVAL, "negative image interval");
+    if (img.colormap < GREYSCALE || img.colormap > BWR) throw Error(MC3D_EINVAL, "unknown colormap");
+    if (!(img.fraction >= 0.0 && img.fraction <= 1.0)) throw Error(MC3D_EINVAL, "plane fraction must be in [0, 1]");
+    if (img.var_name.empty()) throw Error(MC3D_EINVAL, "empty image name");
+    if (!field_by_name(img.field)) throw Error(MC3D_EINVAL, "unknown field " + img.field);
+    if (dom.members > 1) throw Error(MC3D_ECONFIG, "image outputs take one member");
+    images_.push_back(img);
+}
+
+void CurvilinearCSolver::slice(const std::string &name, int dir, int index, double *host_out) {
+    const double *f = field_by_name(name);
+    if (!f) throw Error(MC3D_EINVAL, "unknown field " + name);
+    if (dir < 0 || dir > 2) throw Error(MC3D_EINVAL, "slice direction must be 0, 1 or 2");
+    const int n[3] = {dom.gNx, dom.gNy, dom.gNz};
+    if (index < 0 || index >= n[dir]) throw Error(MC3D_EINVAL, "slice index out of range");
+    const size_t np = (size_t)N / n[dir];
+    if (np > slice_cap_) {
+        if (slice_buf_) hip_check(hipFree(slice_buf_), "slice buffer");
+        slice_buf_ = nullptr;
+        slice_cap_ = 0;
+        hip_check(hipMalloc(&slice_buf_, sizeof(double) * np), "slice buffer");
+        slice_cap_ = np;
+    }
+    hip_check(launch_slice(f, slice_buf_, dir, index, n[0], n[1], n[2], st), "slice");
+    hip_check(hipMemcpyAsync(host_out, slice_buf_, sizeof(double) * np, hipMemcpyDeviceToHost, st),
+              "slice download");
+    hip_check(hipStreamSynchronize(st), "slice download");
+}
+
+// src/CurvilinearCSolver_Core.cpp:1442-1492: at the first step every image
+// whose interval is 0 or divides the step, later the ones whose interval does
+void CurvilinearCSolver::writeImages() {
+    if (images_.empty() || images_step_ == timeStep) return;
+    const bool first = timeStep == opt.timeStep;
+    char step[32];
+    std::snprintf(step, sizeof(step), "%06d", timeStep);
+    std::vector<double> plane;
+    for (const ImageOutput &img : images_) {
+        const bool due = img.interval > 0 ? timeStep % img.interval == 0 : first;
+        if (!due) continue;
+        const int n[3] = {dom.gNx, dom.gNy, dom.gNz};
+        const int a1 = img.plane == 0 ? n[1] : img.plane == 1 ? n[2] : n[0];
+        const int a2 = img.plane == 0 ? n[2] : img.plane == 1 ? n[0] : n[1];
+        plane.resize((size_t)a1 * a2);
+        slice(img.field, img.plane, plane_index(img.fraction, n[img.plane]), plane.data());
+        const std::vector<unsigned char> rgb = render_plane(plane.data(), a1, a2, img);
+        const std::string path = (out_dir_.empty() ? std::string() : out_dir_ + "/") + img.var_name + "." + step + ".png";
+        if (!write_png(path, rgb.data(), img.width, img.height))
+            std::printf("Warning: could not open png file: %s\n", path.c_str());
+    }
+    images_step_ = timeStep;
+}
+
 void CurvilinearCSolver::preStep() {
+    if (timeStep == opt.timeStep) writeImages();
     timed("k_cfl", 7.0, [&] {
         hip_check(launch_cfl(pr, msh, dom.dx, dom.dy, dom.dz, cfl_partial, dom.gN, dom.members, st), "cfl");
     });
@@ -1034,7 +1093,6 @@ void CurvilinearCSolver::updateNonConservedData() {
     timed("k_eos", 12.0, [&] { hip_check(launch_eos(rkLast ? r1 : rk_, pr, ig, N, st), "eos"); });
 }
 
-// checkEnd without the kill file: src/CurvilinearCSolver_Core.cpp:1798-1819
 // src/Utils.cpp:415-431: the sequential fold std::min(phi, acc) from 1e6 (and
 // std::max from -1e6); for finite data the block-wise min/max gives the same
 void CurvilinearCSolver::getRange(const double *phi, const char *name) {
@@ -1115,6 +1173,16 @@ void CurvilinearCSolver::postStep() {
                              mesh_[2].empty() ? nullptr : mesh_[2].data()};
     // src/CurvilinearCSolver.hpp:367-369
     if (dumps_ && timeStep % opt.dumpStep == 0) dumpSolution(std::string(), mesh);
+    writeImages();
+    // src/CurvilinearCSolver_Core.cpp:1761-1796: a file named killsolver in the
+    // working directory ends the run (and is deleted)
+    if (FILE *kf = std::fopen("killsolver", "r")) {
+        std::fclose(kf);
+        if (std::remove("killsolver") != 0) std::printf(" > ERROR DELETING KILLFILE! \n");
+        std::printf("===================\n FOUND KILL SOLVER \n===================\n");
+        std::fflush(stdout);
+        endFlag = true;
+    }
     if (timeStep >= opt.maxTimeStep) endFlag = true;
     if (!endFlag) {
         StepScalars h;

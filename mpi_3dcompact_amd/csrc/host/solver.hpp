@@ -17,6 +17,7 @@
 #include "../lineop_api.h"
 #include "../pointwise.h"
 #include "mc3d.h"
+#include "images.hpp"
 #include "options.hpp"
 #include "runstats.hpp"
 #include "timer.hpp"
@@ -338,6 +339,14 @@ class CurvilinearCSolver {
     // dumps every DUMPSTEP steps and at the end, as the reference's postStep
     // does, into dir (mesh arrays copied; null entries are written as zeros)
     void set_output(const std::string &dir, const double *const mesh[3]);
+    // PngWriter image outputs (CurvilinearCSolver::addImageOutput,
+    // src/CurvilinearCSolver_Core.cpp:1438-1440), written at the first step
+    // and every interval steps after the dump (:1442-1492)
+    void addImageOutput(const ImageOutput &img);
+    void writeImages();
+    // index plane `index` normal to dir of a field into host memory, as
+    // (n2, n1) with a1 fastest (the image planes' axes)
+    void slice(const std::string &name, int dir, int index, double *host_out);
     void setInitialConditions_host(const double *rho0, const double *U0, const double *V0,
                                    const double *W0, const double *p0);
 
@@ -410,6 +419,10 @@ class CurvilinearCSolver {
     std::chrono::steady_clock::time_point last_check_ = std::chrono::steady_clock::now();
     bool dumps_ = false;
     std::string out_dir_;
+    std::vector<ImageOutput> images_;
+    double *slice_buf_ = nullptr;
+    size_t slice_cap_ = 0;
+    int images_step_ = -1;  // the step whose images were last written
     std::vector<double> mesh_[3];
     bool eos_done_ = false;  // the last stage kernel also updated the non-conserved data
     // boundaries

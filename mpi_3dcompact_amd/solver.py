@@ -31,6 +31,13 @@ _SIGS = [
     ("mc3d_solver_eval_sgs", ctypes.c_int, [_VP]),
     ("mc3d_solver_hit_stats", ctypes.c_int, [_VP, _D]),
     ("mc3d_solver_stats_weights", ctypes.c_int, [_VP, _D, _D]),
+    ("mc3d_solver_add_image", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
+                                             ctypes.c_char_p, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                             ctypes.c_double, ctypes.c_double, ctypes.c_int]),
+    ("mc3d_solver_slice", ctypes.c_int, [_VP, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _D]),
+    ("mc3d_image_write", ctypes.c_int, [ctypes.c_char_p, _D, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_int]),
     ("mc3d_solver_closure_stats", ctypes.c_int, [_VP] + [ctypes.POINTER(ctypes.c_long)] * 4),
     ("mc3d_solver_timing_report", ctypes.c_int, [_VP, ctypes.c_char_p, ctypes.c_int,
                                                  ctypes.POINTER(ctypes.c_int)]),
@@ -50,6 +57,19 @@ def _bind():
 
 def _p(a):
     return a.ctypes.data_as(_D)
+
+
+COLORMAPS = {"GREYSCALE": 0, "RAINBOW": 1, "BWR": 2}
+
+
+def image_write(path, plane, width, height, vrange=None, colormap="RAINBOW"):
+    """Draw an (n2, n1) plane as the solver's image outputs do and write the PNG
+    (host only)."""
+    a = np.ascontiguousarray(plane, dtype=np.float64)
+    cm = COLORMAPS[colormap] if isinstance(colormap, str) else int(colormap)
+    lo, hi = vrange if vrange is not None else (0.0, 1.0)
+    check(_bind().mc3d_image_write(str(path).encode(), _p(a), a.shape[1], a.shape[0], width, height,
+                                   vrange is not None, lo, hi, cm))
 
 
 class Solver:
@@ -107,6 +127,25 @@ class Solver:
         out = (ctypes.c_double * 11)()
         check(self._L.mc3d_solver_hit_stats(self._h, out))
         return dict(zip(self.HIT_STATS, out[:]))
+
+    def add_image(self, field, var_name, plane=2, fraction=0.5, width=512, height=512, interval=0,
+                  vrange=None, colormap="RAINBOW"):
+        """PngWriter image output: <var_name>.<step>.png of the index plane nearest
+        `fraction` along `plane`, every `interval` steps (0: the first step only),
+        colour range vrange = (min, max) or the plane's extremes."""
+        cm = COLORMAPS[colormap] if isinstance(colormap, str) else int(colormap)
+        lo, hi = vrange if vrange is not None else (0.0, 1.0)
+        check(self._L.mc3d_solver_add_image(self._h, interval, width, height, field.encode(), var_name.encode(),
+                                            plane, fraction, vrange is not None, lo, hi, cm))
+
+    def slice(self, field, direction, index):
+        """Index plane of a field normal to `direction`, shaped (n2, n1) with the
+        image axes: direction 0 -> (k, j), 1 -> (i, k), 2 -> (j, i)."""
+        nz, ny, nx = self.shape
+        n1, n2 = {0: (ny, nz), 1: (nz, nx), 2: (nx, ny)}[direction]
+        out = np.empty((n2, n1))
+        check(self._L.mc3d_solver_slice(self._h, field.encode(), direction, index, _p(out)))
+        return out
 
     def stats_weights(self):
         """Accumulated weights (time covered) of the velocity and thermo statistics."""

@@ -169,6 +169,26 @@ def test_dump_and_restart(gpu, tmp_path):
     assert np.array_equal(dump.read(str(out / "SolutionDump.7"))["rho"], b.field("rho"))
 
 
+def test_kill_file_ends_the_run(gpu, tmp_path, monkeypatch):
+    """A file named killsolver in the working directory ends run() after the
+    step that finds it, is deleted, and the final state is dumped
+    (src/CurvilinearCSolver_Core.cpp:1761-1823)."""
+    from mpi_3dcompact_amd.solver import Solver
+    N = 16
+    monkeypatch.chdir(tmp_path)
+    g = Solver(hit.options_text(N, max_steps=100000, fd="PADE6", filt="COMPACT10", rk="TVDRK3", mu_ref=1e-3),
+               (N, N, N))
+    g.set_metrics(hit.identity_metrics(N))
+    g.set_initial(hit.hit_initial_condition(N, k0=3.0, Mt=0.2, seed=1))
+    g.step(2)
+    (tmp_path / "killsolver").write_text("")
+    g.set_output(str(tmp_path))
+    g.run()
+    assert g.scalars()[2] == 3
+    assert not (tmp_path / "killsolver").exists()
+    assert (tmp_path / "SolutionDump.3").exists()
+
+
 def test_full_les_256_properties(gpu):
     """Full plain LES at 256^3 (PENTA10, COMPACT8, KENRK4, VREMAN, mu = 1/5000):
     conservation of mass, momentum and total energy and kinetic-energy decay
