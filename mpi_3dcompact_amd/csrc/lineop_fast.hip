@@ -218,7 +218,13 @@ __global__ __launch_bounds__(1024) void lineop_strided(OpDev op, LaunchArgs a) {
 template <bool PER>
 constexpr int contig_lines() { return PER ? 32 : 64; }
 
-constexpr int kBatch = 8;  // loads in flight per thread while staging (MM is a multiple)
+// loads in flight per thread while staging.  The tile caps a CU at one or two
+// workgroups, so the fill is latency bound: 16 at a time where the registers
+// allow it (8 at a time left Penta10 x at 3.4 TB/s); the 16-row Dirichlet
+// kernels (64-line tiles of 1024 threads, 128 VGPRs) and the 16-row H = 5
+// periodic ones spill with 16
+template <int H, int MM, bool PER>
+constexpr int contig_batch() { return MM == 32 || (PER && H < 5) ? 16 : 8; }
 
 // (line, point) of element tid + k * nthr of a LINES x N tile, advanced
 // incrementally (nthr = qd * N + rd)
@@ -250,21 +256,22 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
     const int q0 = blockIdx.x * L;
     const double *__restrict__ in = a.in[blockIdx.y];
     double *__restrict__ out = a.out[blockIdx.y];
-    // fill: every thread issues batches of kBatch coalesced loads before the LDS writes
+    // fill: every thread issues its coalesced loads in batches before the LDS writes
     {
         TileWalk t{tid / N, tid % N};
+        constexpr int kB = contig_batch<H, MM, PER>();
 #pragma unroll 1
-        for (int k0 = 0; k0 < MM; k0 += kBatch) {
-            double v[kBatch];
+        for (int k0 = 0; k0 < MM; k0 += kB) {
+            double v[kB];
             TileWalk u = t;
 #pragma unroll
-            for (int k = 0; k < kBatch; ++k) {
+            for (int k = 0; k < kB; ++k) {
                 const int q = q0 + u.ll;
                 v[k] = q < a.nlines ? __builtin_nontemporal_load(in + (long)q * N + u.p) : 0.0;
                 u.next(qd, rd, N);
             }
 #pragma unroll
-            for (int k = 0; k < kBatch; ++k) {
+            for (int k = 0; k < kB; ++k) {
                 tile[(t.p + H) * LS + t.ll] = v[k];
                 if (PER) {  // halo rows: periodic images
                     if (t.p < H) tile[(t.p + H + N) * LS + t.ll] = v[k];
@@ -347,20 +354,21 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
         for (int i = 0; i < MM; ++i) tile[(cs + H + i) * LS + l] = y[i];
     }
     __syncthreads();
-    // store: batches of kBatch coalesced stores (and loads when accumulating)
+    // store: batches of coalesced stores (and loads when accumulating)
     {
         TileWalk t{tid / N, tid % N};
+        constexpr int kB = contig_batch<H, MM, PER>();
 #pragma unroll 1
-        for (int k0 = 0; k0 < MM; k0 += kBatch) {
-            double v[kBatch];
+        for (int k0 = 0; k0 < MM; k0 += kB) {
+            double v[kB];
             TileWalk u = t;
 #pragma unroll
-            for (int k = 0; k < kBatch; ++k) {
+            for (int k = 0; k < kB; ++k) {
                 v[k] = tile[(u.p + H) * LS + u.ll];
                 u.next(qd, rd, N);
             }
 #pragma unroll
-            for (int k = 0; k < kBatch; ++k) {
+            for (int k = 0; k < kB; ++k) {
                 const int qq = q0 + t.ll;
                 if (qq < a.nlines) {
                     double *g = out + (long)qq * N + t.p;

@@ -136,11 +136,11 @@ def test_inplace_filter_fast_kernels(torch_cuda, kind, af, bc3):
 def test_kernel_choice(torch_cuda):
     """Evenly chunked lines take the fast kernels; ragged lines, CD2 with its
     periodic wrap rows and TPB take the LDS-tiled kernel.  x-lines take 16-row
-    chunks except the Penta10 derivative on lines of 256 points or more."""
+    chunks."""
     from mpi_3dcompact_amd.ops import LineOp
     dims = (64, 17, 48)
     assert LineOp.deriv(P.PENTA10, 0, P.PERIODIC, dims, 1 / 64).kernel() == "lineop_contig<2, 3, 16, true>"
-    assert LineOp.deriv(P.PENTA10, 0, P.PERIODIC, (256, 17, 48), 1 / 256).kernel() == "lineop_contig<2, 3, 32, true>"
+    assert LineOp.deriv(P.PENTA10, 0, P.PERIODIC, (256, 17, 48), 1 / 256).kernel() == "lineop_contig<2, 3, 16, true>"
     assert LineOp.deriv(P.PADE6, 0, P.PERIODIC, (256, 17, 48), 1 / 256).kernel() == "lineop_contig<1, 2, 16, true>"
     assert LineOp.filter(P.COMMN5PADE, 0.0, 0, (0, 0, 0), (256, 17, 48)).kernel() == "lineop_contig<2, 5, 16, true>"
     assert LineOp.deriv(P.PENTA10, 1, P.PERIODIC, (17, 256, 48), 1 / 256).kernel() == "lineop_strided<2, 3, 32, true>"
