@@ -5,10 +5,16 @@
 // reference's postStep, src/CurvilinearCSolver.hpp:359).
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <fstream>
 #include <vector>
 
 #include "../bc.h"
 #include "solver.hpp"
+
+#include <cstdio>
+#include <fstream>
+#include <vector>
 
 namespace mc3d {
 
@@ -105,9 +111,25 @@ RkBc CurvilinearCSolver::rkBoundaryArgs() const {
 }
 
 // src/CurvilinearCSolver_BC.cpp:324-345: the averages start as the initial
-// conserved fields (after the wall fixes)
+// conserved fields (after the wall fixes), or, with SPONGEAVGFROMRESTART, as the
+// five fields of SPONGEAVGFILENAME (SpongeBC::readSpongeAvgFromRestart,
+// src/SpongeBC.cpp:426-475: rho, rhoU, rhoV, rhoW, rhoE, x fastest, no header)
 void CurvilinearCSolver::initSpongeAverages() {
     if (!sponge_ || avg_given_) return;
+    if (opt.spongeFromRestart) {
+        std::ifstream in(opt.sponge_filename, std::ios::binary);
+        if (!in) throw Error(MC3D_EINVAL, "cannot open sponge averages " + opt.sponge_filename);
+        std::vector<double> buf((size_t)N);
+        for (int q = 0; q < 5; ++q) {
+            in.read(reinterpret_cast<char *>(buf.data()), (std::streamsize)(sizeof(double) * N));
+            if (!in) throw Error(MC3D_EINVAL, "sponge averages file too short: " + opt.sponge_filename);
+            hip_check(hipMemcpyAsync(spgAvg_[q], buf.data(), sizeof(double) * N, hipMemcpyHostToDevice, st),
+                      "sponge averages");
+            hip_check(hipStreamSynchronize(st), "sponge averages");
+        }
+        std::printf(" > Initializing sponge average field from restart file: %s\n", opt.sponge_filename.c_str());
+        return;
+    }
     for (int q = 0; q < 5; ++q)
         hip_check(hipMemcpyAsync(spgAvg_[q], r1.q[q], sizeof(double) * N, hipMemcpyDeviceToDevice, st),
                   "sponge averages");

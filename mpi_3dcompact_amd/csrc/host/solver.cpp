@@ -434,9 +434,6 @@ CurvilinearCSolver::CurvilinearCSolver(const Options &o, hipStream_t s, int memb
     ig.cp = ig.R_gas * ig.gamma / (ig.gamma - 1.0);
     ig.mu_member = nullptr;
     ig.member_pts = 0;
-    if (opt.spongeFromRestart)
-        throw Error(MC3D_ECONFIG, "RESTART.SPONGEAVGFROMRESTART is not supported (the averages start "
-                                  "from the initial state)");
     // field slab
     const int nfields = 10 + 20 + 10 + 12 + 15 + 5 + 10 + 8;
     slab_bytes_ = sizeof(double) * (size_t)N * nfields + 4096;
@@ -774,7 +771,6 @@ void CurvilinearCSolver::set_output(const std::string &dir, const double *const 
     dumps_ = true;
 }
 
-// calcDtFromCFL, src/CurvilinearCSolver_Core.cpp:189-237
 void CurvilinearCSolver::addImageOutput(const ImageOutput &img) {
     if (img.width < 1 || img.height < 1) throw Error(MC3D_EINVAL, "image size must be positive");
     if (img.plane < 0 || img.plane > 2) throw Error(MC3D_EINVAL, "image plane must be 0, 1 or 2");
@@ -831,6 +827,8 @@ void CurvilinearCSolver::writeImages() {
     images_step_ = timeStep;
 }
 
+// calcDtFromCFL, src/CurvilinearCSolver_Core.cpp:189-237 (images first at the
+// first step, src/CurvilinearCSolver.hpp:310-316)
 void CurvilinearCSolver::preStep() {
     if (timeStep == opt.timeStep) writeImages();
     timed("k_cfl", 7.0, [&] {

@@ -81,11 +81,13 @@ def options_text(N, fd="PADE6", filt="COMPACT10", alpha_f=0.49, rk="TVDRK3", cfl
                  filter_step=1, substep_filtering=False, dsm_filter=1, use_taukk=True,
                  coeff_range=False, ts_type="CONST_CFL", dt=None, restart_file=None,
                  dump_step=1000000, disp=None, closure_file=None, closure_fallback=True,
-                 faces=None, sponge=None, closure_fallback_fraction=None, stats=None, dims=None):
+                 faces=None, sponge=None, closure_fallback_fraction=None, stats=None, dims=None,
+                 sponge_avg_file=None):
     """MPI_3DCompact.in text for a periodic HIT box of N^3 points (restart_file:
     start from that SolutionDump, which also sets the grid; disp: 3x3
     PERIODICDISP rows, default the 2 pi box; stats: dict velocity, thermo (bools),
-    avg (STATSAVGTYPE name), interval; dims: (Nx, Ny, Nz) instead of N^3)."""
+    avg (STATSAVGTYPE name), interval; dims: (Nx, Ny, Nz) instead of N^3;
+    sponge_avg_file: SpongeAvgDump file the sponge averages start from)."""
     Nx, Ny, Nz = dims if dims is not None else (N, N, N)
     fds = fd if isinstance(fd, (list, tuple)) else (fd, fd, fd)
     lines = ["[RESTART]"]
@@ -95,7 +97,8 @@ def options_text(N, fd="PADE6", filt="COMPACT10", alpha_f=0.49, rk="TVDRK3", cfl
         lines += ["FROMRESTART = FALSE", "ONLYGRIDFROMRESTART = FALSE"]
     fk = face_kinds(faces)
     lines += [
-        "SPONGEAVGFROMRESTART = FALSE",
+        "SPONGEAVGFROMRESTART = " + ("TRUE" if sponge_avg_file else "FALSE"),
+    ] + ([f"SPONGEAVGFILENAME = {sponge_avg_file}"] if sponge_avg_file else []) + [
         "[TIMESTEPPING]", f"TSTYPE = {ts_type}",
         (f"CFL = {cfl!r}" if ts_type == "CONST_CFL" else f"DT = {dt!r}"),
         "MAXTIME = 1000000.0", f"MAXTIMESTEP = {max_steps}", f"FILTERSTEP = {filter_step}",

@@ -114,3 +114,46 @@ def test_sponge_averages_written_beside_the_dump(gpu, tmp_path):
     for k, name in enumerate(("rhoAvg", "rhoUAvg", "rhoVAvg", "rhoWAvg", "rhoEAvg")):
         assert np.array_equal(raw[k * N ** 3:(k + 1) * N ** 3].reshape(N, N, N), g.field(name)), name
     assert not os.path.exists(tmp_path / "VelocityStats.2")
+
+
+def test_sponge_averages_from_a_restart_file(gpu, tmp_path):
+    """SPONGEAVGFROMRESTART: the sponge averages start from the five fields of a
+    SpongeAvgDump file (src/SpongeBC.cpp:426-475) instead of the initial state,
+    and the step relaxes towards them: one step against the oracle given the
+    same averages."""
+    from mpi_3dcompact_amd.solver import Solver
+    N = 16
+    faces = {"X0": "SPONGE", "X1": "SPONGE"}
+    sponge = {"KIND": "RECTILINEAR", "RECTX0PERC": 0.25, "RECTX1PERC": 0.25}
+    x, y, z = hit.box_mesh(N, faces)
+    a = Solver(hit.options_text(N, faces=faces, sponge=sponge, **CASE), (N, N, N))
+    a.set_mesh(x, y, z)
+    a.set_initial(hit.hit_initial_condition(N, k0=2.0, Mt=0.2, seed=2))
+    a.write_dump(str(tmp_path / "SolutionDump.0"))
+    path = tmp_path / "SpongeAvgDump.0"
+    avg = {k: a.field(k + "Avg") for k in ("rho", "rhoU", "rhoV", "rhoW", "rhoE")}
+    ic = hit.hit_initial_condition(N, k0=3.0, Mt=0.3, seed=9)
+    b = Solver(hit.options_text(N, faces=faces, sponge=sponge, sponge_avg_file=str(path), **CASE), (N, N, N))
+    b.set_mesh(x, y, z)
+    b.set_initial(ic)
+    for k, v in avg.items():
+        assert np.array_equal(b.field(k + "Avg"), v), k
+    assert not np.array_equal(b.field("rhoU"), avg["rhoU"])  # a different state (rho is 1 in both)
+    # one step relaxes towards the file's averages, as the oracle does with them
+    fk = [P.FACE[k] for k in hit.face_kinds(faces)]
+    sig = P.sponge_sigma(x, y, z, fk, kind=0, strength=1.0, perc=[0.25, 0.25, 0, 0, 0, 0])
+    o = P.Solver(hit.oracle_config(N, faces=faces, **CASE), [b.field(n) for n in
+                 ("J", "J11", "J12", "J13", "J21", "J22", "J23", "J31", "J32", "J33")], ic)
+    o.set_sponge(sig, avg)
+    b.step(1)
+    o.steps(1)
+    for n in ("rho", "rhoU", "rhoE"):
+        e = np.linalg.norm(b.field(n) - o.field(n)) / np.linalg.norm(o.field(n))
+        assert e <= 1e-12, (n, e)
+    # a missing file is an error at set-up
+    from mpi_3dcompact_amd._lib import Mc3dError
+    c = Solver(hit.options_text(N, faces=faces, sponge=sponge, sponge_avg_file=str(tmp_path / "none"), **CASE),
+               (N, N, N))
+    c.set_mesh(x, y, z)
+    with pytest.raises(Mc3dError):
+        c.set_initial(ic)
