@@ -9,7 +9,7 @@
 // row offset in a scalar register), and all window loads are issued before
 // any other work of the lane.
 //
-// Contiguous direction (x): the workgroup stages 32 (periodic) or 64 lines
+// Contiguous direction (x): the workgroup stages 16 (periodic) or 64 lines
 // into LDS in a transposed [point][line] layout (row stride lines + 1) with
 // halo rows, so the coalesced fill and the per-lane chunk reads are both free
 // of bank conflicts; each thread issues batches of loads before its LDS
@@ -212,11 +212,14 @@ __global__ __launch_bounds__(1024) void lineop_strided(OpDev op, LaunchArgs a) {
     for (int i = 0; i < MM; ++i) bstore(y[i], rout, boff, (unsigned)(cs + i) * rowb);
 }
 
-// lines per workgroup of the contiguous kernel: 32 for periodic lines (every
-// table read is chunk independent, two workgroups fit one CU), 64 otherwise
-// (a wave is then 64 lines x one chunk, so the end-row branches are uniform)
+// lines per workgroup of the contiguous kernel: 16 for periodic lines (every
+// table read is chunk independent, so a wave may span chunks; the smaller tile
+// lets four workgroups of 256-point lines share a CU, which overlaps one
+// workgroup's staging with another's solve: 32 lines measured 4-14% slower,
+// 8 lines mixed), 64 otherwise (a wave is then 64 lines x one chunk, so the
+// end-row branches are uniform)
 template <bool PER>
-constexpr int contig_lines() { return PER ? 32 : 64; }
+constexpr int contig_lines() { return PER ? 16 : 64; }
 
 // loads in flight per thread while staging.  The tile caps a CU at one or two
 // workgroups, so the fill is latency bound: 16 at a time where the registers
