@@ -98,6 +98,11 @@ constexpr int kRawBufferWord3 = 0x00020000;  // gfx9-family raw buffer descripto
 __device__ __forceinline__ double bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
+// rows no other chunk reads: non-temporal loads (cache-policy bit 1), which keep
+// the halo rows the neighbouring chunks re-read in the cache
+__device__ __forceinline__ double bload_nt(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 2));
+}
 
 // results are written once and read by a later launch (from HBM at these sizes):
 // non-temporal stores (cache-policy bit 1)
@@ -158,6 +163,8 @@ __global__ __launch_bounds__(1024) void lineop_strided(OpDev op, LaunchArgs a) {
             } else {
                 w[j] = outside ? 0.0 : bload(rin, boff, (unsigned)p * rowb);
             }
+        } else if (j >= 2 * H && j < MM) {
+            w[j] = bload_nt(rin, boff, (unsigned)(s0 + j) * rowb);
         } else {
             w[j] = bload(rin, boff, (unsigned)(s0 + j) * rowb);
         }
@@ -206,7 +213,7 @@ __global__ __launch_bounds__(1024) void lineop_strided(OpDev op, LaunchArgs a) {
     }
     if (a.accumulate) {
 #pragma unroll
-        for (int i = 0; i < MM; ++i) y[i] = bload(rout, boff, (unsigned)(cs + i) * rowb) + y[i];
+        for (int i = 0; i < MM; ++i) y[i] = bload_nt(rout, boff, (unsigned)(cs + i) * rowb) + y[i];
     }
 #pragma unroll
     for (int i = 0; i < MM; ++i) bstore(y[i], rout, boff, (unsigned)(cs + i) * rowb);
