@@ -178,6 +178,12 @@ int mc3d_solver_write_dump(mc3d_solver *s, const char *path, const double *x, co
  * into dir (NULL: working directory), with the given host mesh (copied). */
 int mc3d_solver_set_output(mc3d_solver *s, const char *dir, const double *x, const double *y,
                            const double *z);
+/* The metric identities I_k = dJ_1k/dxi + dJ_2k/deta + dJ_3k/dzeta (k = 1..3)
+ * of the current metrics with the solver's first derivatives, as
+ * range = {min I_1, max I_1, min I_2, max I_2, min I_3, max I_3}: the check
+ * AbstractSingleBlockMesh::solveForJacobians prints
+ * (src/AbstractSingleBlockMesh.cpp:1336-1393; set_mesh prints it too). */
+int mc3d_solver_metric_identity(mc3d_solver *s, double range[6]);
 /* PNG image output of a field plane (PngWriter, src/PngWriter.hpp:83-190;
  * CurvilinearCSolver::addImageOutput, src/CurvilinearCSolver_Core.cpp:1438):
  * every `interval` steps (0: the first step only) the index plane nearest to

@@ -76,6 +76,12 @@ int mc3d_solver_set_mesh(mc3d_solver *s, const double *x, const double *y, const
     });
 }
 
+int mc3d_solver_metric_identity(mc3d_solver *s, double range[6]) {
+    if (!s || !range) return mc3d::fail(MC3D_EINVAL, "null argument");
+    if (!s->metrics) return mc3d::fail(MC3D_ECONFIG, "no metrics: mc3d_solver_set_mesh or mc3d_solver_set_metrics");
+    return guarded([&] { s->cs->metricIdentity(range, false); });
+}
+
 int mc3d_mesh_metrics(const int fd[3], const int bc[3], const int dims[3], const double trans[9],
                       const double *x, const double *y, const double *z, double *const out[10],
                       void *stream) {

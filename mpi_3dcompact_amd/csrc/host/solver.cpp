@@ -695,6 +695,12 @@ void CurvilinearCSolver::set_mesh_host(const double *x, const double *y, const d
     for (int k = 0; k < 3; ++k)
         hip_check(hipMemcpyAsync(d + k * N, h[k], sizeof(double) * N, hipMemcpyHostToDevice, st), "mesh upload");
     solveForJacobians(d, d + N, d + 2 * N);
+    // the reference reports the metric ranges and identities after forming them
+    // (src/AbstractSingleBlockMesh.cpp:1323-1393)
+    static const char *names[10] = {"J", "J11", "J12", "J13", "J21", "J22", "J23", "J31", "J32", "J33"};
+    for (int k = 0; k < 10; ++k) getRange(msh_rw(k), names[k]);
+    double range[6];
+    metricIdentity(range, true);
     setSpongeFromMesh(x, y, z);
 }
 
@@ -1093,19 +1099,52 @@ void CurvilinearCSolver::updateNonConservedData() {
 
 // src/Utils.cpp:415-431: the sequential fold std::min(phi, acc) from 1e6 (and
 // std::max from -1e6); for finite data the block-wise min/max gives the same
-void CurvilinearCSolver::getRange(const double *phi, const char *name) {
+void CurvilinearCSolver::fieldRange(const double *phi, double &lo, double &hi) {
     const int nb = 512;  // 2 * nb doubles fit the scratch buffer
     hip_check(launch_minmax_partial(phi, cfl_partial, nb, N, st), "range");
     std::vector<double> part(2 * nb);
     hip_check(hipMemcpyAsync(part.data(), cfl_partial, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, st),
               "range read-back");
     hip_check(hipStreamSynchronize(st), "range read-back");
-    double lo = 1000000.0, hi = -1000000.0;
+    lo = 1000000.0;
+    hi = -1000000.0;
     for (int b = 0; b < nb; ++b) {
         lo = std::min(part[2 * b], lo);
         hi = std::max(part[2 * b + 1], hi);
     }
+}
+
+void CurvilinearCSolver::getRange(const double *phi, const char *name) {
+    double lo, hi;
+    fieldRange(phi, lo, hi);
     std::printf("   Range of %s: %g:%g\n", name, lo, hi);
+}
+
+// src/AbstractSingleBlockMesh.cpp:1336-1393: I_k = (dJ_1k/dxi + dJ_2k/deta) +
+// dJ_3k/dzeta with the solver's plain first derivatives (no periodic
+// transformation, as in the reference), then the ranges
+void CurvilinearCSolver::metricIdentity(double range[6], bool print) {
+    void *mem = nullptr;
+    hip_check(hipMalloc(&mem, sizeof(double) * (size_t)N), "identity scratch");
+    struct Free {
+        void *p;
+        hipStream_t s;
+        ~Free() {
+            (void)hipStreamSynchronize(s);
+            (void)hipFree(p);
+        }
+    } guard{mem, st};
+    double *sum = static_cast<double *>(mem);
+    if (print) std::printf(" > Checking the values of the metric indentities, values should be small\n");
+    for (int k = 0; k < 3; ++k) {
+        const double *a = msh_rw(1 + k), *b = msh_rw(4 + k), *c = msh_rw(7 + k);
+        derivX->calc1stDerivFields(1, &a, &sum, false);
+        derivY->calc1stDerivFields(1, &b, &sum, true);
+        derivZ->calc1stDerivFields(1, &c, &sum, true);
+        fieldRange(sum, range[2 * k], range[2 * k + 1]);
+        if (print) std::printf("   Range of Metric Identity %d: %g:%g\n", k + 1, range[2 * k], range[2 * k + 1]);
+    }
+    if (print) std::fflush(stdout);
 }
 
 // src/CurvilinearCSolver_Core.cpp:1244-1329

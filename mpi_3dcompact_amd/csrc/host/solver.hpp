@@ -361,6 +361,11 @@ class CurvilinearCSolver {
     void postStep();              // checks, dumps, end conditions
     void checkSolution();         // field ranges every CHECKSTEP steps
     void getRange(const double *phi, const char *name);
+    void fieldRange(const double *phi, double &lo, double &hi);
+    // the metric identities I_k = d(J_1k)/dxi + d(J_2k)/deta + d(J_3k)/dzeta of
+    // the current metrics with the solver's derivatives, as (min, max) pairs
+    // (src/AbstractSingleBlockMesh.cpp:1336-1393); print: the reference's log lines
+    void metricIdentity(double range[6], bool print);
     void stage_update(int op, double a, double b, double c);
     void filterConservedData();
     void updateNonConservedData();

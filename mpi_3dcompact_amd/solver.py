@@ -34,6 +34,7 @@ _SIGS = [
     ("mc3d_solver_add_image", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
                                              ctypes.c_char_p, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                              ctypes.c_double, ctypes.c_double, ctypes.c_int]),
+    ("mc3d_solver_metric_identity", ctypes.c_int, [_VP, _D]),
     ("mc3d_solver_slice", ctypes.c_int, [_VP, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _D]),
     ("mc3d_image_write", ctypes.c_int, [ctypes.c_char_p, _D, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
@@ -137,6 +138,12 @@ class Solver:
         lo, hi = vrange if vrange is not None else (0.0, 1.0)
         check(self._L.mc3d_solver_add_image(self._h, interval, width, height, field.encode(), var_name.encode(),
                                             plane, fraction, vrange is not None, lo, hi, cm))
+
+    def metric_identity(self):
+        """(min, max) of the three metric identities of the current metrics."""
+        out = (ctypes.c_double * 6)()
+        check(self._L.mc3d_solver_metric_identity(self._h, out))
+        return [(out[2 * k], out[2 * k + 1]) for k in range(3)]
 
     def slice(self, field, direction, index):
         """Index plane of a field normal to `direction`, shaped (n2, n1) with the

@@ -58,6 +58,17 @@ def test_cylinder_case_matches_the_oracle(gpu, fd, filt, rk):
     worst = max(np.max(np.abs(a - b)) / (sj if k == 0 else sm) for k, (a, b) in enumerate(zip(gm, om)))
     _say(f"  metrics: {worst:.2e} of the largest value (J apart from J_ij)")
     assert worst <= 1e-9
+    # the metric identities the reference prints after forming the metrics
+    # (src/AbstractSingleBlockMesh.cpp:1336-1393): plain first derivatives of
+    # the J_ij, summed (xi + eta) + zeta, against the oracle's derivatives
+    bc = (P.PERIODIC, P.DIRICHLET, P.PERIODIC)
+    dd = (1.0 / Nx, 1.0 / (Ny - 1), 1.0 / Nz)
+    got = g.metric_identity()
+    for k in range(3):
+        terms = [P.deriv_field(fdc[0], d, bc[d], gm[1 + 3 * d + k], dd[d]) for d in range(3)]
+        ident = (terms[0] + terms[1]) + terms[2]
+        scale = max(np.max(np.abs(t)) for t in terms)
+        assert abs(got[k][0] - ident.min()) <= 1e-12 * scale and abs(got[k][1] - ident.max()) <= 1e-12 * scale, k
     # the cylindrical sponge strength from the same mesh
     fk = [P.FACE[k] for k in hit.face_kinds(cylinder.FACES)]
     sig = P.sponge_sigma(x, y, z, fk, kind=1, strength=1.0, cyl_orient=2, axis=(0.0, 0.0, 0.0), rmin=rmin)

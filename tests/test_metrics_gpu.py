@@ -143,6 +143,21 @@ def test_solver_set_mesh_matches_oracle_run(torch):
                                               "J33")], ref_m)
     _say("   set_mesh max rel err:", " ".join(f"{e:.1e}" for e in errs))
     assert max(errs) <= TOL, errs
+    # the metric identities (src/AbstractSingleBlockMesh.cpp:1336-1393) of these
+    # metrics against the oracle's plain derivatives of the same fields: the
+    # transformed-periodic metrics of the sheared box vary near the wrap (the
+    # reference's own behaviour), so the identities are not zero here
+    gm = [g.field(n) for n in ("J", "J11", "J12", "J13", "J21", "J22", "J23", "J31", "J32", "J33")]
+    got = g.metric_identity()
+    for k in range(3):
+        terms = [P.deriv_field(P.PENTA10, d, PER, gm[1 + 3 * d + k], 1.0 / N) for d in range(3)]
+        ident = (terms[0] + terms[1]) + terms[2]
+        scale = max(np.max(np.abs(t)) for t in terms)
+        assert abs(got[k][0] - ident.min()) <= 1e-12 * scale and abs(got[k][1] - ident.max()) <= 1e-12 * scale, k
+    # constant metrics: the identities vanish to rounding
+    h = Solver(hit.options_text(N, **case), (N, N, N))
+    h.set_metrics(hit.identity_metrics(N))
+    assert max(max(abs(lo), abs(hi)) for lo, hi in h.metric_identity()) <= 1e-13
     ic = hit.hit_initial_condition(N, k0=3.0, Mt=0.2, seed=4)
     g.set_initial(ic)
     g.step(10)
