@@ -21,6 +21,9 @@ struct mc3d_lineop {
     int device = 0;
     bool fast = false;      // plain applications use the fast kernels
     std::string kname;      // kernel of plain applications
+    // transformed-periodic applications run the LDS-tiled kernel, which takes
+    // 16- and 32-row chunks only: a 16-row twin when the plain plan has others
+    mc3d_lineop *tpb = nullptr;
 };
 
 namespace {
@@ -82,7 +85,7 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
     // 16-row chunks double them, and measured faster for every x operator (the
     // Penta10 derivative of 256-point lines inside the LES step: 3.3 -> 2.9 ms).
     int first = tune.mm;
-    if (!first && dir == 0) first = 16;
+    if (!first && dir == 0 && dims[0] % 16 == 0) first = 16;
     const int mms[2] = {first, first ? first : 16};
     for (int mm : mms) {
         mc3d::PlanRequest rq{cls, kind, bc, dims[dir], dd, alphaF, mm};
@@ -119,6 +122,10 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
         if (tune.family == 2) op->fast = false;
         if (op->fast || p.MM == 16 || first) break;
     }
+    if (!op->fast && op->plan.MM != 16 && op->plan.MM != 32) {  // the LDS-tiled kernel's chunk sizes
+        delete op;
+        return mc3d::fail(MC3D_EINVAL, "no kernel for this line operator and chunking");
+    }
     const mc3d::Plan &p = op->plan;
     if (tune.family == 1 && !op->fast) {
         delete op;
@@ -154,6 +161,16 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
         if (op->dmem) (void)hipFree(op->dmem);
         delete op;
         return rc;
+    }
+    if (cls == mc3d::kDeriv && bc == MC3D_PERIODIC && p.MM != 16 && p.MM != 32) {
+        mc3d::LineTune t{};
+        t.mm = 16;
+        t.family = 2;
+        rc = create(cls, kind, dir, bc, bc3, dims, dd, alphaF, nbatch, t, &op->tpb);
+        if (rc) {
+            mc3d_lineop_destroy(op);
+            return rc;
+        }
     }
     *out = op;
     return MC3D_OK;
@@ -213,7 +230,7 @@ int mc3d_deriv_apply_tpb(const mc3d_lineop *op, const double *in, double *out,
     for (int k = 3 - op->plan.H; k < 3 + op->plan.H; ++k)
         if (!wrap[k]) return mc3d::fail(MC3D_EINVAL, "missing wrap array for the scheme's half width");
     if (in == out) return mc3d::fail(MC3D_EINVAL, "derivative input and output must not alias");
-    return apply(op, &in, &out, 1, mc3d::kModeTPB, wrap, 0, stream);
+    return apply(op->tpb ? op->tpb : op, &in, &out, 1, mc3d::kModeTPB, wrap, 0, stream);
 }
 
 int mc3d_filter_create(int kind, double alphaF, int dir, const int bc3[3], const int dims[3],
@@ -233,6 +250,7 @@ int mc3d_filter_apply(const mc3d_lineop *op, const double *const *in, double *co
 
 void mc3d_lineop_destroy(mc3d_lineop *op) {
     if (!op) return;
+    mc3d_lineop_destroy(op->tpb);
     if (op->dmem) {
         int cur = 0;
         (void)hipGetDevice(&cur);

@@ -26,6 +26,7 @@
 
 #include "lineop.h"
 #include "lineop_launch.h"
+#include "lineop_plan.h"
 
 namespace mc3d {
 
@@ -234,7 +235,9 @@ constexpr int contig_lines() { return PER ? 16 : 64; }
 // kernels (64-line tiles of 1024 threads, 128 VGPRs) and the 16-row H = 5
 // periodic ones spill with 16
 template <int H, int MM, bool PER>
-constexpr int contig_batch() { return MM == 32 || (PER && H < 5) ? 16 : 8; }
+constexpr int contig_batch() {
+    return MM % 16 ? (MM <= 12 ? MM : MM / 2) : (MM == 32 || (PER && H < 5) ? 16 : 8);
+}
 
 // (line, point) of element tid + k * nthr of a LINES x N tile, advanced
 // incrementally (nthr = qd * N + rd)
@@ -412,8 +415,18 @@ hipError_t launch_fast_bc(const OpDev &op, const LaunchArgs &a, int nfields, hip
 
 template <int W, int H>
 hipError_t launch_fast_mm(const OpDev &op, const LaunchArgs &a, int nfields, hipStream_t st) {
-    return op.MM == 32 ? launch_fast_bc<W, H, 32>(op, a, nfields, st)
-                       : launch_fast_bc<W, H, 16>(op, a, nfields, st);
+    switch (op.MM) {
+        case 32: return launch_fast_bc<W, H, 32>(op, a, nfields, st);
+        case 28: return launch_fast_bc<W, H, 28>(op, a, nfields, st);
+        case 24: return launch_fast_bc<W, H, 24>(op, a, nfields, st);
+        case 20: return launch_fast_bc<W, H, 20>(op, a, nfields, st);
+        case 16: return launch_fast_bc<W, H, 16>(op, a, nfields, st);
+        // 8- and 12-row chunks: periodic lines only (the end rows of a
+        // Dirichlet line read 16 window points)
+        case 12: return launch_fast<W, H, 12, true>(op, a, nfields, st);
+        case 8: return launch_fast<W, H, 8, true>(op, a, nfields, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace
@@ -434,8 +447,8 @@ bool lineop_fast_supported(const OpDev &op, size_t lds_limit) {
     const int threads = 64 * op.P;
     // Dirichlet interior rows must not reach past the line ends
     const bool ends = op.periodic || (op.nbl >= h && op.nbr >= h && op.nbl <= kSpecRows && op.nbr <= kSpecRows);
-    return pair && ends && (op.MM == 16 || op.MM == 32) && op.N == op.P * op.MM && threads <= 1024 &&
-           lineop_fast_lds_bytes(op) <= lds_limit;
+    const bool rows = chunk_rows_ok(op.MM) && (op.MM >= 16 || op.periodic) && op.MM >= 2 * h;
+    return pair && ends && rows && op.N == op.P * op.MM && threads <= 1024 && lineop_fast_lds_bytes(op) <= lds_limit;
 }
 
 hipError_t lineop_launch_fast(const OpDev &op, const LaunchArgs &a, int nfields, hipStream_t st) {

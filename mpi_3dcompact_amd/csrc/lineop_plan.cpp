@@ -414,10 +414,21 @@ int build_plan(const PlanRequest &rq, Plan &plan, std::string &err) {
     }
     const int W = s.W;
     // chunking: 32-row chunks when the line splits evenly into at least two,
-    // else 16-row chunks (the last one possibly shorter)
+    // 16-row ones when it splits evenly into those, else the largest of 28, 24,
+    // 20 (and, periodic lines only, 12 and 8) rows that splits it evenly into
+    // 2..16 chunks, so that the fast kernels take it; else 16-row chunks, the
+    // last one shorter (the LDS-tiled kernel)
     int mm = rq.mm;
-    if (mm == 0) mm = (n % 32 == 0 && n / 32 >= 2) ? 32 : 16;
-    if (mm != 16 && mm != 32) { err = "chunk rows must be 16 or 32"; return -1; }
+    if (mm == 0) {
+        if (n % 32 == 0 && n / 32 >= 2) mm = 32;
+        else if (n % 16 == 0) mm = 16;
+        else {
+            for (int c : {28, 24, 20, 12, 8})
+                if (n % c == 0 && n / c >= 2 && n / c <= 16 && (per || c >= 16)) { mm = c; break; }
+            if (!mm) mm = 16;
+        }
+    }
+    if (!chunk_rows_ok(mm)) { err = "chunk rows must be 8, 12, 16, 20, 24, 28 or 32"; return -1; }
     int P = (n + mm - 1) / mm;
     if (per && W > 0 && P < 2) P = 2;
     int M = (n + P - 1) / P;

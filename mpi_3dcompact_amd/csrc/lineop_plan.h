@@ -8,6 +8,12 @@
 
 namespace mc3d {
 
+// rows per chunk the plans and the fast kernels are built for (the LDS-tiled
+// kernel takes 16 and 32 only)
+inline bool chunk_rows_ok(int mm) {
+    return mm == 8 || mm == 12 || mm == 16 || mm == 20 || mm == 24 || mm == 28 || mm == 32;
+}
+
 struct PlanRequest {
     int cls;        // kDeriv or kFilter
     int kind;       // scheme / filter id

@@ -221,3 +221,43 @@ def test_kernel_level_config_256(torch_cuda):
             assert relerr(o[:7], ref[:7]) <= TOL
         else:
             assert relerr(o, ref) <= TOL
+
+
+# lines that split evenly into 20-, 24-, 28-row chunks (any boundary) or 12-row
+# ones (periodic): the fast kernels take them (src/Pade6.cpp etc. restated by the
+# oracle; the chunking is this library's)
+EVEN_SHAPES = [((24, 56, 40), {0: 20, 1: 28, 2: 12}), ((36, 72, 24), {0: 12, 1: 24, 2: 12}),
+               ((12, 16, 200), {0: 20, 1: 16, 2: 16})]
+
+
+@pytest.mark.parametrize("shape,mm", EVEN_SHAPES, ids=["40x56x24", "24x72x36", "200x16x12"])
+def test_uneven_sizes_take_the_fast_kernels(torch_cuda, shape, mm):
+    from mpi_3dcompact_amd.ops import LineOp
+    torch = torch_cuda
+    Nz, Ny, Nx = shape
+    dims = (Nx, Ny, Nz)
+    f = field(shape, 7)
+    fin = torch.from_numpy(f).cuda()
+    out = torch.empty_like(fin)
+    for d in range(3):
+        n = dims[d]
+        for bc in (P.PERIODIC, P.DIRICHLET):
+            for scheme in (P.PADE6, P.PENTA10):
+                dd = 1.0 / n if bc == P.PERIODIC else 1.0 / (n - 1)
+                op = LineOp.deriv(scheme, d, bc, dims, dd)
+                kname = op.kernel()
+                _say(f"deriv scheme={scheme} bc={bc} shape={shape} dir={d} kernel={kname}")
+                fast = mm[d] >= 16 or bc == P.PERIODIC
+                if n % mm[d] == 0 and n // mm[d] >= 2 and fast:  # a periodic line needs two chunks
+                    want = ("lineop_contig<" if d == 0 else "lineop_strided<")
+                    assert kname.startswith(want) and f", {mm[d]}, " in kname, kname
+                op.apply([fin], [out])
+                torch.cuda.synchronize()
+                assert relerr(out.cpu().numpy(), P.deriv_field(scheme, d, bc, f, dd)) <= TOL, (scheme, bc, d)
+        for kind, af in ((P.COMPACT8, 0.49), (P.COMPACT10, 0.49), (P.COMMN5PADE, 0.0)):
+            for bc3 in ((0, 0, 0), (1, 1, 1)):
+                _say(f"filter kind={kind} bc3={bc3} shape={shape} dir={d}")
+                op = LineOp.filter(kind, af, d, bc3, dims)
+                op.apply([fin], [out])
+                torch.cuda.synchronize()
+                assert relerr(out.cpu().numpy(), P.filter_field(kind, af, d, list(bc3), f)) <= TOL, (kind, bc3, d)
