@@ -196,7 +196,7 @@ def _check(t, ref_fn, n):
 
 @pytest.mark.parametrize("mm", [0, 16])
 @pytest.mark.parametrize("bc", [P.PERIODIC, P.DIRICHLET])
-@pytest.mark.parametrize("n", [11, 17, 32, 48, 64, 96, 256])
+@pytest.mark.parametrize("n", [11, 17, 32, 36, 40, 48, 56, 64, 72, 96, 200, 256])
 @pytest.mark.parametrize("scheme", DERIVS)
 def test_deriv_plan_matches_oracle(scheme, n, bc, mm):
     dd = 1.0 / n if bc == P.PERIODIC else 1.0 / (n - 1)
@@ -206,7 +206,7 @@ def test_deriv_plan_matches_oracle(scheme, n, bc, mm):
 
 @pytest.mark.parametrize("mm", [0, 16])
 @pytest.mark.parametrize("bc", [P.PERIODIC, P.DIRICHLET])
-@pytest.mark.parametrize("n", [11, 17, 32, 48, 64, 96, 256])
+@pytest.mark.parametrize("n", [11, 17, 32, 36, 40, 48, 56, 64, 72, 96, 200, 256])
 @pytest.mark.parametrize("kind,af", FILTS)
 def test_filter_plan_matches_oracle(kind, af, n, bc, mm):
     t = tables(1, kind, bc, n, 1.0, af, mm=mm)
@@ -220,6 +220,18 @@ def test_chunking_choice():
     assert tables(0, P.PENTA10, P.PERIODIC, 48, 1 / 48)["MM"] == 16
     assert tables(0, P.PENTA10, P.PERIODIC, 32, 1 / 32)["MM"] == 16
     assert tables(0, P.PADE6, P.DIRICHLET, 17, 1 / 16)["MM"] == 16
+    # lengths that are not multiples of 16: the largest of 28, 24, 20 rows (and
+    # 12, 8 on periodic lines) that splits the line into 2..16 equal chunks
+    for n, bc, mm, chunks in ((200, P.PERIODIC, 20, 10), (200, P.DIRICHLET, 20, 10), (56, P.DIRICHLET, 28, 2),
+                              (72, P.PERIODIC, 24, 3), (36, P.PERIODIC, 12, 3), (24, P.PERIODIC, 12, 2)):
+        t = tables(0, P.PENTA10, bc, n, 1.0 / n)
+        assert (t["MM"], t["P"]) == (mm, chunks), (n, bc)
+    # no such divisor, or a Dirichlet line that only 12 rows would split: 16-row chunks, the last shorter
+    assert tables(0, P.PENTA10, P.PERIODIC, 150, 1 / 150)["MM"] == 16
+    assert tables(0, P.PENTA10, P.DIRICHLET, 36, 1 / 35)["MM"] == 16
+    from mpi_3dcompact_amd._lib import Mc3dError
+    with pytest.raises(Mc3dError):
+        tables(0, P.PENTA10, P.PERIODIC, 100, 1 / 100, mm=10)
 
 
 @pytest.mark.parametrize("kind", [P.PADE6, P.PENTA10, P.CD2])
