@@ -7,14 +7,24 @@
  * mc3d_last_error().  The library never aborts the process.
  *
  * Conventions
- *  - Fields are device-resident FP64 arrays of Nx*Ny*Nz values in one layout,
- *    x fastest: index = (k*Ny + j)*Nx + i.  The reference keeps three pencil
- *    layouts and transposes between them; per-line arithmetic does not depend
- *    on the layout, so this library keeps one and solves strided lines in place.
- *  - "Lines" of direction d are enumerated by the two other indices with the
- *    lower direction fastest (x-lines: q = k*Ny + j; y-lines: q = k*Nx + i;
- *    z-lines: q = j*Nx + i).  Per-line side arrays (transformed-periodic wrap
- *    values) use this order.
+ *  - Fields are device-resident FP64 arrays of Nx*Ny*Nz values.  The solver
+ *    keeps one layout, x fastest: index = (k*Ny + j)*Nx + i, and solves the
+ *    strided lines in place.  The operators and SGS models also take the
+ *    reference's other two pencil layouts (src/Macros.hpp:16-18), selected by
+ *    a layout argument:
+ *      MC3D_XPENCIL  index = (k*Ny + j)*Nx + i   (x fastest, then y, z)
+ *      MC3D_YPENCIL  index = (i*Nz + k)*Ny + j   (y fastest, then z, x)
+ *      MC3D_ZPENCIL  index = (j*Nx + i)*Nz + k   (z fastest, then x, y)
+ *    The reference calls calc1stDerivField / filterField of direction x, y, z
+ *    with fields in the X, Y, Z pencil respectively (src/Pade6.cpp:6-43), and
+ *    its SGS models work on Y pencils (src/AbstractSGS.cpp, FOR_XYZ_YPEN).
+ *  - "Lines" of direction d are enumerated by the two other indices in the
+ *    order of the layout, the faster one fastest.  x-fastest: x-lines
+ *    q = k*Ny + j, y-lines q = k*Nx + i, z-lines q = j*Nx + i.  In each
+ *    direction's own pencil this is the reference's order of its per-line
+ *    wrap arrays (x: k*Ny + j, y: i*Nz + k, z: j*Nx + i;
+ *    src/Pade6.cpp:56,76,95, src/AbstractSingleBlockMesh.cpp:151,61,242).
+ *    Per-line side arrays (transformed-periodic wrap values) use this order.
  *  - Calls are asynchronous on the given stream (a hipStream_t passed as
  *    void*; NULL is the default stream).  The caller owns all field memory;
  *    handles own their precomputed factor tables.
@@ -38,6 +48,8 @@ extern "C" {
 #define MC3D_ECONFIG (-5)     /* options file / configuration error */
 
 enum { MC3D_PERIODIC = 0, MC3D_DIRICHLET = 1 };
+/* field layouts: the reference's X, Y and Z pencils (src/Macros.hpp:16-18) */
+enum { MC3D_XPENCIL = 0, MC3D_YPENCIL = 1, MC3D_ZPENCIL = 2 };
 /* derivative schemes: src/Options.hpp:22 (FDType) */
 enum { MC3D_CD2 = 0, MC3D_PADE6 = 1, MC3D_PENTA10 = 2 };
 /* filter kinds: src/Options.hpp:23 (FilterType) plus the two DSM test filters */
@@ -53,9 +65,11 @@ int mc3d_device_count(int *count);
  * Replaces: AbstractDerivatives (src/AbstractDerivatives.hpp:13-53) and its
  * constructors Pade6(Domain*, BCType, Direct) src/Pade6.hpp:39,
  * Penta10(...) src/Penta10.hpp:38, CD2(...) src/CD2.hpp:25.
- * dims = {Nx, Ny, Nz}; dd = grid spacing along dir in computational space.
+ * dims = {Nx, Ny, Nz}; dd = grid spacing along dir in computational space;
+ * layout = MC3D_XPENCIL / YPENCIL / ZPENCIL, the layout of the fields the
+ * operator is applied to (the reference's: the pencil of dir).
  */
-int mc3d_deriv_create(int scheme, int dir, int bc, const int dims[3], double dd,
+int mc3d_deriv_create(int scheme, int dir, int bc, const int dims[3], double dd, int layout,
                       mc3d_lineop **out);
 
 /* out[f] = d(in[f])/d(xi_dir) for f < nfields (nfields <= 16; in and out
@@ -80,7 +94,7 @@ int mc3d_deriv_apply_tpb(const mc3d_lineop *op, const double *in, double *out,
  * Compact10Filter src/Compact10Filter.hpp:30, CommN5PadeTestFilter
  * src/CommN5PadeTestFilter.hpp:28, CommN5ExpTestFilter src/CommN5ExpTestFilter.hpp:24.
  */
-int mc3d_filter_create(int kind, double alphaF, int dir, const int bc3[3], const int dims[3],
+int mc3d_filter_create(int kind, double alphaF, int dir, const int bc3[3], const int dims[3], int layout,
                        mc3d_lineop **out);
 
 /* Replaces filterField(in, out) src/AbstractFilter.hpp:34.  in may equal out. */
@@ -98,6 +112,76 @@ int mc3d_lineop_info(const mc3d_lineop *op, int *chunks, int *chunk_rows, int *l
  * "lineop_strided<2, 3, 32, true>" (half bandwidth, stencil half width, rows
  * per chunk, periodic).  NULL for a NULL operator. */
 const char *mc3d_lineop_kernel(const mc3d_lineop *op);
+
+/* ---------------------------------------------------------------------------
+ * CFL time step over caller-owned fields.
+ * Replaces CurvilinearCSolver::calcDtFromCFL (src/CurvilinearCSolver_Core.cpp:189-237):
+ *   maxspeed = max over points of J11 (|U| + c)/dx + J22 (|V| + c)/dy + J33 (|W| + c)/dz,
+ *   dt = cfl / maxspeed (TIMESTEPPING CONST_CFL; for CONST_DT the reference
+ *   sets CFL = dt * maxspeed, which the caller forms from out[1]).
+ * fields[7] = {U, V, W, sos, J11, J22, J33}, device, npts * members doubles
+ * each, in any layout (the maximum does not depend on it); members boxes of
+ * npts points stacked one after the other each get their own maximum and dt:
+ * out[2 b] = dt, out[2 b + 1] = maxspeed of box b (device doubles).
+ * spacing = {dx, dy, dz} of the computational grid.  The per-workgroup
+ * partial maxima (up to 2048 doubles per box) are allocated for the call,
+ * and the call waits for the stream before it returns.
+ */
+int mc3d_cfl_dt(const double *const fields[7], long npts, int members, const double spacing[3], double cfl,
+                double *out, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * SGS models over caller-owned device fields.
+ * Replaces: AbstractSGS (src/AbstractSGS.hpp:9-42) and its subclasses
+ * VremanSGS(cs) src/VremanSGS.hpp:15-41, DSMSGS(cs) src/DSMSGS.hpp:34-111, and
+ * the learned closure beside them; doAveraging (src/AbstractSGS.cpp:13-164)
+ * runs inside each call as the reference's models call it.
+ * options_text: the [DOMAIN] NX NY NZ, [BC] BCX/BCY/BCZ types and [LES] keys
+ * (LESMODEL VREMAN | DSM | LEARNED, LESAVERAGING, DSMTESTFILTER, DSMUSETAUKK,
+ * DSMCOEFFRANGE, CLOSUREFILE, CLOSUREFALLBACK ...) of the reference's options
+ * file (other sections may be present and are ignored).
+ * layout: of every field passed (the reference's SGS fields are Y pencils,
+ * MC3D_YPENCIL).  J: device Jacobian field (the mesh's J,
+ * src/AbstractSingleBlockMesh.cpp:1297-1308), kept by pointer for the
+ * filter width (dx dy dz / J)^(1/3).  Work is queued on `stream` (a
+ * hipStream_t, NULL: the default stream); the model's work fields (up to about
+ * 40 fields for the DSM) are allocated at creation.
+ * gradU[3*i + j] = d u_i / d x_j (Cartesian, the reference's gradU[i][j],
+ * src/CurvilinearCSolver_Core.cpp:548-561); gradT[j] = dT/dx_j.
+ * Outputs are caller-owned device fields.
+ */
+typedef struct mc3d_sgs mc3d_sgs;
+typedef struct mc3d_closure mc3d_closure;
+
+int mc3d_sgs_create(const char *options_text, int layout, const double *J, void *stream, mc3d_sgs **out);
+void mc3d_sgs_destroy(mc3d_sgs *s);
+/* VremanSGS::getSGSViscosity (src/VremanSGS.hpp:43-96): mu_sgs, then the
+ * LESAVERAGING smoothing of mu_sgs (Compact10 alpha_f -0.49 filters for LOCAL). */
+int mc3d_sgs_vreman(mc3d_sgs *s, const double *const gradU[9], const double *rho, double *mu_sgs);
+/* DSMSGS::calcMuSGSTaukk (src/DSMSGS.hpp:124-343): mu_sgs and tau_kk (zero
+ * unless DSMUSETAUKK).  The caller applies tau_kk to p as the solver does
+ * (src/CurvilinearCSolver_Core.cpp:571-584) before forming gradT. */
+int mc3d_sgs_dsm_mu_taukk(mc3d_sgs *s, const double *const gradU[9], const double *rho, const double *rhoU,
+                          const double *rhoV, const double *rhoW, double *mu_sgs, double *taukk);
+/* DSMSGS::calcKSGS (src/DSMSGS.hpp:345-444): k_sgs; uses the test-filtered
+ * fields of the preceding mc3d_sgs_dsm_mu_taukk call (same state). */
+int mc3d_sgs_dsm_ksgs(mc3d_sgs *s, const double *const gradT[3], const double *rho, const double *rhoU,
+                      const double *rhoV, const double *rhoW, const double *T, double *k_sgs);
+/* DSMCOEFFRANGE: C, CI and Prt of the last DSM calls
+ * (src/DSMSGS.hpp:104-109,330-341,435-442) copied into caller fields (stream
+ * ordered); NULL outputs are skipped. */
+int mc3d_sgs_coefficients(mc3d_sgs *s, double *C, double *CI, double *Prt);
+/* Learned closure (LESMODEL LEARNED; network from CLOSUREFILE or
+ * mc3d_sgs_set_closure): mu_sgs and k_sgs of the network, with the DSM
+ * fallback of CLOSUREFALLBACK, which needs gradT, rhoU, rhoV, rhoW and T
+ * (they may be NULL with CLOSUREFALLBACK FALSE).  Waits for the stream
+ * (the fallback decision reads the out-of-range count). */
+int mc3d_sgs_set_closure(mc3d_sgs *s, const mc3d_closure *c);
+int mc3d_sgs_learned(mc3d_sgs *s, const double *const gradU[9], const double *const gradT[3], const double *rho,
+                     const double *rhoU, const double *rhoV, const double *rhoW, const double *T, double *mu_sgs,
+                     double *k_sgs);
+int mc3d_sgs_closure_stats(mc3d_sgs *s, long *evaluations, long *outside_points, long *fallback_stages,
+                           long *fallback_points);
 
 /* ---------------------------------------------------------------------------
  * Whole solver step (CurvilinearCSolver + RK + SGS + wall and sponge
@@ -262,8 +346,6 @@ int mc3d_solver_timing_report(mc3d_solver *s, char *buf, int buflen, int *needed
  * b3[64] W4[2][64] b4[2].  Training data: X[n][7] standardised inputs and
  * T[n][2] standardised targets, FP32 device arrays.
  * ------------------------------------------------------------------------- */
-typedef struct mc3d_closure mc3d_closure;
-
 /* weights U(-s, s), s = sqrt(3 / fan_in), zero biases; identity scaling */
 int mc3d_closure_create(unsigned long long seed, mc3d_closure **out);
 void mc3d_closure_destroy(mc3d_closure *c);

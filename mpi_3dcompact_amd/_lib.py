@@ -11,6 +11,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmc3d.so")
 
 MC3D_PERIODIC, MC3D_DIRICHLET = 0, 1
+MC3D_XPENCIL, MC3D_YPENCIL, MC3D_ZPENCIL = 0, 1, 2
 MC3D_CD2, MC3D_PADE6, MC3D_PENTA10 = 0, 1, 2
 MC3D_COMPACT8, MC3D_COMPACT10, MC3D_COMMN5PADE, MC3D_COMMN5EXP = 0, 1, 2, 3
 
@@ -23,12 +24,12 @@ SIGNATURES = [
     ("mc3d_last_error", ctypes.c_char_p, []),
     ("mc3d_device_count", ctypes.c_int, [_I]),
     ("mc3d_deriv_create", ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _I,
-                                         ctypes.c_double, ctypes.POINTER(_VP)]),
+                                         ctypes.c_double, ctypes.c_int, ctypes.POINTER(_VP)]),
     ("mc3d_deriv_apply", ctypes.c_int, [_VP, ctypes.POINTER(_VP), ctypes.POINTER(_VP),
                                         ctypes.c_int, _VP]),
     ("mc3d_deriv_apply_tpb", ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(_VP), _VP]),
     ("mc3d_filter_create", ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int, _I, _I,
-                                          ctypes.POINTER(_VP)]),
+                                          ctypes.c_int, ctypes.POINTER(_VP)]),
     ("mc3d_filter_apply", ctypes.c_int, [_VP, ctypes.POINTER(_VP), ctypes.POINTER(_VP),
                                          ctypes.c_int, _VP]),
     ("mc3d_lineop_destroy", None, [_VP]),

@@ -15,7 +15,8 @@ struct mc3d_lineop {
     mc3d::Plan plan;
     mc3d::OpDev dev{};
     int cls = 0, kind = 0, dir = 0, bc = 0;
-    int dims[3] = {0, 0, 0};
+    int dims[3] = {0, 0, 0};  // as the kernels see the field: dims[dir] is the line length
+    int layout = MC3D_XPENCIL;  // caller's axis order (a rotation of x, y, z)
     int nbatch = 1;         // members stacked along z
     void *dmem = nullptr;
     int device = 0;
@@ -54,6 +55,23 @@ int upload(mc3d_lineop *op) {
     d.clen = static_cast<const int *>(put(p.clen.data(), sizeof(int) * p.P));
     if (!d.rows || !d.drows || !d.lu || !d.sp || !d.G || !d.cstart || !d.clen)
         return mc3d::fail(MC3D_EHIP, "upload of operator tables failed");
+    return MC3D_OK;
+}
+
+// The caller's field is x-fastest over the axes rotated by `layout`
+// (MC3D_XPENCIL: x, y, z; MC3D_YPENCIL: y, z, x; MC3D_ZPENCIL: z, x, y from
+// fastest to slowest, src/Macros.hpp:16-18).  The kernels see it as an
+// x-fastest field of dims vd, in which physical direction dir is direction
+// vdir and the boundary types follow the same rotation.
+int rotate(int layout, int dir, const int dims[3], const int bc3[3], int vd[3], int vb[3], int *vdir) {
+    if (layout < MC3D_XPENCIL || layout > MC3D_ZPENCIL) return mc3d::fail(MC3D_EINVAL, "unknown layout");
+    if (dir < 0 || dir > 2) return mc3d::fail(MC3D_EINVAL, "dir must be 0, 1 or 2");
+    if (!dims) return mc3d::fail(MC3D_EINVAL, "null dims");
+    for (int m = 0; m < 3; ++m) {
+        vd[m] = dims[(m + layout) % 3];
+        if (vb) vb[m] = bc3[(m + layout) % 3];
+    }
+    *vdir = (dir - layout + 3) % 3;
     return MC3D_OK;
 }
 
@@ -206,9 +224,15 @@ int apply(const mc3d_lineop *op, const double *const *in, double *const *out, in
 
 extern "C" {
 
-int mc3d_deriv_create(int scheme, int dir, int bc, const int dims[3], double dd, mc3d_lineop **out) {
+int mc3d_deriv_create(int scheme, int dir, int bc, const int dims[3], double dd, int layout,
+                      mc3d_lineop **out) {
     if (scheme < MC3D_CD2 || scheme > MC3D_PENTA10) return mc3d::fail(MC3D_EINVAL, "unknown derivative scheme");
-    return create(mc3d::kDeriv, scheme, dir, bc, nullptr, dims, dd, 0.0, 1, mc3d::LineTune{}, out);
+    int vd[3], vdir = 0;
+    int rc = rotate(layout, dir, dims, nullptr, vd, nullptr, &vdir);
+    if (rc) return rc;
+    rc = create(mc3d::kDeriv, scheme, vdir, bc, nullptr, vd, dd, 0.0, 1, mc3d::LineTune{}, out);
+    if (!rc) (*out)->layout = layout;
+    return rc;
 }
 
 int mc3d_deriv_apply(const mc3d_lineop *op, const double *const *in, double *const *out, int nfields,
@@ -233,12 +257,16 @@ int mc3d_deriv_apply_tpb(const mc3d_lineop *op, const double *in, double *out,
     return apply(op->tpb ? op->tpb : op, &in, &out, 1, mc3d::kModeTPB, wrap, 0, stream);
 }
 
-int mc3d_filter_create(int kind, double alphaF, int dir, const int bc3[3], const int dims[3],
+int mc3d_filter_create(int kind, double alphaF, int dir, const int bc3[3], const int dims[3], int layout,
                        mc3d_lineop **out) {
     if (kind < MC3D_COMPACT8 || kind > MC3D_COMMN5EXP) return mc3d::fail(MC3D_EINVAL, "unknown filter kind");
     if (!bc3) return mc3d::fail(MC3D_EINVAL, "null bc3");
-    if (dir < 0 || dir > 2) return mc3d::fail(MC3D_EINVAL, "dir must be 0, 1 or 2");
-    return create(mc3d::kFilter, kind, dir, bc3[dir], bc3, dims, 1.0, alphaF, 1, mc3d::LineTune{}, out);
+    int vd[3], vb[3], vdir = 0;
+    int rc = rotate(layout, dir, dims, bc3, vd, vb, &vdir);
+    if (rc) return rc;
+    rc = create(mc3d::kFilter, kind, vdir, vb[vdir], vb, vd, 1.0, alphaF, 1, mc3d::LineTune{}, out);
+    if (!rc) (*out)->layout = layout;
+    return rc;
 }
 
 int mc3d_filter_apply(const mc3d_lineop *op, const double *const *in, double *const *out, int nfields,

@@ -460,8 +460,17 @@ struct GradIn {
     Metrics m;
 };
 
-// d u_i / d x_j from d u_i / d xi_k, src/CurvilinearCSolver_Core.cpp:548-561
+// d u_i / d x_j from d u_i / d xi_k, src/CurvilinearCSolver_Core.cpp:548-561;
+// without metric terms (m.J11 null) the inputs are the Cartesian gradient itself
 __device__ __forceinline__ void cart_grad(const GradIn &a, long p, double A[3][3]) {
+    if (!a.m.J11) {
+        const double *const *d[3] = {a.dU, a.dV, a.dW};
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) A[i][j] = d[i][j][p];
+        return;
+    }
     const double J11 = a.m.J11[p], J12 = a.m.J12[p], J13 = a.m.J13[p];
     const double J21 = a.m.J21[p], J22 = a.m.J22[p], J23 = a.m.J23[p];
     const double J31 = a.m.J31[p], J32 = a.m.J32[p], J33 = a.m.J33[p];

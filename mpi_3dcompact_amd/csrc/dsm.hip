@@ -18,9 +18,14 @@ constexpr double kTf2 = 2.0 * 2.0;  // test-filter width ratio squared, src/DSMS
 
 inline dim3 grid_for(long n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 
-// gradU[i][j] = d u_i / d x_j from d u_i / d xi_k, src/CurvilinearCSolver_Core.cpp:548-561
+// gradU[i][j] = d u_i / d x_j from d u_i / d xi_k, src/CurvilinearCSolver_Core.cpp:548-561;
+// without metric terms (m.J11 null) the inputs are the Cartesian gradient itself
 __device__ __forceinline__ void cart(const Metrics &m, long ip, double d0, double d1, double d2,
                                      double &gx, double &gy, double &gz) {
+    if (!m.J11) {
+        gx = d0; gy = d1; gz = d2;
+        return;
+    }
     gx = ldnt(m.J11, ip) * d0 + ldnt(m.J21, ip) * d1 + ldnt(m.J31, ip) * d2;
     gy = ldnt(m.J12, ip) * d0 + ldnt(m.J22, ip) * d1 + ldnt(m.J32, ip) * d2;
     gz = ldnt(m.J13, ip) * d0 + ldnt(m.J23, ip) * d1 + ldnt(m.J33, ip) * d2;

@@ -197,7 +197,7 @@ LearnedSGS::LearnedSGS(CurvilinearCSolver *c) {
         delta2_ = c->alloc_field();
     }
     if (musgsAvgType == Options::XI1_AVG || musgsAvgType == Options::XI2_AVG || musgsAvgType == Options::XI3_AVG) {
-        line_dir_ = musgsAvgType == Options::XI1_AVG ? 0 : musgsAvgType == Options::XI2_AVG ? 1 : 2;
+        line_dir_ = c->axis_of[musgsAvgType == Options::XI1_AVG ? 0 : musgsAvgType == Options::XI2_AVG ? 1 : 2];
         const Domain &d = c->dom;
         nlines_ = (long)d.gNx * d.gNy * d.gNz / (line_dir_ == 0 ? d.gNx : line_dir_ == 1 ? d.gNy : d.gNz);
         for (auto &f : lines_) f = dev_alloc<double>((size_t)nlines_, "closure line means");
@@ -211,7 +211,7 @@ LearnedSGS::LearnedSGS(CurvilinearCSolver *c) {
                             c->opt.bcZType == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET};
         std::unique_ptr<AbstractFilter> *fv[3] = {&filtX, &filtY, &filtZ};
         for (int d = 0; d < 3; ++d) {
-            const auto dir = (AbstractDerivatives::Direct)d;
+            const auto dir = (AbstractDerivatives::Direct)c->axis_of[d];
             if (c->opt.testFilterType == 1) fv[d]->reset(new CommN5PadeTestFilter(c->dom, bc3, dir, c->st));
             else fv[d]->reset(new CommN5ExpTestFilter(c->dom, bc3, dir, c->st));
             static_cast<LineFilter *>(fv[d]->get())->timer = &c->timer;

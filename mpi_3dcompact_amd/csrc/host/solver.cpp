@@ -134,7 +134,7 @@ void AbstractSGS::doAveraging(const double *phi, double *phiF) {
         case Options::XI1_AVG:
         case Options::XI2_AVG:
         case Options::XI3_AVG: {
-            const int dir = musgsAvgType == Options::XI1_AVG ? 0 : musgsAvgType == Options::XI2_AVG ? 1 : 2;
+            const int dir = cs->axis_of[musgsAvgType == Options::XI1_AVG ? 0 : musgsAvgType == Options::XI2_AVG ? 1 : 2];
             hip_check(launch_xi_average(phi, phiF, dir, d.gNx, d.gNy, d.gNz, cs->st), "line average");
             break;
         }
@@ -153,9 +153,10 @@ VremanSGS::VremanSGS(CurvilinearCSolver *c) {
     const int bc3[3] = {c->opt.bcXType == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET,
                         c->opt.bcYType == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET,
                         c->opt.bcZType == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET};
-    filtX.reset(new Compact10Filter(-0.49, c->dom, bc3, AbstractDerivatives::DIRX, c->st));
-    filtY.reset(new Compact10Filter(-0.49, c->dom, bc3, AbstractDerivatives::DIRY, c->st));
-    filtZ.reset(new Compact10Filter(-0.49, c->dom, bc3, AbstractDerivatives::DIRZ, c->st));
+    auto axis = [c](int d) { return (AbstractDerivatives::Direct)c->axis_of[d]; };
+    filtX.reset(new Compact10Filter(-0.49, c->dom, bc3, axis(0), c->st));
+    filtY.reset(new Compact10Filter(-0.49, c->dom, bc3, axis(1), c->st));
+    filtZ.reset(new Compact10Filter(-0.49, c->dom, bc3, axis(2), c->st));
     for (auto *f : {filtX.get(), filtY.get(), filtZ.get()}) static_cast<LineFilter *>(f)->timer = &c->timer;
     mu_sgs = c->alloc_field();
 }
@@ -183,7 +184,7 @@ DSMSGS::DSMSGS(CurvilinearCSolver *c, bool fallback_only) {
                         c->opt.bcZType == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET};
     std::unique_ptr<AbstractFilter> *fv[3] = {&filtX, &filtY, &filtZ};
     for (int d = 0; d < 3; ++d) {
-        auto dir = (AbstractDerivatives::Direct)d;
+        auto dir = (AbstractDerivatives::Direct)c->axis_of[d];
         // src/DSMSGS.hpp:57-67: 1 = Pade-type, 2 = explicit
         if (c->opt.testFilterType == 1) fv[d]->reset(new CommN5PadeTestFilter(c->dom, bc3, dir, c->st));
         else fv[d]->reset(new CommN5ExpTestFilter(c->dom, bc3, dir, c->st));
@@ -424,16 +425,7 @@ CurvilinearCSolver::CurvilinearCSolver(const Options &o, hipStream_t s, int memb
             throw Error(MC3D_ECONFIG, "stacked members keep no running statistics");
     }
     timeStep = opt.timeStep;
-    // src/IdealGas.hpp:35-48
-    ig.gamma = 1.4;
-    ig.Pr = 0.7;
-    const double p_ref = 1.0 / ig.gamma, rho_ref = 1.0;
-    ig.T_ref = 1.0;
-    ig.mu_ref = opt.mu_ref;
-    ig.R_gas = p_ref / rho_ref / ig.T_ref;
-    ig.cp = ig.R_gas * ig.gamma / (ig.gamma - 1.0);
-    ig.mu_member = nullptr;
-    ig.member_pts = 0;
+    initGas();
     // field slab
     const int nfields = 10 + 20 + 10 + 12 + 15 + 5 + 10 + 8;
     slab_bytes_ = sizeof(double) * (size_t)N * nfields + 4096;
@@ -505,6 +497,59 @@ CurvilinearCSolver::CurvilinearCSolver(const Options &o, hipStream_t s, int memb
         case Options::KENRK4: rk.reset(new KenRK4(this)); break;
         default: rk.reset(new LSLDDRK4(this)); break;
     }
+}
+
+// src/IdealGas.hpp:35-48
+void CurvilinearCSolver::initGas() {
+    ig.gamma = 1.4;
+    ig.Pr = 0.7;
+    const double p_ref = 1.0 / ig.gamma, rho_ref = 1.0;
+    ig.T_ref = 1.0;
+    ig.mu_ref = opt.mu_ref;
+    ig.R_gas = p_ref / rho_ref / ig.T_ref;
+    ig.cp = ig.R_gas * ig.gamma / (ig.gamma - 1.0);
+    ig.mu_member = nullptr;
+    ig.member_pts = 0;
+}
+
+namespace {
+// the options with the grid and boundary types in the axis order of a layout
+// (MC3D_XPENCIL: x, y, z; YPENCIL: y, z, x; ZPENCIL: z, x, y)
+Options rotated(const Options &o, int r) {
+    if (r < MC3D_XPENCIL || r > MC3D_ZPENCIL) throw Error(MC3D_EINVAL, "unknown layout");
+    Options v = o;
+    const int n[3] = {o.Nx, o.Ny, o.Nz};
+    const Options::BCType b[3] = {o.bcXType, o.bcYType, o.bcZType};
+    v.Nx = n[r % 3]; v.Ny = n[(1 + r) % 3]; v.Nz = n[(2 + r) % 3];
+    v.bcXType = b[r % 3]; v.bcYType = b[(1 + r) % 3]; v.bcZType = b[(2 + r) % 3];
+    return v;
+}
+}  // namespace
+
+CurvilinearCSolver::CurvilinearCSolver(const Options &o, hipStream_t s, SgsOnly t)
+    : opt(rotated(o, t.layout)), dom(opt), st(s), timer(s) {
+    if (opt.lesModel != Options::VREMAN && opt.lesModel != Options::DSM && opt.lesModel != Options::LEARNED)
+        throw Error(MC3D_ECONFIG, "an SGS handle needs LESMODEL VREMAN, DSM or LEARNED");
+    if (!t.J) throw Error(MC3D_EINVAL, "null Jacobian field");
+    N = dom.gN;
+    timeStep = opt.timeStep;
+    // the filter width (dxi deta dzeta / J)^(1/3) multiplies the spacings in x, y, z order
+    const Domain phys(o);
+    dom.dx = phys.dx; dom.dy = phys.dy; dom.dz = phys.dz;
+    for (int d = 0; d < 3; ++d) axis_of[d] = (d - t.layout + 3) % 3;
+    initGas();
+    slab_bytes_ = sizeof(double) * (size_t)N + 4096;
+    hip_check(hipMalloc(&slab_, slab_bytes_), "field allocation");
+    tmp[9] = alloc_field();
+    msh = Metrics{t.J, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    void *p = nullptr;
+    hip_check(hipMalloc(&p, sizeof(double) * std::max<size_t>(4096, (size_t)cfl_blocks(dom.gN))),
+              "scratch allocation");
+    extra_.push_back(p);
+    cfl_partial = static_cast<double *>(p);
+    if (opt.lesModel == Options::VREMAN) les.reset(new VremanSGS(this));
+    else if (opt.lesModel == Options::DSM) les.reset(new DSMSGS(this));
+    else les.reset(new LearnedSGS(this));
 }
 
 CurvilinearCSolver::~CurvilinearCSolver() {

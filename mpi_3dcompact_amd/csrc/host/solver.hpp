@@ -286,7 +286,20 @@ class CurvilinearCSolver {
     // its own MU_REF (periodic boxes, no LES model or Vreman with NONE / LOCAL
     // averaging)
     CurvilinearCSolver(const Options &opt, hipStream_t st, int members = 1, const double *mu_members = nullptr);
+    // The SGS model of the options alone, over caller-owned fields (the
+    // mc3d_sgs handle): no flow derivatives, filters, RK or field slab beyond
+    // the model's own work fields.  The fields are in `layout` (MC3D_XPENCIL,
+    // YPENCIL or ZPENCIL): the domain takes the layout's axis order, axis_of
+    // maps each physical direction to its axis, and the Cartesian gradients
+    // are given (no metric terms: msh holds the caller's J alone).
+    struct SgsOnly {
+        int layout;
+        const double *J;
+    };
+    CurvilinearCSolver(const Options &opt, hipStream_t st, SgsOnly t);
     ~CurvilinearCSolver();
+    // axis of the fields along physical direction d (x, y, z)
+    int axis_of[3] = {0, 1, 2};
 
     Options opt;
     Domain dom;
@@ -420,6 +433,7 @@ class CurvilinearCSolver {
     void hitStats(double out[11]);
 
   private:
+    void initGas();
     std::vector<double> mu_members_;  // per-member MU_REF when members > 1
     std::chrono::steady_clock::time_point last_check_ = std::chrono::steady_clock::now();
     bool dumps_ = false;

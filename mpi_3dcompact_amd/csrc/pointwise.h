@@ -107,6 +107,9 @@ hipError_t launch_cfl(Prim pr, Metrics m, double dx, double dy, double dz, doubl
                       long n, int nmember, hipStream_t st);
 hipError_t launch_cfl_finish(const double *partial, int nblocks, StepScalars *sc, int nmember, int const_cfl,
                              double cfl_target, int first_step, double time0, hipStream_t st);
+// the same maxima into out[2 b] = cfl / maxspeed, out[2 b + 1] = maxspeed of each member
+hipError_t launch_cfl_out(const double *partial, int nblocks, int nmember, double cfl, double *out,
+                          hipStream_t st);
 hipError_t launch_flux(const FluxArgs &a, Gas g, long n, hipStream_t st);
 hipError_t launch_rhs_rk(const RkArgs &a, long n, hipStream_t st);
 hipError_t launch_vreman(const double *const dU[3], const double *const dV[3],

@@ -128,6 +128,24 @@ __global__ __launch_bounds__(kBlock) void k_cfl_finish(const double *partial, in
     }
 }
 
+// one workgroup per member: out[2 b] = dt = cfl / maxspeed, out[2 b + 1] = maxspeed
+__global__ __launch_bounds__(kBlock) void k_cfl_out(const double *partial, int nb, double cfl, double *out) {
+    __shared__ double red[kBlock];
+    partial += (long)blockIdx.x * nb;
+    double v = -100000.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) v = fmax(v, partial[i]);
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * blockIdx.x] = cfl / red[0];
+        out[2 * blockIdx.x + 1] = red[0];
+    }
+}
+
 // Stress, heat flux and contravariant fluxes / J, src/CurvilinearCSolver_Core.cpp:548-561,615-752
 __global__ __launch_bounds__(kBlock) void k_flux(FluxArgs a, Gas g, long n) {
     const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -310,21 +328,28 @@ __global__ __launch_bounds__(kBlock) void k_vreman(const double *dU0, const doub
                                                    double dx, double dy, double dz, long n) {
     const long ip = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (ip >= n) return;
-    const double J11 = ldnt(m.J11, ip), J12 = ldnt(m.J12, ip), J13 = ldnt(m.J13, ip);
-    const double J21 = ldnt(m.J21, ip), J22 = ldnt(m.J22, ip), J23 = ldnt(m.J23, ip);
-    const double J31 = ldnt(m.J31, ip), J32 = ldnt(m.J32, ip), J33 = ldnt(m.J33, ip);
     const double u1 = ldnt(dU0, ip), u2 = ldnt(dU1, ip), u3 = ldnt(dU2, ip);
     const double v1 = ldnt(dV0, ip), v2 = ldnt(dV1, ip), v3 = ldnt(dV2, ip);
     const double w1 = ldnt(dW0, ip), w2 = ldnt(dW1, ip), w3 = ldnt(dW2, ip);
-    const double d1v1 = J11 * u1 + J21 * u2 + J31 * u3;
-    const double d2v1 = J12 * u1 + J22 * u2 + J32 * u3;
-    const double d3v1 = J13 * u1 + J23 * u2 + J33 * u3;
-    const double d1v2 = J11 * v1 + J21 * v2 + J31 * v3;
-    const double d2v2 = J12 * v1 + J22 * v2 + J32 * v3;
-    const double d3v2 = J13 * v1 + J23 * v2 + J33 * v3;
-    const double d1v3 = J11 * w1 + J21 * w2 + J31 * w3;
-    const double d2v3 = J12 * w1 + J22 * w2 + J32 * w3;
-    const double d3v3 = J13 * w1 + J23 * w2 + J33 * w3;
+    double d1v1, d2v1, d3v1, d1v2, d2v2, d3v2, d1v3, d2v3, d3v3;
+    if (m.J11) {  // gradU = J^T d(u)/d(xi), src/CurvilinearCSolver_Core.cpp:548-561
+        const double J11 = ldnt(m.J11, ip), J12 = ldnt(m.J12, ip), J13 = ldnt(m.J13, ip);
+        const double J21 = ldnt(m.J21, ip), J22 = ldnt(m.J22, ip), J23 = ldnt(m.J23, ip);
+        const double J31 = ldnt(m.J31, ip), J32 = ldnt(m.J32, ip), J33 = ldnt(m.J33, ip);
+        d1v1 = J11 * u1 + J21 * u2 + J31 * u3;
+        d2v1 = J12 * u1 + J22 * u2 + J32 * u3;
+        d3v1 = J13 * u1 + J23 * u2 + J33 * u3;
+        d1v2 = J11 * v1 + J21 * v2 + J31 * v3;
+        d2v2 = J12 * v1 + J22 * v2 + J32 * v3;
+        d3v2 = J13 * v1 + J23 * v2 + J33 * v3;
+        d1v3 = J11 * w1 + J21 * w2 + J31 * w3;
+        d2v3 = J12 * w1 + J22 * w2 + J32 * w3;
+        d3v3 = J13 * w1 + J23 * w2 + J33 * w3;
+    } else {  // the inputs are the Cartesian gradient, gradU[i][j] = d u_i / d x_j
+        d1v1 = u1; d2v1 = u2; d3v1 = u3;
+        d1v2 = v1; d2v2 = v2; d3v2 = v3;
+        d1v3 = w1; d2v3 = w2; d3v3 = w3;
+    }
     const double vol = dx * dy * dz / ldnt(m.J, ip);
     const double del = pow(vol, 1.0 / 3.0);
     const double d1 = del * del, d2 = d1, d3 = d1;
@@ -518,6 +543,13 @@ hipError_t launch_cfl_finish(const double *partial, int nblocks, StepScalars *sc
     if (nmember < 1) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_cfl_finish, dim3(nmember), dim3(kBlock), 0, st, partial, nblocks, sc, const_cfl,
                        cfl_target, first_step, time0);
+    return hipGetLastError();
+}
+
+hipError_t launch_cfl_out(const double *partial, int nblocks, int nmember, double cfl, double *out,
+                          hipStream_t st) {
+    if (nmember < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_cfl_out, dim3(nmember), dim3(kBlock), 0, st, partial, nblocks, cfl, out);
     return hipGetLastError();
 }
 

@@ -1,13 +1,16 @@
 """Python handles over the derivative / filter operators of libmc3d.so.
 
-Fields are torch tensors on the GPU, float64, shape (Nz, Ny, Nx) contiguous
-(x fastest), passed to the C ABI as raw device pointers.
+Fields are torch tensors on the GPU, float64, contiguous, passed to the C ABI
+as raw device pointers.  In the default layout (MC3D_XPENCIL) a field has
+shape (Nz, Ny, Nx), x fastest; in the reference's Y pencil (MC3D_YPENCIL)
+shape (Nx, Nz, Ny), y fastest; in its Z pencil (MC3D_ZPENCIL) shape
+(Ny, Nx, Nz), z fastest.
 """
 import ctypes
 
 import numpy as np
 
-from ._lib import (MC3D_PERIODIC, check, lib)
+from ._lib import (MC3D_PERIODIC, MC3D_XPENCIL, check, lib)
 
 _VP = ctypes.c_void_p
 
@@ -33,26 +36,33 @@ def _stream_ptr(stream):
 class LineOp:
     """A derivative (``kind='deriv'``) or filter (``kind='filter'``) along one direction."""
 
-    def __init__(self, handle, cls, dims, direction):
+    def __init__(self, handle, cls, dims, direction, layout=MC3D_XPENCIL):
         self._h = handle
         self.cls = cls
         self.dims = tuple(dims)
         self.direction = direction
+        self.layout = layout
 
     @classmethod
-    def deriv(cls, scheme, direction, bc, dims, dd):
+    def deriv(cls, scheme, direction, bc, dims, dd, layout=MC3D_XPENCIL):
         h = _VP()
         d = (ctypes.c_int * 3)(*dims)
-        check(lib().mc3d_deriv_create(scheme, direction, bc, d, dd, ctypes.byref(h)))
-        return cls(h, "deriv", dims, direction)
+        check(lib().mc3d_deriv_create(scheme, direction, bc, d, dd, layout, ctypes.byref(h)))
+        return cls(h, "deriv", dims, direction, layout)
 
     @classmethod
-    def filter(cls, kind, alpha_f, direction, bc3, dims):
+    def filter(cls, kind, alpha_f, direction, bc3, dims, layout=MC3D_XPENCIL):
         h = _VP()
         d = (ctypes.c_int * 3)(*dims)
         b = (ctypes.c_int * 3)(*bc3)
-        check(lib().mc3d_filter_create(kind, alpha_f, direction, b, d, ctypes.byref(h)))
-        return cls(h, "filter", dims, direction)
+        check(lib().mc3d_filter_create(kind, alpha_f, direction, b, d, layout, ctypes.byref(h)))
+        return cls(h, "filter", dims, direction, layout)
+
+    def shape(self):
+        """Tensor shape of a field in this operator's layout (slowest axis first)."""
+        r = self.layout
+        fast = [self.dims[(m + r) % 3] for m in range(3)]
+        return (fast[2], fast[1], fast[0])
 
     def info(self):
         p, m, l = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
@@ -69,7 +79,7 @@ class LineOp:
         n = len(ins)
         if len(outs) != n:
             raise ValueError("ins and outs differ in length")
-        shape = (self.dims[2], self.dims[1], self.dims[0])
+        shape = self.shape()
         for t in ins + outs:
             if tuple(t.shape) != shape:
                 raise ValueError(f"field shape {tuple(t.shape)} != {shape}")
