@@ -15,8 +15,10 @@ bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on this
 workload (profiles/les256_traffic.json, counters scaled by their reading for
 a copy of known size), when the profiled kernel is the dominant one.
 cpu_baseline = the CPU oracle (a plain C++ restatement of the reference
-algorithm, OpenMP over lines) timed on this host for one step of the same
-configuration (256^3 by default).
+algorithm, OpenMP over lines) timed on this host: 1 warm-up and 3 timed steps
+of the same configuration (256^3 by default) on the threads OpenMP gives it,
+the thread count read back from the runtime, the host's core counts and CPU
+model, and a 1-thread rate at 128^3.
 effective_bandwidth = the design-independent dataflow of the reference for
 this configuration (9592 B per point-step: every reference operator's distinct
 full-size FP64 inputs and outputs, KenRK4 + Vreman LOCAL) times the step rate.
@@ -25,9 +27,12 @@ ensemble = a second workload of 64 HIT boxes of uneven sizes advanced together
 shapes): its point-step rate, the same with direct launches and with default
 shapes, the members run one after another as standalone solvers, and a
 parity sample of members against standalone runs.
-learned_closure = the learned-vs-DSM half of the metric as the committed
-closure studies measured it (speedup and KE / dissipation errors at 256^3 and
-64^3), not re-run here.
+learned_closure = the learned-vs-DSM half of the metric: "timed" runs the
+DSM (LESAVERAGING LOCAL, mu 1e-4) and the learned closure (the committed
+network trained on that DSM's pairs) from one 256^3 initial field here and
+reports the step-time ratio; "studies" holds the errors the committed closure
+studies measured over their online runs (256^3 LOCAL and XI1 averaging, 64^3),
+not re-run here.
 """
 import argparse
 import json
@@ -67,41 +72,130 @@ def measured_traffic(kernel, n):
     return t.get("traffic_bytes_per_launch")
 
 
+# the learned-closure workload: DSM LES (PENTA10, COMPACT8, KENRK4, Pade test
+# filter, tau_kk on, LESAVERAGING LOCAL) against the learned closure with the
+# same averaging and its DSM fallback, mu_ref 1e-4, M_t 0.3, from one initial field
+CLOSURE_NET = os.path.join(ROOT, "mpi_3dcompact_amd", "networks", "hit256_local.net")
+
+
 def learned_closure_study():
-    """The learned-closure half of the metric, from the committed studies
-    (python -m mpi_3dcompact_amd.closure_study --N <n> --avg XI1_AVG: training and
-    1000-step runs take minutes, so they are not repeated here)."""
+    """Errors of the learned closure against the DSM from the committed closure
+    studies (python -m mpi_3dcompact_amd.closure_study; training on 2e7 DSM
+    pairs and the online runs take minutes, so they are not repeated here),
+    each labelled with its LESAVERAGING and the steps compared."""
     out = {}
-    for n in (256, 64):
-        path = os.path.join(ROOT, "profiles", f"closure_study_{n}_xi1.json")
+    for tag in ("256_local", "256_xi1", "64_xi1"):
+        path = os.path.join(ROOT, "profiles", f"closure_study_{tag}.json")
         try:
             with open(path) as f:
                 d = json.load(f)
         except (OSError, ValueError):
             continue
-        out[f"{n}^3"] = {"speedup_vs_dsm": round(d["speedup_vs_dsm"], 3),
-                          "ms_per_step": {k: round(v, 3) for k, v in d["ms_per_step"].items()},
-                          "ke_max_rel": d["learned_vs_dsm"]["ke_max_rel"],
-                          "dissipation_max_rel": d["learned_vs_dsm"]["dissipation_max_rel"],
-                          "constant_coefficients": d["constant_vs_dsm"],
-                          "source": f"profiles/closure_study_{n}_xi1.json"}
+        out[tag] = {"averaging": d.get("avg", "XI1_AVG"), "online_steps": d["online_steps"],
+                    "compared_steps": d.get("compared_steps"), "blowup_step": d.get("blowup_step"),
+                    "speedup_vs_dsm": round(d["speedup_vs_dsm"], 3),
+                    "learned_vs_dsm": d["learned_vs_dsm"], "constant_vs_dsm": d.get("constant_vs_dsm"),
+                    "nomodel_vs_dsm": d.get("nomodel_vs_dsm"), "heldout_mse": d.get("heldout_mse"),
+                    "source": f"profiles/closure_study_{tag}.json"}
     return out or None
 
 
-def cpu_baseline(n_cpu):
-    """Oracle step rate on the host (test infrastructure, used only as the checker/baseline)."""
+def closure_leg(N, steps, warmup, net_path=CLOSURE_NET):
+    """Step time of the DSM and of the learned closure on the same N^3 initial
+    field (each its own solver, `warmup` untimed steps, then `steps` timed with
+    the stream synchronised around them), the resolved KE and dissipation of
+    both after the timed steps, and the closure's fallback counters."""
+    from mpi_3dcompact_amd import hit
+    from mpi_3dcompact_amd.closure_study import case
+    from mpi_3dcompact_amd.solver import Solver
+    if not os.path.exists(net_path):
+        raise SystemExit(f"closure network {net_path} is missing")
+    ic = hit.hit_initial_condition(N, k0=4.0, Mt=0.3, seed=0)
+    met = hit.identity_metrics(N)
+    res = {}
+    for model in ("DSM", "LEARNED"):
+        log(f"[bench] closure leg: {model} at {N}^3")
+        g = Solver(case(N, model, 1e-4, "LOCAL", closure_file=net_path if model == "LEARNED" else None),
+                   (N, N, N))
+        g.set_metrics(met)
+        g.set_initial(ic)
+        g.step(warmup)
+        g.sync()
+        t0 = time.perf_counter()
+        g.step(steps)
+        g.sync()
+        wall = time.perf_counter() - t0
+        st = g.hit_stats()
+        res[model] = {"ms_per_step": wall * 1e3 / steps, "ke": 0.5 * st["rho_uu"], "dissipation": st["dissipation"]}
+        if model == "LEARNED":
+            res[model]["closure"] = g.closure_stats()
+        g.close()
+    d, l = res["DSM"], res["LEARNED"]
+    return {"workload": f"decaying HIT {N}^3, PENTA10 + COMPACT8 (alpha_f 0.49) + KENRK4 (CFL 0.5), mu 1e-4, "
+                        "M_t 0.3; DSM (Pade test filter, tau_kk, LESAVERAGING LOCAL) vs the learned closure "
+                        "(same averaging, DSM fallback on), network "
+                        + os.path.relpath(net_path, ROOT),
+            "steps": steps, "warmup": warmup,
+            "ms_per_step": {"DSM": round(d["ms_per_step"], 3), "LEARNED": round(l["ms_per_step"], 3)},
+            "speedup_vs_dsm": round(d["ms_per_step"] / l["ms_per_step"], 3),
+            "ke_rel_diff": abs(l["ke"] / d["ke"] - 1.0),
+            "dissipation_rel_diff": abs(l["dissipation"] / d["dissipation"] - 1.0),
+            "fallback": l["closure"]}
+
+
+def host_cpu():
+    """Cores this process may use and the CPU model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "model": model}
+
+
+def cpu_baseline(n_cpu, steps=3, n_one=128):
+    """Oracle step rate on the host (test infrastructure, used only as the
+    checker/baseline): 1 warm-up and `steps` timed steps at n_cpu^3 on the
+    threads OpenMP gives the oracle, then one step at n_one^3 on one thread."""
     from mpi_3dcompact_amd import hit
     from oracle import pyoracle as P
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    threads = P.max_threads()
     ic = hit.hit_initial_condition(n_cpu, k0=4.0, Mt=0.3, seed=0)
     s = P.Solver(hit.oracle_config(n_cpu, **CASE), hit.identity_metrics(n_cpu), ic)
+    del ic
     t0 = time.perf_counter()
     s.steps(1)
-    dt = time.perf_counter() - t0
+    warm = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    s.steps(steps)
+    dt = (time.perf_counter() - t0) / steps
+    del s
+    log(f"[bench] CPU oracle {n_cpu}^3 on {threads} threads: warm-up {warm:.1f} s, {dt:.2f} s per step")
+    one = None
+    if n_one:
+        ic = hit.hit_initial_condition(n_one, k0=4.0, Mt=0.3, seed=0)
+        s1 = P.Solver(hit.oracle_config(n_one, **CASE), hit.identity_metrics(n_one), ic)
+        P.set_threads(1)
+        try:
+            s1.steps(1)  # one step alone: the whole step is the sample
+            t0 = time.perf_counter()
+            s1.steps(1)
+            dt1 = time.perf_counter() - t0
+        finally:
+            P.set_threads(threads)
+        one = {"value": n_one ** 3 / dt1 / 1e6, "unit": "Mpt-steps/s", "cores": 1,
+               "sample": f"1 step at {n_one}^3 after 1 warm-up step ({dt1:.1f} s)"}
+        log(f"[bench] CPU oracle {n_one}^3 on 1 thread: {dt1:.2f} s per step")
     return {"value": n_cpu ** 3 / dt / 1e6, "unit": "Mpt-steps/s", "cores": threads,
             "kind": "port",
-            "sample": f"1 KENRK4 step of the same PENTA10/COMPACT8/VREMAN case at {n_cpu}^3 "
-                      f"({dt:.1f} s), OpenMP over independent lines"}
+            "sample": f"{steps} KENRK4 steps after 1 warm-up step of the same PENTA10/COMPACT8/VREMAN LOCAL "
+                      f"case at {n_cpu}^3 ({dt:.2f} s per step), OpenMP over independent lines on "
+                      f"{threads} threads (omp_get_max_threads)",
+            "host": host_cpu(), "one_core": one}
 
 
 # ensemble workload: 64 decaying-HIT boxes, N drawn from {32, 48, 64, 96, 128},
@@ -232,7 +326,12 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="grid points per direction")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=3, help="timed CPU oracle steps after one warm-up")
+    ap.add_argument("--cpu-one-core-n", type=int, default=128, help="grid of the 1-thread CPU sample (0: skip)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble workload")
+    ap.add_argument("--no-closure", action="store_true", help="skip the learned-closure timing leg")
+    ap.add_argument("--closure-only", action="store_true", help="run the learned-closure timing leg alone")
+    ap.add_argument("--closure-steps", type=int, default=5)
     ap.add_argument("--ensemble-only", action="store_true", help="run the ensemble workload alone")
     ap.add_argument("--ensemble-steps", type=int, default=20)
     ap.add_argument("--ensemble-cache", default=None, help="line-operator tuning cache file")
@@ -248,6 +347,9 @@ def main():
     from mpi_3dcompact_amd import hit
     from mpi_3dcompact_amd.solver import Solver
 
+    if args.closure_only:
+        print(json.dumps({"learned_closure": closure_leg(args.n, args.closure_steps, 2)}), flush=True)
+        return
     if args.ensemble_only:
         print(json.dumps({"ensemble": ensemble_leg(args.ensemble_steps, 4, cache=args.ensemble_cache,
                                                    step_only=args.ensemble_step_only)}), flush=True)
@@ -292,7 +394,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         log(f"[bench] CPU oracle baseline at {args.cpu_n}^3")
-        cpu = cpu_baseline(args.cpu_n)
+        cpu = cpu_baseline(args.cpu_n, args.cpu_steps, args.cpu_one_core_n)
 
     out = {
         "metric": METRIC,
@@ -323,7 +425,9 @@ def main():
     }
     if cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 1)
-    out["learned_closure"] = learned_closure_study()
+    lc = {} if args.no_closure else {"timed": closure_leg(N, args.closure_steps, 2)}
+    lc["studies"] = learned_closure_study()
+    out["learned_closure"] = lc
     if not args.no_ensemble:
         out["ensemble"] = ensemble_leg(args.ensemble_steps, 4, cache=args.ensemble_cache)
     print(json.dumps(out), flush=True)

@@ -196,6 +196,17 @@ int mc3d_sgs_closure_stats(mc3d_sgs *s, long *evaluations, long *outside_points,
 typedef struct mc3d_solver mc3d_solver;
 
 int mc3d_solver_create(const char *options_text, mc3d_solver **out);
+/* Parse and validate an options text as mc3d_solver_create does, without a
+ * device (host only): the reference's Options(string) constructor
+ * (src/Options.hpp:131-426) and bcValidation (src/Options.cpp:178-268),
+ * with MC3D_ECONFIG and the message in mc3d_last_error() where the reference
+ * prints and calls MPI_Abort.  On success the parsed values are written to
+ * buf as "SECTION.KEY = value" lines (enums by name, booleans TRUE / FALSE,
+ * numbers with 17 significant digits), truncated to buflen - 1 characters;
+ * *needed (if not NULL) receives the full length plus one.  With a restart
+ * (FROMRESTART / ONLYGRIDFROMRESTART) the grid is read from the dump only by
+ * mc3d_solver_create. */
+int mc3d_options_check(const char *options_text, char *buf, int buflen, int *needed);
 /* metrics[10] = {J, J11, J12, J13, J21, J22, J23, J31, J32, J33} (host arrays),
  * the fields produced by src/AbstractSingleBlockMesh.cpp:23-1439. */
 int mc3d_solver_set_metrics(mc3d_solver *s, const double *const metrics[10]);
@@ -296,6 +307,13 @@ int mc3d_image_write(const char *path, const double *plane, int n1, int n2, int 
  * u', Kolmogorov length, rho', dilatation', vorticity', <rho u.u>}.
  * Overwrites the velocity-derivative work fields.  Synchronous. */
 int mc3d_solver_hit_stats(mc3d_solver *s, double stats[11]);
+/* The HIT driver's fullStepTemporalHook
+ * (tests/hit/Samtaney_D4/MPI_3DCompact.cpp:372-557): after every full step
+ * whose number is a multiple of `every` (the driver's 25), append one line of
+ * the 11 statistics above to `path` (the driver's turbdata.out), each written
+ * with 17 significant digits and followed by a space, as the driver's
+ * ofstream with precision(17) writes them.  NULL or "" switches it off. */
+int mc3d_solver_set_turbdata(mc3d_solver *s, const char *path, int every);
 int mc3d_solver_field_device(mc3d_solver *s, const char *name, double **dev_ptr);
 /* Running statistics ([STATS] VELOCITYSTATS / THERMOSTATS, src/Stats.hpp):
  * their accumulated weights (the time they cover); the fields are read with

@@ -107,7 +107,7 @@ def online(args, model, net, torch):
     g.set_initial(hit.hit_initial_condition(N, k0=args.k0, Mt=args.Mt, seed=args.eval_seed))
     if net is not None:
         g.set_closure(net)
-    hist, steps, t_run = [], 0, 0.0
+    hist, steps, t_run, blowup = [], 0, 0.0, None
     st = g.hit_stats()
     hist.append((0, st["time"], 0.5 * st["rho_uu"], st["dissipation"]))
     while steps < args.online_steps:
@@ -120,14 +120,18 @@ def online(args, model, net, torch):
         steps += n
         st = g.hit_stats()
         if not all(np.isfinite(v) for v in st.values()):
-            raise RuntimeError(f"{model}: non-finite statistics at step {steps}")
+            if not args.allow_blowup:
+                raise RuntimeError(f"{model}: non-finite statistics at step {steps}")
+            _say(f"[closure] {model}: non-finite statistics at step {steps}; the run ends there")
+            blowup = steps
+            break
         hist.append((steps, st["time"], 0.5 * st["rho_uu"], st["dissipation"]))
         if steps % (10 * args.stats_every) == 0 or steps == args.online_steps:
             _say(f"[closure] {model}{'' if net is None else '(' + net_label(net) + ')'} step {steps}: "
                  f"KE {hist[-1][2]:.6e} eps {hist[-1][3]:.6e} {1e3 * t_run / steps:.2f} ms/step")
     extra = g.closure_stats() if model == "LEARNED" else None
     del g
-    return dict(hist=hist, ms_per_step=1e3 * t_run / steps, closure=extra)
+    return dict(hist=hist, ms_per_step=1e3 * t_run / steps, closure=extra, blowup_step=blowup)
 
 
 def net_label(net):
@@ -135,7 +139,10 @@ def net_label(net):
 
 
 def errors(run, ref):
-    h, r = np.array(run["hist"]), np.array(ref["hist"])
+    """Largest relative KE and dissipation differences over the steps both runs
+    reached with finite statistics."""
+    n = min(len(run["hist"]), len(ref["hist"]))
+    h, r = np.array(run["hist"][:n]), np.array(ref["hist"][:n])
     ke = np.max(np.abs(h[:, 2] / r[:, 2] - 1.0))
     eps = np.max(np.abs(h[:, 3] / r[:, 3] - 1.0))
     return float(ke), float(eps)
@@ -162,6 +169,9 @@ def main(argv=None):
     ap.add_argument("--stats-every", type=int, default=25)
     ap.add_argument("--skip-baselines", action="store_true", help="online DSM and learned runs only")
     ap.add_argument("--out", default=None)
+    ap.add_argument("--allow-blowup", action="store_true", help="end an online run at its first non-finite "
+                    "statistics instead of failing; errors are then taken over the steps both runs reached")
+    ap.add_argument("--save-net", default=None, help="write the trained network (mc3d_closure_save) here")
     args = ap.parse_args(argv)
     import torch
 
@@ -188,6 +198,9 @@ def main(argv=None):
          + " ".join(f"{v:.4f}" for v in losses) + f"; held-out MSE {mse} (constant predictor: 1)")
     del Xt, Tt, Xh, Th
     net.label = "learned"
+    if args.save_net:
+        net.save(args.save_net)
+        _say(f"[closure] network written to {args.save_net}")
     const = Closure(0)
     const.set_params(np.zeros(8962, np.float32))
     const.set_scaling(sc["in_mean"], sc["in_std"], sc["out_mean"], sc["out_std"], np.full(7, -1e30),
@@ -198,7 +211,8 @@ def main(argv=None):
     if not args.skip_baselines:
         runs["CONSTANT"] = online(args, "LEARNED", const, torch)
         runs["NOMODEL"] = online(args, "NOMODEL", None, torch)
-    res = dict(N=args.N, samples=n, epochs=args.epochs, batch=args.batch, lr=args.lr, train_seconds=t_tr,
+    res = dict(N=args.N, avg=args.avg, mu=args.mu, Mt=args.Mt, k0=args.k0, gen_steps=args.gen_steps,
+               train_seed=args.train_seed, eval_seed=args.eval_seed, samples=n, epochs=args.epochs, batch=args.batch, lr=args.lr, train_seconds=t_tr,
                epoch_losses=losses, heldout_mse=mse.tolist(), scaling={k: v.tolist() for k, v in sc.items()},
                online_steps=args.online_steps, ms_per_step={k: v["ms_per_step"] for k, v in runs.items()},
                fallback=runs["LEARNED"]["closure"], total_seconds=time.time() - t0)
@@ -207,6 +221,9 @@ def main(argv=None):
             ke, eps = errors(v, runs["DSM"])
             res[f"{k.lower()}_vs_dsm"] = dict(ke_max_rel=ke, dissipation_max_rel=eps)
     res["speedup_vs_dsm"] = runs["DSM"]["ms_per_step"] / runs["LEARNED"]["ms_per_step"]
+    res["blowup_step"] = {k: v["blowup_step"] for k, v in runs.items()}
+    res["compared_steps"] = {k: v["hist"][min(len(v["hist"]), len(runs["DSM"]["hist"])) - 1][0]
+                             for k, v in runs.items() if k != "DSM"}
     res["history"] = {k: v["hist"] for k, v in runs.items()}
     line = json.dumps(res)
     print(line, flush=True)

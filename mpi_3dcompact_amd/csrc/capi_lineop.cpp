@@ -98,21 +98,21 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
     auto *op = new mc3d_lineop();
     std::string err;
     int rc = 0;
-    // 32-row chunks only pay off in the fast kernels; otherwise plan 16-row chunks.
-    // Contiguous (x) lines are staged whole in LDS, which bounds the waves per CU;
-    // 16-row chunks double them, and measured faster for every x operator (the
-    // Penta10 derivative of 256-point lines inside the LES step: 3.3 -> 2.9 ms).
-    int first = tune.mm;
-    if (!first && dir == 0 && dims[0] % 16 == 0) first = 16;
-    const int mms[2] = {first, first ? first : 16};
-    for (int mm : mms) {
+    // Chunk rows to try, in order, until one gives a fast kernel: the tuned
+    // value alone; else for x-lines that split into 16-row chunks 16 first
+    // (contiguous lines are staged whole in LDS, which bounds the waves per
+    // CU; 16-row chunks double them, and measured faster for every x operator:
+    // the Penta10 derivative of 256-point lines inside the LES step 3.3 ->
+    // 2.9 ms), then the plan's own choice (32 rows or an even 8..28-row split),
+    // which also serves lines too long for 16-row chunks in one workgroup.
+    // When none is fast, 16-row chunks for the LDS-tiled kernel.
+    int cand[2] = {tune.mm, -1};
+    if (!tune.mm && dir == 0 && dims[0] % 16 == 0) cand[0] = 16, cand[1] = 0;
+    auto plan_with = [&](int mm) -> int {
         mc3d::PlanRequest rq{cls, kind, bc, dims[dir], dd, alphaF, mm};
         op->plan = mc3d::Plan();
-        rc = mc3d::build_plan(rq, op->plan, err);
-        if (rc) {
-            delete op;
-            return mc3d::fail(rc, err);
-        }
+        int r = mc3d::build_plan(rq, op->plan, err);
+        if (r) return r;
         const mc3d::Plan &p = op->plan;
         mc3d::OpDev &d = op->dev;
         d = mc3d::OpDev{};
@@ -138,7 +138,23 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
         const bool offsets_ok = npts * 8 < 2147483648L;
         op->fast = p.uniform && rows_ok && offsets_ok && mc3d::lineop_fast_supported(d, (size_t)lds_limit);
         if (tune.family == 2) op->fast = false;
-        if (op->fast || p.MM == 16 || first) break;
+        return MC3D_OK;
+    };
+    for (int mm : cand) {
+        if (mm < 0) break;
+        rc = plan_with(mm);
+        if (rc) {
+            delete op;
+            return mc3d::fail(rc, err);
+        }
+        if (op->fast) break;
+    }
+    if (!op->fast && !tune.mm && op->plan.MM != 16) {
+        rc = plan_with(16);
+        if (rc) {
+            delete op;
+            return mc3d::fail(rc, err);
+        }
     }
     if (!op->fast && op->plan.MM != 16 && op->plan.MM != 32) {  // the LDS-tiled kernel's chunk sizes
         delete op;

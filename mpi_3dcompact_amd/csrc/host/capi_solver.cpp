@@ -1,6 +1,7 @@
 // C ABI of the whole solver (include/mc3d.h).  Exceptions never cross it.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -23,6 +24,23 @@ struct mc3d_solver {
 using mc3d::guarded;
 
 extern "C" {
+
+int mc3d_options_check(const char *options_text, char *buf, int buflen, int *needed) {
+    if (!options_text) return mc3d::fail(MC3D_EINVAL, "null options text");
+    if (buflen < 0 || (buflen > 0 && !buf)) return mc3d::fail(MC3D_EINVAL, "bad output buffer");
+    mc3d::Options o;
+    std::string err;
+    const int rc = mc3d::Options::parse(options_text, o, err);
+    if (rc) return mc3d::fail(rc, "options: " + err);
+    const std::string d = o.describe();
+    if (needed) *needed = (int)d.size() + 1;
+    if (buflen > 0) {
+        const size_t n = std::min((size_t)buflen - 1, d.size());
+        std::memcpy(buf, d.data(), n);
+        buf[n] = '\0';
+    }
+    return MC3D_OK;
+}
 
 int mc3d_solver_create(const char *options_text, mc3d_solver **out) {
     if (!out) return mc3d::fail(MC3D_EINVAL, "null output handle");
@@ -153,6 +171,11 @@ int mc3d_solver_set_output(mc3d_solver *s, const char *dir, const double *x, con
         const double *mesh[3] = {x, y, z};
         s->cs->set_output(dir ? std::string(dir) : std::string(), mesh);
     });
+}
+
+int mc3d_solver_set_turbdata(mc3d_solver *s, const char *path, int every) {
+    if (!s) return mc3d::fail(MC3D_EINVAL, "null solver");
+    return guarded([&] { s->cs->set_turbdata(path ? std::string(path) : std::string(), every); });
 }
 
 int mc3d_dump_read_header(const char *path, int *timestep, double *time, int dims[3]) {

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cctype>
+#include <cstdio>
 #include <cstdlib>
 #include <sstream>
 
@@ -246,6 +247,83 @@ int Options::parse(const std::string &text, Options &o, std::string &err) {
     if (o.filterStep < 1) { err = "FILTERSTEP must be >= 1"; return MC3D_ECONFIG; }
     if (o.testFilterType != 1 && o.testFilterType != 2) { err = "unknown DSM test filter type"; return MC3D_ECONFIG; }
     return MC3D_OK;
+}
+
+std::string Options::describe() const {
+    static const char *ts[] = {"CONST_DT", "CONST_CFL"};
+    static const char *bt[] = {"PERIODIC_SOLVE", "DIRICHLET_SOLVE"};
+    static const char *bk[] = {"PERIODIC", "INTERNALLY_PERIODIC", "SPONGE", "ADIABATIC_WALL", "CONST_T_WALL",
+                               "MOVING_ADIABATIC_WALL", "INLET"};
+    static const char *sk[] = {"RECTILINEAR", "CYLINDRICAL"};
+    static const char *fd[] = {"CD2", "PADE6", "PENTA10"};
+    static const char *ft[] = {"COMPACT8", "COMPACT10"};
+    static const char *lm[] = {"NOMODEL", "VREMAN", "DSM", "LEARNED"};
+    static const char *av[] = {"NONE", "XI1_AVG", "XI2_AVG", "XI3_AVG", "LOCAL", "GLOBAL"};
+    static const char *rk[] = {"TVDRK3", "RK4", "KENRK4", "LSLDDRK4"};
+    std::string o;
+    auto put = [&o](const char *k, const std::string &v) { o += std::string(k) + " = " + v + "\n"; };
+    auto num = [](double v) {
+        char b[40];
+        std::snprintf(b, sizeof b, "%.17g", v);
+        return std::string(b);
+    };
+    auto b = [](bool v) { return std::string(v ? "TRUE" : "FALSE"); };
+    auto i = [](int v) { return std::to_string(v); };
+    put("TIMESTEPPING.TSTYPE", ts[TSType]);
+    put("TIMESTEPPING.CFL", num(CFL));
+    put("TIMESTEPPING.DT", num(dt));
+    put("TIMESTEPPING.MAXTIME", num(maxTime));
+    put("TIMESTEPPING.MAXTIMESTEP", i(maxTimeStep));
+    put("TIMESTEPPING.FILTERSTEP", i(filterStep));
+    put("TIMESTEPPING.CHECKSTEP", i(checkStep));
+    put("TIMESTEPPING.DUMPSTEP", i(dumpStep));
+    put("TIMESTEPPING.SUBSTEPFILTERING", b(subStepFiltering));
+    put("BC.BCXTYPE", bt[bcXType]);
+    put("BC.BCYTYPE", bt[bcYType]);
+    put("BC.BCZTYPE", bt[bcZType]);
+    const BCKind faces[6] = {bcX0, bcX1, bcY0, bcY1, bcZ0, bcZ1};
+    const char *fn[6] = {"BC.BCX0", "BC.BCX1", "BC.BCY0", "BC.BCY1", "BC.BCZ0", "BC.BCZ1"};
+    for (int f = 0; f < 6; ++f) put(fn[f], bk[faces[f]]);
+    const char *ax = "XYZ";
+    for (int d = 0; d < 3; ++d)
+        for (int e = 0; e < 3; ++e) {
+            const std::string k = std::string("BC.PERIODICDISP") + ax[d] + "_" + ax[e];
+            put(k.c_str(), num(periodicDisp[d][e]));
+        }
+    put("SPONGE.KIND", sk[spongeKind]);
+    put("SPONGE.STRENGTH", num(spongeStrength));
+    put("SPONGE.AVGTIME", num(spongeAvgT));
+    put("SPONGE.PINF", num(spongeP));
+    put("SPONGE.RMIN", num(rMin));
+    put("DOMAIN.NX", i(Nx));
+    put("DOMAIN.NY", i(Ny));
+    put("DOMAIN.NZ", i(Nz));
+    put("PENCILDECOMP.PROW", i(pRow));
+    put("PENCILDECOMP.PCOL", i(pCol));
+    put("SOLVER.RKTYPE", rk[rkType]);
+    put("SOLVER.ALPHAF", num(alphaF));
+    put("SOLVER.MU_REF", num(mu_ref));
+    put("SOLVER.USETIMING", b(useTiming));
+    put("SOLVER.XFDTYPE", fd[xFDType]);
+    put("SOLVER.YFDTYPE", fd[yFDType]);
+    put("SOLVER.ZFDTYPE", fd[zFDType]);
+    put("SOLVER.FILTERTYPE", ft[filterType]);
+    put("RESTART.FROMRESTART", b(fromRestart));
+    put("RESTART.ONLYGRIDFROMRESTART", b(onlyGridFromRestart));
+    put("RESTART.FILENAME", filename);
+    put("RESTART.SPONGEAVGFROMRESTART", b(spongeFromRestart));
+    put("STATS.INTERVAL", i(stats_interval));
+    put("STATS.VELOCITYSTATS", b(velocityStats));
+    put("STATS.THERMOSTATS", b(thermoStats));
+    put("STATS.STATSAVGTYPE", av[statsAvgType]);
+    put("LES.LESMODEL", lm[lesModel]);
+    put("LES.LESAVERAGING", av[musgsAvgType]);
+    put("LES.DSMTESTFILTER", i(testFilterType));
+    put("LES.DSMUSETAUKK", b(useTaukk));
+    put("LES.DSMCOEFFRANGE", b(dumpCoeffRange));
+    put("LES.CLOSUREFILE", closureFile);
+    put("LES.CLOSUREFALLBACK", b(closureFallback));
+    return o;
 }
 
 }  // namespace mc3d

@@ -70,6 +70,9 @@ struct Options {
 
     // Parses INI text.  Returns 0, or a negative code with a message.
     static int parse(const std::string &text, Options &out, std::string &err);
+    // The parsed values as "SECTION.KEY = value" lines in the reference's
+    // spellings (enums by name, booleans TRUE / FALSE, numbers %.17g).
+    std::string describe() const;
 };
 
 }  // namespace mc3d

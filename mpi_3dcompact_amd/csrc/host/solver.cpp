@@ -1241,7 +1241,28 @@ void CurvilinearCSolver::checkSolution() {
     std::fflush(stdout);
 }
 
+void CurvilinearCSolver::set_turbdata(const std::string &path, int every) {
+    if (every < 1) throw Error(MC3D_EINVAL, "the statistics interval must be positive");
+    if (members() > 1) throw Error(MC3D_ECONFIG, "stacked members keep no HIT statistics");
+    turb_path_ = path;
+    turb_every_ = every;
+}
+
 void CurvilinearCSolver::postStep() {
+    // fullStepTemporalHook of the HIT driver (tests/hit/Samtaney_D4/MPI_3DCompact.cpp:372-557,
+    // called first in postStep, src/CurvilinearCSolver.hpp:355-357): every 25
+    // steps one line of 11 statistics, precision 17, appended to turbdata.out
+    if (!turb_path_.empty() && timeStep % turb_every_ == 0) {
+        double v[11];
+        hitStats(v);
+        FILE *f = std::fopen(turb_path_.c_str(), "a");
+        if (!f) throw Error(MC3D_EINVAL, "cannot append to " + turb_path_);
+        bool ok = true;
+        for (int k = 0; k < 11; ++k) ok = std::fprintf(f, "%.17g ", v[k]) > 0 && ok;
+        ok = std::fputc('\n', f) != EOF && ok;
+        ok = std::fclose(f) == 0 && ok;
+        if (!ok) throw Error(MC3D_EINVAL, "short write to " + turb_path_);
+    }
     checkSolution();
     // src/CurvilinearCSolver.hpp:362-364
     if (stats && timeStep % opt.stats_interval == 0) {

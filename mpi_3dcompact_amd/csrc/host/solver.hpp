@@ -431,8 +431,13 @@ class CurvilinearCSolver {
     // Re_lambda, M_t, epsilon, u', Kolmogorov scale, rho', dil', omega',
     // <rho u.u>
     void hitStats(double out[11]);
+    // the HIT driver's fullStepTemporalHook: append the hitStats line to
+    // `path` every `every` steps (empty path: off)
+    void set_turbdata(const std::string &path, int every);
 
   private:
+    std::string turb_path_;
+    int turb_every_ = 25;
     void initGas();
     std::vector<double> mu_members_;  // per-member MU_REF when members > 1
     std::chrono::steady_clock::time_point last_check_ = std::chrono::steady_clock::now();

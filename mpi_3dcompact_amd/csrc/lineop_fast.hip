@@ -444,7 +444,9 @@ bool lineop_fast_supported(const OpDev &op, size_t lds_limit) {
     const int w = op.W, h = op.H;
     const bool pair = (w == 0 && (h == 1 || h == 3)) || (w == 1 && (h == 2 || h == 4 || h == 5)) ||
                       (w == 2 && (h == 3 || h == 5));
-    const int threads = 64 * op.P;
+    // workgroup size as launched: (lines per workgroup) x P
+    const int lines = op.dir != 0 ? 64 : op.periodic ? contig_lines<true>() : contig_lines<false>();
+    const int threads = lines * op.P;
     // Dirichlet interior rows must not reach past the line ends
     const bool ends = op.periodic || (op.nbl >= h && op.nbr >= h && op.nbl <= kSpecRows && op.nbr <= kSpecRows);
     const bool rows = chunk_rows_ok(op.MM) && (op.MM >= 16 || op.periodic) && op.MM >= 2 * h;

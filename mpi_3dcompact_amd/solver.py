@@ -9,6 +9,8 @@ _VP = ctypes.c_void_p
 _D = ctypes.POINTER(ctypes.c_double)
 
 _SIGS = [
+    ("mc3d_options_check", ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_int)]),
     ("mc3d_solver_create", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_VP)]),
     ("mc3d_solver_set_metrics", ctypes.c_int, [_VP, ctypes.POINTER(_D)]),
     ("mc3d_solver_set_mesh", ctypes.c_int, [_VP, _D, _D, _D]),
@@ -30,6 +32,7 @@ _SIGS = [
     ("mc3d_solver_sample_closure", ctypes.c_int, [_VP, _VP, _VP, ctypes.c_long, ctypes.c_long, ctypes.c_long]),
     ("mc3d_solver_eval_sgs", ctypes.c_int, [_VP]),
     ("mc3d_solver_hit_stats", ctypes.c_int, [_VP, _D]),
+    ("mc3d_solver_set_turbdata", ctypes.c_int, [_VP, ctypes.c_char_p, ctypes.c_int]),
     ("mc3d_solver_stats_weights", ctypes.c_int, [_VP, _D, _D]),
     ("mc3d_solver_add_image", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_char_p,
                                              ctypes.c_char_p, ctypes.c_int, ctypes.c_double, ctypes.c_int,
@@ -54,6 +57,22 @@ def _bind():
             fn.argtypes = args
         L._solver_bound = True
     return L
+
+
+def options_check(text):
+    """Parse and validate an options text without a device; returns the parsed
+    values as a {"SECTION.KEY": "value"} dict (raises Mc3dError with the
+    reference's complaint, MC3D_ECONFIG, on a bad deck)."""
+    L = _bind()
+    need = ctypes.c_int(0)
+    check(L.mc3d_options_check(text.encode(), None, 0, ctypes.byref(need)))
+    buf = ctypes.create_string_buffer(need.value)
+    check(L.mc3d_options_check(text.encode(), buf, need.value, None))
+    out = {}
+    for line in buf.value.decode().splitlines():
+        k, _, v = line.partition(" = ")
+        out[k] = v
+    return out
 
 
 def _p(a):
@@ -128,6 +147,10 @@ class Solver:
         out = (ctypes.c_double * 11)()
         check(self._L.mc3d_solver_hit_stats(self._h, out))
         return dict(zip(self.HIT_STATS, out[:]))
+
+    def set_turbdata(self, path, every=25):
+        """Append the hit_stats line to `path` (turbdata.out) every `every` steps; None: off."""
+        check(self._L.mc3d_solver_set_turbdata(self._h, None if path is None else str(path).encode(), every))
 
     def add_image(self, field, var_name, plane=2, fraction=0.5, width=512, height=512, interval=0,
                   vrange=None, colormap="RAINBOW"):

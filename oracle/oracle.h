@@ -72,6 +72,10 @@ int orc_metrics(const int fd[3], const int bc[3], const int N[3], const double t
 void orc_periodic_translations(const int bc[3], const int face_lo[3], const int face_hi[3],
                                const double disp[3][3], double trans[3][3], int set[3]);
 
+/* ---- OpenMP threads of the oracle's parallel loops (CPU baseline timing) ---- */
+int orc_max_threads(void);            /* omp_get_max_threads() */
+void orc_set_threads(int n);          /* omp_set_num_threads(n) */
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,6 +4,8 @@
 // kept so that, built without FP contraction, results match the reference.
 #include "oracle.h"
 
+#include <omp.h>
+
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -77,6 +79,13 @@ double minor3(const double m[16], int r0, int r1, int r2, int c0, int c1, int c2
 }
 
 }  // namespace
+
+extern "C" {
+
+int orc_max_threads(void) { return omp_get_max_threads(); }
+void orc_set_threads(int n) { omp_set_num_threads(n); }
+
+}  // extern "C"
 
 extern "C" {
 

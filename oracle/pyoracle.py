@@ -51,8 +51,22 @@ def lib():
         L.orc_solve_penta.argtypes = [_D, _D, _D, _D, _D, _D, _D, ctypes.c_int]
         L.orc_cyclic_penta.argtypes = [_D, _D, _D, _D, _D, _D, _D, _D, ctypes.c_int]
         L.orc_invert4.argtypes = [_D, _D]
+        L.orc_max_threads.restype = ctypes.c_int
+        L.orc_max_threads.argtypes = []
+        L.orc_set_threads.restype = None
+        L.orc_set_threads.argtypes = [ctypes.c_int]
         _LIB = L
     return _LIB
+
+
+def max_threads():
+    """Threads the oracle's OpenMP loops run with (omp_get_max_threads)."""
+    return lib().orc_max_threads()
+
+
+def set_threads(n):
+    """Threads for the oracle's later OpenMP loops (omp_set_num_threads)."""
+    lib().orc_set_threads(int(n))
 
 
 def _p(a):

@@ -261,3 +261,100 @@ def test_uneven_sizes_take_the_fast_kernels(torch_cuda, shape, mm):
                 op.apply([fin], [out])
                 torch.cuda.synchronize()
                 assert relerr(out.cpu().numpy(), P.filter_field(kind, af, d, list(bc3), f)) <= TOL, (kind, bc3, d)
+
+
+def test_eight_row_chunks(torch_cuda):
+    """88-point periodic lines split into eleven 8-row chunks: the fast kernels
+    take derivatives and the Compact8 filter (half-width 4 fits 8 rows);
+    Compact10 and CommN5Pade (half-width 5) take the LDS-tiled kernel.  24-point
+    z-lines split into 12-row chunks.  Dirichlet lines of these lengths have no
+    even split the fast kernels take."""
+    from mpi_3dcompact_amd.ops import LineOp
+    torch = torch_cuda
+    shape = (24, 88, 88)
+    Nz, Ny, Nx = shape
+    dims = (Nx, Ny, Nz)
+    mm = {0: 8, 1: 8, 2: 12}
+    f = field(shape, 11)
+    fin = torch.from_numpy(f).cuda()
+    out = torch.empty_like(fin)
+    for d in range(3):
+        fam = "lineop_contig<" if d == 0 else "lineop_strided<"
+        for bc in (P.PERIODIC, P.DIRICHLET):
+            n = dims[d]
+            for scheme in (P.PADE6, P.PENTA10):
+                dd = 1.0 / n if bc == P.PERIODIC else 1.0 / (n - 1)
+                op = LineOp.deriv(scheme, d, bc, dims, dd)
+                kname = op.kernel()
+                _say(f"deriv scheme={scheme} bc={bc} dir={d} kernel={kname}")
+                if bc == P.PERIODIC:
+                    assert kname.startswith(fam) and kname.endswith(f", {mm[d]}, true>"), kname
+                op.apply([fin], [out])
+                torch.cuda.synchronize()
+                assert relerr(out.cpu().numpy(), P.deriv_field(scheme, d, bc, f, dd)) <= TOL, (scheme, bc, d)
+        for kind, af in ((P.COMPACT8, 0.49), (P.COMPACT10, 0.49), (P.COMMN5PADE, 0.0)):
+            op = LineOp.filter(kind, af, d, (0, 0, 0), dims)
+            kname = op.kernel()
+            _say(f"filter kind={kind} dir={d} kernel={kname}")
+            if kind == P.COMPACT8:
+                assert kname.startswith(fam) and kname.endswith(f", {mm[d]}, true>"), kname
+            elif mm[d] == 8:
+                assert kname.startswith("lineop_kernel<"), kname
+            op.apply([fin], [out])
+            torch.cuda.synchronize()
+            assert relerr(out.cpu().numpy(), P.filter_field(kind, af, d, [0, 0, 0], f)) <= TOL, (kind, d)
+
+
+@pytest.mark.parametrize("nx", [384, 512])
+def test_long_periodic_x_lines(torch_cuda, nx):
+    """x-lines longer than sixteen 16-row chunks: the contiguous kernel's
+    workgroup is 16 lines x P chunks (384 or 512 lanes), so it still takes
+    them with 16-row chunks."""
+    from mpi_3dcompact_amd.ops import LineOp
+    torch = torch_cuda
+    shape = (4, 20, nx)
+    dims = (nx, 20, 4)
+    f = field(shape, 12)
+    fin = torch.from_numpy(f).cuda()
+    out = torch.empty_like(fin)
+    for scheme in (P.PADE6, P.PENTA10):
+        op = LineOp.deriv(scheme, 0, P.PERIODIC, dims, 1.0 / nx)
+        kname = op.kernel()
+        _say(f"deriv scheme={scheme} nx={nx} kernel={kname}")
+        assert kname.startswith("lineop_contig<") and kname.endswith(", 16, true>"), kname
+        op.apply([fin], [out])
+        torch.cuda.synchronize()
+        assert relerr(out.cpu().numpy(), P.deriv_field(scheme, 0, P.PERIODIC, f, 1.0 / nx)) <= TOL, scheme
+    for kind, af in ((P.COMPACT8, 0.49), (P.COMPACT10, 0.49), (P.COMMN5PADE, 0.0)):
+        op = LineOp.filter(kind, af, 0, (0, 0, 0), dims)
+        kname = op.kernel()
+        _say(f"filter kind={kind} nx={nx} kernel={kname}")
+        assert kname.startswith("lineop_contig<") and kname.endswith(", 16, true>"), kname
+        op.apply([fin], [out])
+        torch.cuda.synchronize()
+        assert relerr(out.cpu().numpy(), P.filter_field(kind, af, 0, [0, 0, 0], f)) <= TOL, kind
+
+
+def test_long_dirichlet_x_lines(torch_cuda):
+    """288-point Dirichlet x-lines: 16-row chunks would need 18 x 64 lanes, so
+    the plan's 32-row chunks are tried next (9 x 64 lanes) and take the
+    contiguous kernel when its 64-line tile fits the workgroup's LDS."""
+    from mpi_3dcompact_amd.ops import LineOp
+    torch = torch_cuda
+    nx = 288
+    shape = (4, 20, nx)
+    dims = (nx, 20, 4)
+    f = field(shape, 13)
+    fin = torch.from_numpy(f).cuda()
+    out = torch.empty_like(fin)
+    lds = getattr(torch.cuda.get_device_properties(0), "shared_memory_per_block", "?")
+    for scheme in (P.PADE6, P.PENTA10):
+        dd = 1.0 / (nx - 1)
+        op = LineOp.deriv(scheme, 0, P.DIRICHLET, dims, dd)
+        kname = op.kernel()
+        _say(f"deriv scheme={scheme} nx={nx} dirichlet kernel={kname} lds/block={lds}")
+        assert kname.startswith("lineop_kernel<") or (kname.startswith("lineop_contig<") and
+                                                       kname.endswith(", 32, false>")), kname
+        op.apply([fin], [out])
+        torch.cuda.synchronize()
+        assert relerr(out.cpu().numpy(), P.deriv_field(scheme, 0, P.DIRICHLET, f, dd)) <= TOL, scheme

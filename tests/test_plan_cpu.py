@@ -223,7 +223,8 @@ def test_chunking_choice():
     # lengths that are not multiples of 16: the largest of 28, 24, 20 rows (and
     # 12, 8 on periodic lines) that splits the line into 2..16 equal chunks
     for n, bc, mm, chunks in ((200, P.PERIODIC, 20, 10), (200, P.DIRICHLET, 20, 10), (56, P.DIRICHLET, 28, 2),
-                              (72, P.PERIODIC, 24, 3), (36, P.PERIODIC, 12, 3), (24, P.PERIODIC, 12, 2)):
+                              (72, P.PERIODIC, 24, 3), (36, P.PERIODIC, 12, 3), (24, P.PERIODIC, 12, 2),
+                              (88, P.PERIODIC, 8, 11), (104, P.PERIODIC, 8, 13)):
         t = tables(0, P.PENTA10, bc, n, 1.0 / n)
         assert (t["MM"], t["P"]) == (mm, chunks), (n, bc)
     # no such divisor, or a Dirichlet line that only 12 rows would split: 16-row chunks, the last shorter

@@ -252,3 +252,45 @@ def test_hit_stats_match_numpy(gpu):
         assert abs(st[k] - v) <= 1e-10 * abs(v), (k, st[k], v)
     t, _, _ = g.scalars()
     assert st["time"] == t
+
+
+def test_turbdata_hook(gpu, tmp_path):
+    """The HIT driver's turbdata.out hook: one line every `every` steps, 11
+    values with 17 significant digits each followed by a space; each line is
+    the hit_stats of that step, and writing it leaves the run unchanged."""
+    from mpi_3dcompact_amd.solver import Solver
+    N = 32
+    case = dict(fd="PADE6", filt="COMPACT10", alpha_f=0.495, rk="LSLDDRK4", mu_ref=3.3e-4)
+    ic = hit.hit_initial_condition(N, k0=4.0, Mt=0.3, seed=5)
+
+    def make():
+        g = Solver(hit.options_text(N, **case), (N, N, N))
+        g.set_metrics(hit.identity_metrics(N))
+        g.set_initial(ic)
+        return g
+
+    path = tmp_path / "turbdata.out"
+    a = make()
+    a.set_turbdata(path, every=2)
+    a.step(6)
+    b = make()
+    want = []
+    for _ in range(3):
+        b.step(2)
+        want.append(b.hit_stats())
+    for n in CONS:
+        assert np.array_equal(a.field(n), b.field(n)), n
+    lines = path.read_text().splitlines(keepends=True)
+    assert len(lines) == 3
+    for ln, w in zip(lines, want):
+        assert ln.endswith(" \n")
+        vals = ln.split()
+        assert len(vals) == 11
+        got = [float(v) for v in vals]
+        assert got == [w[k] for k in Solver.HIT_STATS], (got, w)
+        assert vals == [f"{w[k]:.17g}" for k in Solver.HIT_STATS]
+    a.set_turbdata(None)
+    a.step(2)
+    assert len(path.read_text().splitlines()) == 3
+    a.close()
+    b.close()
