@@ -76,10 +76,27 @@ def cart_grad(met, d):
     return [J[0][j] * d[0] + J[1][j] * d[1] + J[2][j] * d[2] for j in range(3)]
 
 
-def setup(N, les, les_avg, seed, k0, Mt, fd="PENTA10", **extra):
+def thermal_ic(ic, amp):
+    """The HIT field with smooth density and pressure fluctuations of relative
+    size amp.  The HIT initial field has uniform rho and p, so its temperature
+    is uniform and the DSM's K_i = (rho u_i T)^ - (rho u_i)^ (rho T)^ / rho and
+    N_i = rho |S| dT/dx_i vanish identically: without tau_kk (which adds
+    temperature fluctuations) k_sgs is then a ratio of rounding residues."""
+    n = ic["rho"].shape[0]
+    t = 2 * np.pi * np.arange(n) / n
+    z, y, x = np.meshgrid(t, t, t, indexing="ij")
+    out = dict(ic)
+    out["rho"] = ic["rho"] * (1 + amp * np.sin(x + 2 * y) * np.cos(z))
+    out["p"] = ic["p"] * (1 + amp * np.cos(2 * x - y + z))
+    return out
+
+
+def setup(N, les, les_avg, seed, k0, Mt, fd="PENTA10", thermal=0.0, **extra):
     case = dict(fd=fd, filt="COMPACT8", alpha_f=0.49, rk="KENRK4", les=les, les_avg=les_avg, mu_ref=1e-4,
                 **extra)
     ic = hit.hit_initial_condition(N, k0=k0, Mt=Mt, seed=seed)
+    if thermal:
+        ic = thermal_ic(ic, thermal)
     met = sheared_metrics(N)
     o = P.Solver(hit.oracle_config(N, **case), met, ic)
     st = {k: o.field(k) for k in ("rho", "rhoU", "rhoV", "rhoW", "U", "V", "W", "p", "T")}
@@ -142,8 +159,9 @@ def test_dsm_in_every_layout(torch_cuda, case):
     torch = torch_cuda
     avg, k0, Mt, filt, taukk_on, tols = case
     N = 24
+    # without tau_kk the temperature needs fluctuations of its own (thermal_ic)
     c, o, st, met, gradU, sch, dd = setup(N, "DSM", avg, seed=5, k0=k0, Mt=Mt, dsm_filter=filt,
-                                          use_taukk=taukk_on, coeff_range=True)
+                                          use_taukk=taukk_on, coeff_range=True, thermal=0.0 if taukk_on else 0.05)
     want = {k: o.field(k) for k in tols}
     for layout in (XP, YP, ZP):
         _say(f"dsm avg={avg} filter={filt} taukk={taukk_on} layout={layout}")
@@ -230,7 +248,13 @@ def test_learned_matches_the_solver_closure(torch_cuda, les_avg, lo, hi):
     LESMODEL LEARNED gives on the same state (the solver's closure is pinned to
     the oracle in test_closure_gpu.py): the network, the feature averaging
     along the layout's axes and, with a narrow trusted range, the DSM fallback
-    on the stages whose points left it."""
+    on the stages whose points left it.  The state has temperature
+    fluctuations (thermal_ic): the fallback DSM runs without tau_kk, and on the
+    plain HIT field its k_sgs would be a ratio of rounding residues.  Under
+    LOCAL averaging in another layout the line solves round differently and
+    the DSM divides by point-wise filtered contractions: the oracle moves by
+    2.5e-11 (mu_sgs) and 2.9e-9 (k_sgs) under a 1e-15 relative perturbation of
+    this state, so that case is held to 1e-9 / 1e-7; every other case to 1e-12."""
     from mpi_3dcompact_amd.sgs import Sgs
     from mpi_3dcompact_amd.solver import Solver
     torch = torch_cuda
@@ -241,7 +265,7 @@ def test_learned_matches_the_solver_closure(torch_cuda, les_avg, lo, hi):
     met = hit.identity_metrics(N)
     g = Solver(opts, (N, N, N))
     g.set_metrics(met)
-    g.set_initial(hit.hit_initial_condition(N, k0=4.0, Mt=0.3, seed=6))
+    g.set_initial(thermal_ic(hit.hit_initial_condition(N, k0=4.0, Mt=0.3, seed=6), 0.05))
     g.set_closure(c)
     g.eval_sgs()
     d = {q: [g.field(f"d{q}{k}") for k in range(3)] for q in "UVWT"}
@@ -265,6 +289,9 @@ def test_learned_matches_the_solver_closure(torch_cuda, les_avg, lo, hi):
         assert ev == 1 and (fb_stages == 1) == (lo > -1e30)
         for name, t in (("mu_sgs", mu), ("k_sgs", ks)):
             e = relerr(from_layout(t.cpu().numpy(), layout), want[name])
-            _say(f"  {name} {e:.2e}")
-            assert e <= 1e-12, (layout, name, e)
+            tol = 1e-12
+            if les_avg == "LOCAL" and layout != XP:
+                tol = {"mu_sgs": 1e-9, "k_sgs": 1e-7}[name]
+            _say(f"  {name} {e:.2e} (tolerance {tol:.0e})")
+            assert e <= tol, (layout, name, e)
         s.close()
