@@ -130,6 +130,18 @@ const char *mc3d_lineop_kernel(const mc3d_lineop *op);
 int mc3d_cfl_dt(const double *const fields[7], long npts, int members, const double spacing[3], double cfl,
                 double *out, void *stream);
 
+/* mc3d_cfl_dt without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as mc3d_cfl_dt; the partial maxima go to a scratch
+ * array the library keeps on each GPU for the life of the process (1 MiB,
+ * 2048 doubles per box), so members <= 64, else MC3D_EINVAL.  The call only
+ * queues its two kernels on `stream` and returns without waiting; out is
+ * ready when the stream reaches it.  Calls that overlap on one GPU (two
+ * streams, two threads) share that scratch array and race. */
+int mc3d_cfl_dt_noalloc(const double *const fields[7], long npts, int members, const double spacing[3],
+                        double cfl, double *out, void *stream);
+
 /* ---------------------------------------------------------------------------
  * SGS models over caller-owned device fields.
  * Replaces: AbstractSGS (src/AbstractSGS.hpp:9-42) and its subclasses

@@ -3,6 +3,7 @@
 // without the solver around it.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 
 #include "capi_common.h"
@@ -42,6 +43,36 @@ hipError_t launch(const double *const fields[7], long npts, int members, const d
     return mc3d::launch_cfl_out(partial, mc3d::cfl_blocks(npts), members, cfl, out, st);
 }
 
+// Partial maxima of mc3d_cfl_dt_noalloc: cfl_blocks(npts) <= 2048 doubles per
+// box for at most kScratchBoxes boxes (1 MiB), held by every GPU the library
+// runs on until the process ends.  Host code never passes `scratch` itself to
+// a kernel (on the host it names a stand-in, not device memory): it passes the
+// device address read once per device with hipGetSymbolAddress.
+constexpr int kScratchBoxes = 64;
+constexpr long kScratchDoubles = 2048L * kScratchBoxes;
+__device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+double *scratch_ptr[kMaxDevices] = {};
+
+// the current device's copy of `scratch`, looked up on the first call there
+int scratch_address(double **out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return mc3d::fail(MC3D_EHIP, "hipGetDevice failed");
+    if (dev < 0 || dev >= kMaxDevices) return mc3d::fail(MC3D_EINVAL, "device index beyond 63");
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return mc3d::fail(MC3D_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = scratch_ptr[dev];
+    return MC3D_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -59,6 +90,22 @@ int mc3d_cfl_dt(const double *const fields[7], long npts, int members, const dou
     (void)hipFree(partial);
     if (e != hipSuccess) return mc3d::fail(MC3D_EHIP, std::string("CFL launch: ") + hipGetErrorString(e));
     if (es != hipSuccess) return mc3d::fail(MC3D_EHIP, std::string("CFL: ") + hipGetErrorString(es));
+    return MC3D_OK;
+}
+
+int mc3d_cfl_dt_noalloc(const double *const fields[7], long npts, int members, const double spacing[3],
+                        double cfl, double *out, void *stream) {
+    int rc = check_args(fields, npts, members, spacing, cfl, out);
+    if (rc) return rc;
+    if ((long)mc3d::cfl_blocks(npts) * members > kScratchDoubles)
+        return mc3d::fail(MC3D_EINVAL, "mc3d_cfl_dt_noalloc takes at most 64 boxes (its scratch holds 2048 "
+                                       "partial maxima per box); use mc3d_cfl_dt");
+    double *partial = nullptr;
+    rc = scratch_address(&partial);
+    if (rc) return rc;
+    // every workgroup of k_cfl writes its own partial, so nothing is cleared
+    const hipError_t e = launch(fields, npts, members, spacing, cfl, partial, out, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return mc3d::fail(MC3D_EHIP, std::string("CFL launch: ") + hipGetErrorString(e));
     return MC3D_OK;
 }
 

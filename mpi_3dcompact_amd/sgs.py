@@ -17,6 +17,8 @@ _L = ctypes.POINTER(ctypes.c_long)
 _SIGS = [
     ("mc3d_cfl_dt", ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_long, ctypes.c_int, _D, ctypes.c_double, _VP,
                                    _VP]),
+    ("mc3d_cfl_dt_noalloc", ctypes.c_int, [ctypes.POINTER(_VP), ctypes.c_long, ctypes.c_int, _D, ctypes.c_double,
+                                           _VP, _VP]),
     ("mc3d_sgs_create", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _VP, _VP, ctypes.POINTER(_VP)]),
     ("mc3d_sgs_destroy", None, [_VP]),
     ("mc3d_sgs_vreman", ctypes.c_int, [_VP, ctypes.POINTER(_VP), _VP, _VP]),
@@ -61,16 +63,21 @@ def _stream(stream):
     return stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream)
 
 
-def cfl_dt(U, V, W, sos, J11, J22, J33, spacing, cfl, members=1, stream=None):
-    """(dt, maxspeed) per box as a (members, 2) float64 GPU tensor."""
+def cfl_dt(U, V, W, sos, J11, J22, J33, spacing, cfl, members=1, stream=None, out=None, noalloc=False):
+    """(dt, maxspeed) per box as a (members, 2) float64 GPU tensor (written
+    into out when given).  noalloc: mc3d_cfl_dt_noalloc, which allocates
+    nothing and returns without waiting for the stream."""
     import torch
     n = U.numel()
     if n % members:
         raise ValueError("points not divisible by members")
-    out = torch.empty((members, 2), dtype=torch.float64, device=U.device)
+    if out is None:
+        out = torch.empty((members, 2), dtype=torch.float64, device=U.device)
+    elif out.numel() < 2 * members:
+        raise ValueError("out holds fewer than 2 * members doubles")
     sp = (ctypes.c_double * 3)(*spacing)
-    check(_bind().mc3d_cfl_dt(_ptrs([U, V, W, sos, J11, J22, J33]), n // members, members, sp, cfl, _ptr(out),
-                              _stream(stream)))
+    fn = _bind().mc3d_cfl_dt_noalloc if noalloc else _bind().mc3d_cfl_dt
+    check(fn(_ptrs([U, V, W, sos, J11, J22, J33]), n // members, members, sp, cfl, _ptr(out), _stream(stream)))
     return out
 
 
