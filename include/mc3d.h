@@ -432,13 +432,15 @@ int mc3d_dump_write(const char *path, int timestep, double time, const int dims[
  * the partition tables an operator would upload, for verification of the
  * factorisation without a GPU.  cls 0 = derivative (kind = scheme), 1 = filter
  * (kind = filter kind); mm = rows per chunk (16 or 32, 0 = the choice the
- * operators make).  sizes[10] receives {N, W, H, P, NI, nbl, nbr, int_off, MM,
- * dense_ok}; each non-NULL table pointer must have room for the sizes
+ * operators make).  sizes[11] receives {N, W, H, P, NI, nbl, nbr, int_off, MM,
+ * dense_ok, gb} (gb: the interface band in chunks the fast kernels use, G
+ * blocks farther from a chunk being below 1e-17 of its largest); each
+ * non-NULL table pointer must have room for the sizes
  * documented in mpi_3dcompact_amd/csrc/lineop.h (lu: P*MM*5, sp: P*MM*2W,
  * G: P*2W*NI, rows: (nbl+nbr)*19, drows: (nbl+nbr)*17, interior: 11,
  * cstart/clen: P). */
 int mc3d_plan_tables(int cls, int kind, int bc, int n, double dd, double alphaF, int mm,
-                     int sizes[10], double *lu, double *sp, double *G, double *rows,
+                     int sizes[11], double *lu, double *sp, double *G, double *rows,
                      double *drows, double *interior, int *cstart, int *clen);
 
 /* ---------------------------------------------------------------------------

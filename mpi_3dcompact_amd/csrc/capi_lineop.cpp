@@ -118,7 +118,7 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
         d = mc3d::OpDev{};
         d.N = p.N; d.dir = dir; d.W = p.W; d.H = p.H; d.P = p.P; d.L = p.L; d.NI = p.NI;
         d.ext = p.ext; d.MM = p.MM; d.periodic = p.periodic; d.nbl = p.nbl; d.nbr = p.nbr;
-        d.int_off = p.int_off; d.int_w = (int)p.int_c.size();
+        d.int_off = p.int_off; d.int_w = (int)p.int_c.size(); d.gb = p.gb;
         for (int k = 0; k < d.int_w; ++k) d.int_c[k] = p.int_c[k];
         d.dd = p.dd; d.divide = p.divide ? 1 : 0;
         // 1/dd is exact when dd is a power of two; then r * (1/dd) == r / dd bitwise
@@ -316,7 +316,7 @@ int mc3d_lineop_info(const mc3d_lineop *op, int *chunks, int *chunk_rows, int *l
 const char *mc3d_lineop_kernel(const mc3d_lineop *op) { return op ? op->kname.c_str() : nullptr; }
 
 int mc3d_plan_tables(int cls, int kind, int bc, int n, double dd, double alphaF, int mm,
-                     int sizes[10], double *lu, double *sp, double *G, double *rows,
+                     int sizes[11], double *lu, double *sp, double *G, double *rows,
                      double *drows, double *interior, int *cstart, int *clen) {
     if (!sizes) return mc3d::fail(MC3D_EINVAL, "null sizes");
     mc3d::PlanRequest rq{cls, kind, bc, n, dd, alphaF, mm};
@@ -324,7 +324,7 @@ int mc3d_plan_tables(int cls, int kind, int bc, int n, double dd, double alphaF,
     std::string err;
     int rc = mc3d::build_plan(rq, p, err);
     if (rc) return mc3d::fail(rc, err);
-    const int v[10] = {p.N, p.W, p.H, p.P, p.NI, p.nbl, p.nbr, p.int_off, p.MM, p.dense_ok ? 1 : 0};
+    const int v[11] = {p.N, p.W, p.H, p.P, p.NI, p.nbl, p.nbr, p.int_off, p.MM, p.dense_ok ? 1 : 0, p.gb};
     std::memcpy(sizes, v, sizeof(v));
     if (lu) std::memcpy(lu, p.lu.data(), sizeof(double) * p.lu.size());
     if (sp) std::memcpy(sp, p.sp.data(), sizeof(double) * p.sp.size());

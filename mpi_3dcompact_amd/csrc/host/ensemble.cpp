@@ -297,34 +297,42 @@ void Ensemble::tuneBucket(Bucket &b, std::map<std::string, std::string> &cache) 
                     hip_check(hipEventElapsedTime(&ms, e0, e1), "tuning");
                     return (double)ms / reps;
                 };
+                // candidates: the untuned default first (the plan's own
+                // chunking, which for line lengths such as 40, 72 or 200 is an
+                // even 8..28-row split that 16- and 32-row chunks do not
+                // give), then 16- and 32-row chunks for the fast kernels and
+                // for the LDS-tiled kernel with 1..256 lines per workgroup
+                std::vector<LineTune> cand(1);
                 for (int mm : {16, 32})
                     for (int family : {1, 2})
                         for (int lines = family == 1 ? 0 : 1; lines <= (family == 1 ? 0 : 256);
-                             lines = lines ? 2 * lines : 1) {
+                             lines = family == 1 ? 1 : 2 * lines) {
                             LineTune t;
                             t.mm = mm;
                             t.family = family;
                             t.lines = lines;
-                            mc3d_lineop *op = nullptr;
-                            if (lineop_create(cls == 0 ? kDeriv : kFilter, kind, d, bc3, b.dims, dd[d],
-                                              opt_.alphaF, nm, t, &op) != MC3D_OK)
-                                continue;
-                            double ms = 0.0;
-                            try {
-                                ms = time_op(op);
-                            } catch (...) {
-                                mc3d_lineop_destroy(op);
-                                throw;
-                            }
-                            mc3d_lineop_destroy(op);
-                            ++best.candidates;
-                            best.trials.emplace_back(t, ms);
-                            if (best.ms < 0.0 || ms < best.ms) {
-                                best.ms = ms;
-                                best.shape = t;
-                            }
-                            if (family == 1) break;
+                            cand.push_back(t);
                         }
+                for (const LineTune &t : cand) {
+                    mc3d_lineop *op = nullptr;
+                    if (lineop_create(cls == 0 ? kDeriv : kFilter, kind, d, bc3, b.dims, dd[d], opt_.alphaF, nm,
+                                      t, &op) != MC3D_OK)
+                        continue;
+                    double ms = 0.0;
+                    try {
+                        ms = time_op(op);
+                    } catch (...) {
+                        mc3d_lineop_destroy(op);
+                        throw;
+                    }
+                    mc3d_lineop_destroy(op);
+                    ++best.candidates;
+                    best.trials.emplace_back(t, ms);
+                    if (best.ms < 0.0 || ms < best.ms) {
+                        best.ms = ms;
+                        best.shape = t;
+                    }
+                }
                 if (best.candidates == 0) throw Error(MC3D_EINVAL, std::string("no launch shape for ") + key);
                 cache[key] = format_shape(best);
             }

@@ -56,6 +56,7 @@ struct OpDev {
     const double *lu;     // P * MM * kLuRec
     const double *sp;     // P * MM * 2W
     const double *G;      // P * 2W * NI
+    int gb;               // interface band in chunks (Plan::gb)
     const int *cstart;    // P
     const int *clen;      // P
 };

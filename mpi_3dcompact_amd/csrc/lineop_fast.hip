@@ -93,6 +93,50 @@ __device__ __forceinline__ void chunk_solve(const double *lu, double *y) {
     }
 }
 
+// s[j] = sum over the chunks u of the band around chunk c of G_c[j][u's 2W
+// columns] . yhat[u's 2W values].  Periodic lines read chunk 0's rows G_0
+// rotated by c chunks (G_c[j][u] = G_0[j][u - 2Wc]); chunk offsets -gb..gb, or
+// every chunk once when the band covers the line.  Dirichlet lines read G_c
+// over chunks max(0, c - gb) .. min(P - 1, c + gb).
+template <int W, bool PER>
+__device__ __forceinline__ void interface_sums(const OpDev &op, int c, const double *yhat, double *s) {
+    constexpr int W2 = 2 * W;
+    const int P = op.P, NI = op.NI, gb = op.gb;
+#pragma unroll
+    for (int j = 0; j < W2; ++j) s[j] = 0.0;
+    if (PER) {
+        const bool band = 2 * gb + 1 < P;
+        const int nb = band ? 2 * gb + 1 : P, d0 = band ? -gb : 0;
+        for (int t = 0; t < nb; ++t) {
+            const int d = d0 + t;
+            const int cb = d < 0 ? d + P : d;  // column block of G_0
+            int ub = c + d;                    // chunk whose interface values it multiplies
+            ub = ub < 0 ? ub + P : (ub >= P ? ub - P : ub);
+            const double *g = op.G + cb * W2;
+            const double *yv = yhat + ub * W2;
+#pragma unroll
+            for (int k = 0; k < W2; ++k) {
+                const double v = yv[k];
+#pragma unroll
+                for (int j = 0; j < W2; ++j) s[j] += g[j * NI + k] * v;
+            }
+        }
+    } else {
+        const double *G = op.G + (size_t)c * W2 * NI;
+        const int lo = c - gb < 0 ? 0 : c - gb, hi = c + gb > P - 1 ? P - 1 : c + gb;
+        for (int ub = lo; ub <= hi; ++ub) {
+            const double *g = G + ub * W2;
+            const double *yv = yhat + ub * W2;
+#pragma unroll
+            for (int k = 0; k < W2; ++k) {
+                const double v = yv[k];
+#pragma unroll
+                for (int j = 0; j < W2; ++j) s[j] += g[j * NI + k] * v;
+            }
+        }
+    }
+}
+
 // raw buffer access: descriptor in SGPRs, one 32-bit lane offset, row offset in an SGPR
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 constexpr int kRawBufferWord3 = 0x00020000;  // gfx9-family raw buffer descriptor, dword 3
@@ -187,15 +231,8 @@ __global__ __launch_bounds__(1024) void lineop_strided(OpDev op, LaunchArgs a) {
     __syncthreads();
     if (W > 0) {
         const double *myred = red + l * (NI + 1);
-        const double *G = op.G + (size_t)c * 2 * W * NI;
         double s[2 * W > 0 ? 2 * W : 1];
-#pragma unroll
-        for (int j = 0; j < 2 * W; ++j) s[j] = 0.0;
-        for (int u = 0; u < NI; ++u) {
-            const double v = myred[u];
-#pragma unroll
-            for (int j = 0; j < 2 * W; ++j) s[j] += G[j * NI + u] * v;
-        }
+        interface_sums<W, PER>(op, c, myred, s);
         const double *sp = op.sp + (PER ? 0 : (size_t)c * MM * 2 * W);
 #pragma unroll
         for (int i = 0; i < MM; ++i) {
@@ -239,17 +276,6 @@ constexpr int contig_batch() {
     return MM % 16 ? (MM <= 12 ? MM : MM / 2) : (MM == 32 || (PER && H < 5) ? 16 : 8);
 }
 
-// (line, point) of element tid + k * nthr of a LINES x N tile, advanced
-// incrementally (nthr = qd * N + rd)
-struct TileWalk {
-    int ll, p;
-    __device__ __forceinline__ void next(int qd, int rd, int N) {
-        ll += qd;
-        p += rd;
-        if (p >= N) { p -= N; ++ll; }
-    }
-};
-
 template <int W, int H, int MM, bool PER>
 __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
     extern __shared__ double smem[];
@@ -264,36 +290,62 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
     constexpr bool kShare = W == 2;
     double *red = kShare ? smem : smem + (size_t)(N + 2 * H) * LS;
     const int tid = threadIdx.y * L + threadIdx.x;
-    const int nthr = L * P;                         // L * N = nthr * MM elements
-    const int qd = nthr / N, rd = nthr - qd * N;
+    const int nthr = L * P;  // L * N = nthr * MM elements
     const int q0 = blockIdx.x * L;
-    const double *__restrict__ in = a.in[blockIdx.y];
-    double *__restrict__ out = a.out[blockIdx.y];
-    // fill: every thread issues its coalesced loads in batches before the LDS writes
+    // The workgroup's lines are consecutive rows of the field: one contiguous
+    // block of L * N doubles, element e at point e % N of line e / N.  Buffer
+    // descriptors over that block end at the last line of the field, so the
+    // loads of a partial last workgroup past it return 0 and its stores there
+    // are dropped.
+    const int rem = a.nlines - q0;
+    const unsigned nrec = (unsigned)(rem < L ? rem : L) * (unsigned)N * 8u;
+    const __amdgpu_buffer_rsrc_t rin = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<double *>(a.in[blockIdx.y]) + (long)q0 * N, (short)0, (int)nrec, kRawBufferWord3);
+    // tile slot (p + H) * LS + line of element tid + k * nthr, advanced k by
+    // k: by (nthr / N lines, nthr % N points), wrapping the point.  When the
+    // chunk rows divide the lines per workgroup, nthr is a whole number of
+    // lines and the slot advances by a constant.
+    constexpr bool kWhole = L % MM == 0;
+    const int ll0 = tid / N, p0 = tid - ll0 * N;
+    const int slot0 = (p0 + H) * LS + ll0;
+    const int qd = nthr / N, rd = nthr - qd * N;
+    const int step = rd * LS + qd, wrapfix = 1 - N * LS;
+    constexpr int kB = contig_batch<H, MM, PER>();
+    // fill: batches of coalesced loads, then their LDS writes
     {
-        TileWalk t{tid / N, tid % N};
-        constexpr int kB = contig_batch<H, MM, PER>();
+        // periodic halo rows: phi[N - H .. N) above row H, phi[0 .. H) below row N + H
+        double hv = 0.0;
+        int hslot = -1;
+        if (PER && tid < 2 * H * L) {
+            const int r = tid / L, ll = tid - r * L;
+            hv = bload(rin, (unsigned)(ll * N + (r < H ? N - H + r : r - H)) * 8u, 0);
+            hslot = (r < H ? r : N + r) * LS + ll;
+        }
+        int slot = slot0, pp = p0;
 #pragma unroll 1
         for (int k0 = 0; k0 < MM; k0 += kB) {
             double v[kB];
-            TileWalk u = t;
+#pragma unroll
+            for (int k = 0; k < kB; ++k) v[k] = bload_nt(rin, (unsigned)(tid + (k0 + k) * nthr) * 8u, 0);
 #pragma unroll
             for (int k = 0; k < kB; ++k) {
-                const int q = q0 + u.ll;
-                v[k] = q < a.nlines ? __builtin_nontemporal_load(in + (long)q * N + u.p) : 0.0;
-                u.next(qd, rd, N);
-            }
-#pragma unroll
-            for (int k = 0; k < kB; ++k) {
-                tile[(t.p + H) * LS + t.ll] = v[k];
-                if (PER) {  // halo rows: periodic images
-                    if (t.p < H) tile[(t.p + H + N) * LS + t.ll] = v[k];
-                    if (t.p >= N - H) tile[(t.p + H - N) * LS + t.ll] = v[k];
+                tile[slot] = v[k];
+                if (kWhole) {
+                    slot += L / MM;
+                } else {
+                    pp += rd;
+                    slot += step;
+                    if (pp >= N) { pp -= N; slot += wrapfix; }
                 }
-                t.next(qd, rd, N);
             }
         }
-        if (!PER) {  // halo rows: zeros (only end rows reach past the ends, with zero weight)
+        if (PER) {
+            if (hslot >= 0) tile[hslot] = hv;
+            for (int e = tid + nthr; e < 2 * H * L; e += nthr) {  // lines shorter than 2H chunks
+                const int r = e / L, ll = e - r * L;
+                tile[(r < H ? r : N + r) * LS + ll] = bload(rin, (unsigned)(ll * N + (r < H ? N - H + r : r - H)) * 8u, 0);
+            }
+        } else {  // halo rows: zeros (only end rows reach past the ends, with zero weight)
             for (int e = tid; e < 2 * H * L; e += nthr) {
                 const int r = e / L, ll = e - r * L;
                 tile[(r < H ? r : N + r) * LS + ll] = 0.0;
@@ -327,26 +379,7 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
     if (W > 0) {
         const double *myred = red + l * (NI + 1);
         double s[2 * W > 0 ? 2 * W : 1];
-#pragma unroll
-        for (int j = 0; j < 2 * W; ++j) s[j] = 0.0;
-        if (PER) {
-            // periodic chunks are identical up to rotation: G_c[j][u] = G_0[j][u - 2Wc]
-            const int rot = c * 2 * W;
-            for (int v = 0; v < NI; ++v) {
-                int u = v + rot;
-                u = u >= NI ? u - NI : u;
-                const double yv = myred[u];
-#pragma unroll
-                for (int j = 0; j < 2 * W; ++j) s[j] += op.G[j * NI + v] * yv;
-            }
-        } else {
-            const double *G = op.G + (size_t)c * 2 * W * NI;
-            for (int u = 0; u < NI; ++u) {
-                const double yv = myred[u];
-#pragma unroll
-                for (int j = 0; j < 2 * W; ++j) s[j] += G[j * NI + u] * yv;
-            }
-        }
+        interface_sums<W, PER>(op, c, myred, s);
         const double *sp = op.sp + (PER ? 0 : (size_t)c * MM * 2 * W);
 #pragma unroll
         for (int i = 0; i < MM; ++i) {
@@ -360,7 +393,7 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
     const bool pass = live && pass_line(op, 0, q, a.Nx, a.Ny);
     if (kShare && pass) {  // the tile no longer holds the input: reload it (nothing is stored yet)
 #pragma unroll
-        for (int i = 0; i < MM; ++i) y[i] = in[(long)q * N + cs + i];
+        for (int i = 0; i < MM; ++i) y[i] = bload(rin, (unsigned)(l * N + cs + i) * 8u, 0);
     }
     if (live && (kShare || !pass)) {
 #pragma unroll
@@ -369,26 +402,32 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
     __syncthreads();
     // store: batches of coalesced stores (and loads when accumulating)
     {
-        TileWalk t{tid / N, tid % N};
-        constexpr int kB = contig_batch<H, MM, PER>();
+        const __amdgpu_buffer_rsrc_t rout = __builtin_amdgcn_make_buffer_rsrc(
+            a.out[blockIdx.y] + (long)q0 * N, (short)0, (int)nrec, kRawBufferWord3);
+        int slot = slot0, pp = p0;
 #pragma unroll 1
         for (int k0 = 0; k0 < MM; k0 += kB) {
             double v[kB];
-            TileWalk u = t;
 #pragma unroll
             for (int k = 0; k < kB; ++k) {
-                v[k] = tile[(u.p + H) * LS + u.ll];
-                u.next(qd, rd, N);
-            }
-#pragma unroll
-            for (int k = 0; k < kB; ++k) {
-                const int qq = q0 + t.ll;
-                if (qq < a.nlines) {
-                    double *g = out + (long)qq * N + t.p;
-                    __builtin_nontemporal_store(a.accumulate ? __builtin_nontemporal_load(g) + v[k] : v[k], g);
+                v[k] = tile[slot];
+                if (kWhole) {
+                    slot += L / MM;
+                } else {
+                    pp += rd;
+                    slot += step;
+                    if (pp >= N) { pp -= N; slot += wrapfix; }
                 }
-                t.next(qd, rd, N);
             }
+            if (a.accumulate) {
+                double o[kB];
+#pragma unroll
+                for (int k = 0; k < kB; ++k) o[k] = bload_nt(rout, (unsigned)(tid + (k0 + k) * nthr) * 8u, 0);
+#pragma unroll
+                for (int k = 0; k < kB; ++k) v[k] = o[k] + v[k];
+            }
+#pragma unroll
+            for (int k = 0; k < kB; ++k) bstore(v[k], rout, (unsigned)(tid + (k0 + k) * nthr) * 8u, 0);
         }
     }
 }

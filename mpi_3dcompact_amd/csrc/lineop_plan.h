@@ -35,6 +35,10 @@ struct Plan {
     std::vector<double> int_c;
     int nbl = 0, nbr = 0;
     std::vector<double> rows, lu, sp, G;
+    // interface band: every entry of G_c outside the column blocks of chunks
+    // c - gb .. c + gb (cyclically on periodic lines) is below 1e-17 of the
+    // largest entry of G_c, so the fast kernels skip those blocks
+    int gb = 0;
     // special rows as dense 16-wide rows: {order (+1 increasing, -1 decreasing), w[16]};
     // left rows over phi[0..15], right rows over phi[N-16..N-1]
     std::vector<double> drows;

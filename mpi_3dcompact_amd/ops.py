@@ -106,10 +106,10 @@ class LineOp:
 
 def plan_tables(cls, kind, bc, n, dd=1.0, alpha_f=0.0, mm=0):
     """Host-built partition tables of an operator (diagnostic; no GPU needed)."""
-    sizes = (ctypes.c_int * 10)()
+    sizes = (ctypes.c_int * 11)()
     check(lib().mc3d_plan_tables(cls, kind, bc, n, dd, alpha_f, mm, sizes, None, None, None, None,
                                  None, None, None, None))
-    N, W, H, P, NI, nbl, nbr, int_off, MM, dense_ok = list(sizes)
+    N, W, H, P, NI, nbl, nbr, int_off, MM, dense_ok, gb = list(sizes)
     W2 = max(2 * W, 1)
     nrow = max(nbl + nbr, 1)
     lu = np.zeros(P * MM * 5)
@@ -128,7 +128,7 @@ def plan_tables(cls, kind, bc, n, dd=1.0, alpha_f=0.0, mm=0):
                                  interior.ctypes.data_as(D),
                                  cstart.ctypes.data_as(I), clen.ctypes.data_as(I)))
     return dict(N=N, W=W, H=H, P=P, NI=NI, nbl=nbl, nbr=nbr, int_off=int_off, MM=MM,
-                dense_ok=bool(dense_ok),
+                dense_ok=bool(dense_ok), gb=gb,
                 lu=lu.reshape(P, MM, 5), sp=sp.reshape(P, MM, W2), G=G.reshape(P, W2, max(NI, 1)),
                 rows=rows.reshape(-1, 19), drows=drows.reshape(-1, 17), interior=interior,
                 cstart=cstart, clen=clen, periodic=(bc == MC3D_PERIODIC))

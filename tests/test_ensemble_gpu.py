@@ -159,6 +159,29 @@ def test_tuned_ensemble_within_1e12_of_standalone_after_200_steps(gpu, tmp_path)
         assert np.array_equal(e2.field(i, "rhoE"), e3.field(i, "rhoE"))
 
 
+def test_tuned_large_members_within_1e12_of_standalone_after_200_steps(gpu, tmp_path):
+    """The same bar for the workload's two largest box sizes: two 96^3 members
+    stacked in one bucket and one 128^3 member, M_t and MU_REF drawn from the
+    workload's ranges, tuned shapes and graph replays, 200 RK4 steps."""
+    r = np.random.default_rng(96128)
+    members = [dict(N=n, Mt=float(r.uniform(0.1, 0.4)), mu=float(r.uniform(1.0 / 3200.0, 1.0 / 800.0)),
+                    seed=int(r.integers(0, 2 ** 31 - 1))) for n in (96, 128, 96)]
+    _say("case: members N = 96, 128, 96, tuned, 200 steps RK4 vs standalone")
+    e = build(members, tune=True, tune_cache=tmp_path / "shapes.txt", **C3)
+    assert sorted(len(b["members"]) for b in e.info()["buckets"]) == [1, 2]
+    e.step(200)
+    worst = 0.0
+    for i, m in enumerate(members):
+        g = standalone(m, 200, **C3)
+        for f in CONS:
+            worst = max(worst, rel_l2(e.field(i, f), g.field(f)))
+        (te, dte, tse), (tg, dtg, tsg) = e.scalars(i), g.scalars()
+        assert tse == tsg == 200 and abs(dte - dtg) <= 1e-12 * dtg and abs(te - tg) <= 1e-12 * tg
+        g.close()
+    _say(f"  worst rel L2 over members and fields: {worst:.2e}")
+    assert worst <= 1e-12
+
+
 def test_ensemble_refuses_what_it_does_not_batch(gpu):
     from mpi_3dcompact_amd._lib import Mc3dError
     from mpi_3dcompact_amd.ensemble import Ensemble
