@@ -206,6 +206,7 @@ LearnedSGS::LearnedSGS(CurvilinearCSolver *c) {
         line_flag_ = dev_alloc<unsigned char>((size_t)nlines_, "closure line flags");
     }
     if (musgsAvgType == Options::LOCAL) {
+        for (auto &f : ftmp_) f = c->alloc_field();
         const int bc3[3] = {c->opt.bcXType == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET,
                             c->opt.bcYType == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET,
                             c->opt.bcZType == Options::PERIODIC_SOLVE ? MC3D_PERIODIC : MC3D_DIRICHLET};
@@ -314,7 +315,17 @@ void LearnedSGS::getSGSViscosity(const double *const dU[3], const double *const 
                       [&] { hip_check(launch_closure_apply(ap, cs->N, cs->st), "closure apply"); });
             count_blocks = line_closure_blocks(nlines_);
         } else {
-            for (int k = 0; k < kMlpIn; ++k) doAveraging(feat_[k], feat_[k]);
+            if (musgsAvgType == Options::LOCAL) {
+                // doAveraging's test filter (Y, then Z in place, then X) of
+                // every feature, one launch per direction
+                const double *in[kMlpIn], *tc[kMlpIn];
+                for (int k = 0; k < kMlpIn; ++k) in[k] = feat_[k], tc[k] = ftmp_[k];
+                filtY->filterFields(kMlpIn, in, ftmp_);
+                filtZ->filterFields(kMlpIn, tc, ftmp_);
+                filtX->filterFields(kMlpIn, tc, feat_);
+            } else {
+                for (int k = 0; k < kMlpIn; ++k) doAveraging(feat_[k], feat_[k]);
+            }
             for (int k = 0; k < kMlpIn; ++k) a.feat[k] = feat_[k];
             a.smag = smag_;
             a.delta2 = delta2_;

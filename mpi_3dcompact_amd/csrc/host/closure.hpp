@@ -80,6 +80,8 @@ class LearnedSGS : public AbstractSGS {
     // averaged raw features (LESAVERAGING other than NONE), with |S| and Delta^2 per point
     double *feat_[kMlpIn] = {};
     double *smag_ = nullptr, *delta2_ = nullptr;
+    // LOCAL: temporaries of the batched test filter of the features
+    double *ftmp_[kMlpIn] = {};
     // XI1 / XI2 / XI3: the network per line (line means, coefficients, flags)
     int line_dir_ = -1;
     long nlines_ = 0;

@@ -76,16 +76,22 @@ __device__ __forceinline__ void chunk_rhs(const OpDev &op, const double *w, doub
 }
 
 // banded LU solve of one chunk block with the factor rows lu[MM][kLuRec]
+// The factor rows are read with scalar loads; a scheduling fence every eight
+// rows keeps the scheduler from hoisting every row's loads to the top, where
+// they outgrow the scalar registers and are spilled to vector-register lanes
+// (one v_writelane per dword, one v_readlane per dword at each use).
 template <int W, int MM>
 __device__ __forceinline__ void chunk_solve(const double *lu, double *y) {
 #pragma unroll
     for (int i = 1; i < MM; ++i) {
+        if (i % 8 == 0) __builtin_amdgcn_sched_barrier(0);
         double z = y[i] - lu[i * kLuRec + 0] * y[i - 1];
         if (W == 2 && i >= 2) z -= lu[i * kLuRec + 1] * y[i - 2];
         y[i] = z;
     }
 #pragma unroll
     for (int i = MM - 1; i >= 0; --i) {
+        if (i % 8 == 7) __builtin_amdgcn_sched_barrier(0);
         double x = y[i];
         if (i + 1 < MM) x -= lu[i * kLuRec + 2] * y[i + 1];
         if (W == 2 && i + 2 < MM) x -= lu[i * kLuRec + 3] * y[i + 2];
@@ -236,6 +242,7 @@ __global__ __launch_bounds__(1024) void lineop_strided(OpDev op, LaunchArgs a) {
         const double *sp = op.sp + (PER ? 0 : (size_t)c * MM * 2 * W);
 #pragma unroll
         for (int i = 0; i < MM; ++i) {
+            if (i % 8 == 0) __builtin_amdgcn_sched_barrier(0);
             double x = y[i];
 #pragma unroll
             for (int j = 0; j < 2 * W; ++j) x -= sp[i * 2 * W + j] * s[j];
@@ -261,10 +268,13 @@ __global__ __launch_bounds__(1024) void lineop_strided(OpDev op, LaunchArgs a) {
 // table read is chunk independent, so a wave may span chunks; the smaller tile
 // lets four workgroups of 256-point lines share a CU, which overlaps one
 // workgroup's staging with another's solve: 32 lines measured 4-14% slower,
-// 8 lines mixed), 64 otherwise (a wave is then 64 lines x one chunk, so the
-// end-row branches are uniform)
-template <bool PER>
-constexpr int contig_lines() { return PER ? 16 : 64; }
+// 8 lines mixed) and for explicit operators (no tables: Dirichlet CD2 and
+// CommN5Exp x 4.7 -> 5.7-5.8 TB/s with 16 lines), 64 for banded Dirichlet
+// ones (a wave is then 64 lines x one chunk, so the chunk's factor tables are
+// scalar loads and the end-row branches uniform; with 16 lines the per-lane
+// table loads made Penta10 x twice slower)
+template <bool PER, int W>
+constexpr int contig_lines() { return PER || W == 0 ? 16 : 64; }
 
 // loads in flight per thread while staging.  The tile caps a CU at one or two
 // workgroups, so the fill is latency bound: 16 at a time where the registers
@@ -279,7 +289,7 @@ constexpr int contig_batch() {
 template <int W, int H, int MM, bool PER>
 __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
     extern __shared__ double smem[];
-    constexpr int L = contig_lines<PER>();
+    constexpr int L = contig_lines<PER, W>();
     constexpr int LS = L + 1;  // row stride of the transposed tile
     const int N = op.N, P = op.P, NI = op.NI;
     double *tile = smem;  // [N + 2H][LS], row e holds phi[e - H]
@@ -354,7 +364,7 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
     }
     __syncthreads();
     const int l = threadIdx.x;
-    const int c = PER ? (int)threadIdx.y : __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int c = L < 64 ? (int)threadIdx.y : __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int q = q0 + l;
     const bool live = q < a.nlines;
     const int cs = c * MM;
@@ -383,6 +393,7 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
         const double *sp = op.sp + (PER ? 0 : (size_t)c * MM * 2 * W);
 #pragma unroll
         for (int i = 0; i < MM; ++i) {
+            if (i % 8 == 0) __builtin_amdgcn_sched_barrier(0);
             double x = y[i];
 #pragma unroll
             for (int j = 0; j < 2 * W; ++j) x -= sp[i * 2 * W + j] * s[j];
@@ -435,7 +446,7 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
 template <int W, int H, int MM, bool PER>
 hipError_t launch_fast(const OpDev &op, const LaunchArgs &a, int nfields, hipStream_t st) {
     if (op.dir == 0) {
-        constexpr int L = contig_lines<PER>();
+        constexpr int L = contig_lines<PER, W>();
         dim3 grid((a.nlines + L - 1) / L, nfields), block(L, op.P);
         hipLaunchKernelGGL((lineop_contig<W, H, MM, PER>), grid, block, lineop_fast_lds_bytes(op), st, op, a);
     } else {
@@ -472,7 +483,7 @@ hipError_t launch_fast_mm(const OpDev &op, const LaunchArgs &a, int nfields, hip
 
 size_t lineop_fast_lds_bytes(const OpDev &op) {
     if (op.dir == 0) {  // the tile, with the interface values inside it (W = 2) or after it
-        const size_t L = op.periodic ? contig_lines<true>() : contig_lines<false>();
+        const size_t L = op.periodic || op.W == 0 ? 16 : contig_lines<false, 1>();
         const size_t tile = (size_t)(op.N + 2 * op.H) * (L + 1), red = L * (op.NI + 1);
         return sizeof(double) * (op.W == 2 ? std::max(tile, red) : tile + red);
     }
@@ -484,7 +495,7 @@ bool lineop_fast_supported(const OpDev &op, size_t lds_limit) {
     const bool pair = (w == 0 && (h == 1 || h == 3)) || (w == 1 && (h == 2 || h == 4 || h == 5)) ||
                       (w == 2 && (h == 3 || h == 5));
     // workgroup size as launched: (lines per workgroup) x P
-    const int lines = op.dir != 0 ? 64 : op.periodic ? contig_lines<true>() : contig_lines<false>();
+    const int lines = op.dir != 0 ? 64 : op.periodic || op.W == 0 ? 16 : contig_lines<false, 1>();
     const int threads = lines * op.P;
     // Dirichlet interior rows must not reach past the line ends
     const bool ends = op.periodic || (op.nbl >= h && op.nbr >= h && op.nbl <= kSpecRows && op.nbr <= kSpecRows);

@@ -9,8 +9,9 @@ The spectrum is E(k) = (k/k0)^4 exp(-2 (k/k0)^2).  Phases come from numpy's
 PCG64 generator with an explicit seed (the Fortran LCG shuffle is not
 reproduced).  The field is scaled so that the turbulent Mach number
 M_t = sqrt(<u.u>)/<c> equals the requested value (the definition of
-tests/hit/Samtaney_D4/MPI_3DCompact.cpp:455,485-486,535-537), with rho = 1,
-p = 1/gamma (so T = 1 and c = 1).
+tests/hit/Samtaney_D4/MPI_3DCompact.cpp:455,485-486,535-537), with rho = 1 and
+p = c0^2/gamma: c0 = 1 by default (T = 1), c0^2 = 11.11111111 in the
+reference's Samtaney D4 case (tests/hit/Samtaney_D4/MPI_3DCompact.cpp:230-231).
 
 The mesh is the HIT identity map x = 2 pi xi (tests/hit/Samtaney_D4/
 AlgebraicSingleBlockMesh.hpp:151-156), whose conservative metrics are
@@ -21,8 +22,9 @@ import numpy as np
 GAMMA = 1.4
 
 
-def hit_velocity(N, k0=4.0, Mt=0.3, seed=0):
-    """Solenoidal random velocity (U, V, W), each of shape (N, N, N) indexed [k][j][i]."""
+def hit_velocity(N, k0=4.0, Mt=0.3, seed=0, c0=1.0):
+    """Solenoidal random velocity (U, V, W), each of shape (N, N, N) indexed
+    [k][j][i], scaled to M_t at the uniform sound speed c0."""
     rng = np.random.default_rng(seed)
     kf = np.fft.fftfreq(N, d=1.0 / N)          # integer wavenumbers, FFT order
     kr = np.fft.rfftfreq(N, d=1.0 / N)
@@ -55,14 +57,14 @@ def hit_velocity(N, k0=4.0, Mt=0.3, seed=0):
         out.append(np.fft.irfftn(uk, s=(N, N, N), axes=(0, 1, 2)))
     U, V, W = out
     urms = np.sqrt(np.mean(U ** 2 + V ** 2 + W ** 2))
-    scale = Mt * 1.0 / urms  # mean speed of sound is 1 at rho = 1, p = 1/gamma
+    scale = Mt * c0 / urms  # the mean speed of sound is c0
     return U * scale, V * scale, W * scale
 
 
-def hit_initial_condition(N, k0=4.0, Mt=0.3, seed=0):
-    U, V, W = hit_velocity(N, k0, Mt, seed)
+def hit_initial_condition(N, k0=4.0, Mt=0.3, seed=0, c0=1.0):
+    U, V, W = hit_velocity(N, k0, Mt, seed, c0)
     rho = np.ones((N, N, N))
-    p = np.full((N, N, N), 1.0 / GAMMA)
+    p = np.full((N, N, N), c0 * c0 / GAMMA)
     return dict(rho=rho, U=U, V=V, W=W, p=p)
 
 
