@@ -73,9 +73,11 @@ def measured_traffic(kernel, n):
 
 
 # the learned-closure workload: DSM LES (PENTA10, COMPACT8, KENRK4, Pade test
-# filter, tau_kk on, LESAVERAGING LOCAL) against the learned closure with the
-# same averaging and its DSM fallback, mu_ref 1e-4, M_t 0.3, from one initial field
-CLOSURE_NET = os.path.join(ROOT, "mpi_3dcompact_amd", "networks", "hit256_local.net")
+# filter, tau_kk on, LESAVERAGING LOCAL) against the learned closure (point
+# features, CLOSUREFEATURES POINT, trained on that DSM's coefficients) with its
+# LOCAL-averaged DSM fallback, mu_ref 1e-4, M_t 0.3, from one initial field
+CLOSURE_NET = os.path.join(ROOT, "mpi_3dcompact_amd", "networks", "hit256_local_point.net")
+CLOSURE_FEATURES = "POINT"
 
 
 def learned_closure_study():
@@ -84,14 +86,15 @@ def learned_closure_study():
     pairs and the online runs take minutes, so they are not repeated here),
     each labelled with its LESAVERAGING and the steps compared."""
     out = {}
-    for tag in ("256_local", "256_xi1", "64_xi1"):
+    for tag in ("256_local_point", "256_local", "256_xi1", "64_xi1"):
         path = os.path.join(ROOT, "profiles", f"closure_study_{tag}.json")
         try:
             with open(path) as f:
                 d = json.load(f)
         except (OSError, ValueError):
             continue
-        out[tag] = {"averaging": d.get("avg", "XI1_AVG"), "online_steps": d["online_steps"],
+        out[tag] = {"averaging": d.get("avg", "XI1_AVG"), "features": d.get("features", "AVERAGED"),
+                    "online_steps": d["online_steps"],
                     "compared_steps": d.get("compared_steps"), "blowup_step": d.get("blowup_step"),
                     "speedup_vs_dsm": round(d["speedup_vs_dsm"], 3),
                     "learned_vs_dsm": d["learned_vs_dsm"], "constant_vs_dsm": d.get("constant_vs_dsm"),
@@ -100,7 +103,7 @@ def learned_closure_study():
     return out or None
 
 
-def closure_leg(N, steps, warmup, net_path=CLOSURE_NET):
+def closure_leg(N, steps, warmup, net_path=CLOSURE_NET, features=CLOSURE_FEATURES):
     """Step time of the DSM and of the learned closure on the same N^3 initial
     field (each its own solver, `warmup` untimed steps, then `steps` timed with
     the stream synchronised around them), the resolved KE and dissipation of
@@ -115,8 +118,8 @@ def closure_leg(N, steps, warmup, net_path=CLOSURE_NET):
     res = {}
     for model in ("DSM", "LEARNED"):
         log(f"[bench] closure leg: {model} at {N}^3")
-        g = Solver(case(N, model, 1e-4, "LOCAL", closure_file=net_path if model == "LEARNED" else None),
-                   (N, N, N))
+        g = Solver(case(N, model, 1e-4, "LOCAL", closure_file=net_path if model == "LEARNED" else None,
+                        features=features if model == "LEARNED" else None), (N, N, N))
         g.set_metrics(met)
         g.set_initial(ic)
         g.step(warmup)
@@ -133,7 +136,7 @@ def closure_leg(N, steps, warmup, net_path=CLOSURE_NET):
     d, l = res["DSM"], res["LEARNED"]
     return {"workload": f"decaying HIT {N}^3, PENTA10 + COMPACT8 (alpha_f 0.49) + KENRK4 (CFL 0.5), mu 1e-4, "
                         "M_t 0.3; DSM (Pade test filter, tau_kk, LESAVERAGING LOCAL) vs the learned closure "
-                        "(same averaging, DSM fallback on), network "
+                        f"(CLOSUREFEATURES {features}, LOCAL-averaged DSM fallback on), network "
                         + os.path.relpath(net_path, ROOT),
             "steps": steps, "warmup": warmup,
             "ms_per_step": {"DSM": round(d["ms_per_step"], 3), "LEARNED": round(l["ms_per_step"], 3)},
@@ -332,6 +335,9 @@ def main():
     ap.add_argument("--no-closure", action="store_true", help="skip the learned-closure timing leg")
     ap.add_argument("--closure-only", action="store_true", help="run the learned-closure timing leg alone")
     ap.add_argument("--closure-steps", type=int, default=5)
+    ap.add_argument("--closure-net", default=CLOSURE_NET, help="network file of the learned-closure leg")
+    ap.add_argument("--closure-features", default=CLOSURE_FEATURES, choices=("AVERAGED", "POINT"),
+                    help="CLOSUREFEATURES the network was trained with")
     ap.add_argument("--ensemble-only", action="store_true", help="run the ensemble workload alone")
     ap.add_argument("--ensemble-steps", type=int, default=20)
     ap.add_argument("--ensemble-cache", default=None, help="line-operator tuning cache file")
@@ -348,7 +354,8 @@ def main():
     from mpi_3dcompact_amd.solver import Solver
 
     if args.closure_only:
-        print(json.dumps({"learned_closure": closure_leg(args.n, args.closure_steps, 2)}), flush=True)
+        print(json.dumps({"learned_closure": closure_leg(args.n, args.closure_steps, 2, args.closure_net,
+                                                         args.closure_features)}), flush=True)
         return
     if args.ensemble_only:
         print(json.dumps({"ensemble": ensemble_leg(args.ensemble_steps, 4, cache=args.ensemble_cache,
@@ -425,7 +432,8 @@ def main():
     }
     if cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 1)
-    lc = {} if args.no_closure else {"timed": closure_leg(N, args.closure_steps, 2)}
+    lc = {} if args.no_closure else {"timed": closure_leg(N, args.closure_steps, 2, args.closure_net,
+                                                          args.closure_features)}
     lc["studies"] = learned_closure_study()
     out["learned_closure"] = lc
     if not args.no_ensemble:

@@ -6,7 +6,8 @@
    DSMUSETAUKK, DSMCOEFFRANGE) of decaying HIT from a training seed; at evenly
    spaced snapshots from step 0 to --gen-steps the DSM's coefficients
    (C, C / Pr_t) and the closure features of strided points are sampled on the
-   GPU, the features averaged as the DSM averages its contractions.
+   GPU, the features averaged as the DSM averages its contractions, or taken
+   at each point (--features POINT).
 2. Standardisation from the samples (targets winsorised at the 0.1 / 99.9
    percentiles), the trusted input range = the sampled range widened by 10%
    of its span, then Adam training with the library's BF16-MFMA kernels.
@@ -35,14 +36,14 @@ def _say(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def case(N, model, mu_ref, avg="LOCAL", closure_file=None):
+def case(N, model, mu_ref, avg="LOCAL", closure_file=None, features=None):
     return hit.options_text(N, fd="PENTA10", filt="COMPACT8", alpha_f=0.49, rk="KENRK4", cfl=0.5, mu_ref=mu_ref,
                             les=model, les_avg=avg, dsm_filter=1, use_taukk=True,
-                            coeff_range=(model == "DSM"), closure_file=closure_file)
+                            coeff_range=(model == "DSM"), closure_file=closure_file, closure_features=features)
 
 
-def make_solver(N, model, mu_ref, seed, k0, Mt, avg="LOCAL"):
-    g = Solver(case(N, model, mu_ref, avg), (N, N, N))
+def make_solver(N, model, mu_ref, seed, k0, Mt, avg="LOCAL", features=None):
+    g = Solver(case(N, model, mu_ref, avg, features=features), (N, N, N))
     g.set_metrics(hit.identity_metrics(N))
     g.set_initial(hit.hit_initial_condition(N, k0=k0, Mt=Mt, seed=seed))
     return g
@@ -51,7 +52,7 @@ def make_solver(N, model, mu_ref, seed, k0, Mt, avg="LOCAL"):
 def generate(args, torch):
     N = args.N
     npts = N ** 3
-    g = make_solver(N, "DSM", args.mu, args.train_seed, args.k0, args.Mt, args.avg)
+    g = make_solver(N, "DSM", args.mu, args.train_seed, args.k0, args.Mt, args.avg, args.features)
     # snapshots at steps 0, every, ..., gen_steps: the pairs span the decay the
     # closure is used over (the online runs start from another realisation)
     nsnap = args.snapshots + 1
@@ -102,14 +103,14 @@ def standardise(X, T, torch):
 
 def online(args, model, net, torch):
     N = args.N
-    g = Solver(case(N, model, args.mu, args.avg), (N, N, N))
+    g = Solver(case(N, model, args.mu, args.avg, features=args.features), (N, N, N))
     g.set_metrics(hit.identity_metrics(N))
     g.set_initial(hit.hit_initial_condition(N, k0=args.k0, Mt=args.Mt, seed=args.eval_seed))
     if net is not None:
         g.set_closure(net)
     hist, steps, t_run, blowup = [], 0, 0.0, None
     st = g.hit_stats()
-    hist.append((0, st["time"], 0.5 * st["rho_uu"], st["dissipation"]))
+    hist.append((0, st["time"], 0.5 * st["rho_uu"], st["dissipation"], 0.0))
     while steps < args.online_steps:
         n = min(args.stats_every, args.online_steps - steps)
         g.sync()
@@ -125,7 +126,8 @@ def online(args, model, net, torch):
             _say(f"[closure] {model}: non-finite statistics at step {steps}; the run ends there")
             blowup = steps
             break
-        hist.append((steps, st["time"], 0.5 * st["rho_uu"], st["dissipation"]))
+        # (step, time, KE, dissipation, wall ms per step so far)
+        hist.append((steps, st["time"], 0.5 * st["rho_uu"], st["dissipation"], 1e3 * t_run / steps))
         if steps % (10 * args.stats_every) == 0 or steps == args.online_steps:
             _say(f"[closure] {model}{'' if net is None else '(' + net_label(net) + ')'} step {steps}: "
                  f"KE {hist[-1][2]:.6e} eps {hist[-1][3]:.6e} {1e3 * t_run / steps:.2f} ms/step")
@@ -156,6 +158,9 @@ def main(argv=None):
     ap.add_argument("--k0", type=float, default=4.0)
     ap.add_argument("--avg", default="LOCAL", help="LESAVERAGING of the DSM (and of its fallback); the "
                     "LOCAL-averaged DSM diverges after a few hundred steps at 256^3 (XI1_AVG runs 1000)")
+    ap.add_argument("--features", choices=("AVERAGED", "POINT"), default=None,
+                    help="CLOSUREFEATURES: the network's inputs averaged as LESAVERAGING averages the DSM's "
+                         "contractions (default), or taken at each point")
     ap.add_argument("--train-seed", type=int, default=1)
     ap.add_argument("--eval-seed", type=int, default=0)
     ap.add_argument("--gen-steps", type=int, default=1000, help="DSM steps of the training run (sampled at 0, "
@@ -211,7 +216,7 @@ def main(argv=None):
     if not args.skip_baselines:
         runs["CONSTANT"] = online(args, "LEARNED", const, torch)
         runs["NOMODEL"] = online(args, "NOMODEL", None, torch)
-    res = dict(N=args.N, avg=args.avg, mu=args.mu, Mt=args.Mt, k0=args.k0, gen_steps=args.gen_steps,
+    res = dict(N=args.N, avg=args.avg, features=args.features or "AVERAGED", mu=args.mu, Mt=args.Mt, k0=args.k0, gen_steps=args.gen_steps,
                train_seed=args.train_seed, eval_seed=args.eval_seed, samples=n, epochs=args.epochs, batch=args.batch, lr=args.lr, train_seconds=t_tr,
                epoch_losses=losses, heldout_mse=mse.tolist(), scaling={k: v.tolist() for k, v in sc.items()},
                online_steps=args.online_steps, ms_per_step={k: v["ms_per_step"] for k, v in runs.items()},
@@ -221,6 +226,12 @@ def main(argv=None):
             ke, eps = errors(v, runs["DSM"])
             res[f"{k.lower()}_vs_dsm"] = dict(ke_max_rel=ke, dissipation_max_rel=eps)
     res["speedup_vs_dsm"] = runs["DSM"]["ms_per_step"] / runs["LEARNED"]["ms_per_step"]
+    # over the steps the DSM run reached (its LOCAL-averaged form can diverge first)
+    last = runs["DSM"]["hist"][-1][0]
+    lw = [h for h in runs["LEARNED"]["hist"] if h[0] <= last]
+    dms, lms = runs["DSM"]["hist"][-1][4], lw[-1][4]
+    res["compared_window"] = dict(steps=last, dsm_ms_per_step=dms, learned_ms_per_step=lms,
+                                  speedup_vs_dsm=dms / lms if lms > 0 else None)
     res["blowup_step"] = {k: v["blowup_step"] for k, v in runs.items()}
     res["compared_steps"] = {k: v["hist"][min(len(v["hist"]), len(runs["DSM"]["hist"])) - 1][0]
                              for k, v in runs.items() if k != "DSM"}

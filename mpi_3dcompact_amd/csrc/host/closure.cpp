@@ -188,7 +188,9 @@ void Closure::load(const std::string &path) {
 LearnedSGS::LearnedSGS(CurvilinearCSolver *c) {
     cs = c;
     fallback = c->opt.closureFallback;
-    musgsAvgType = c->opt.musgsAvgType;
+    // CLOSUREFEATURES POINT: the network reads the features at each point
+    // whatever LESAVERAGING says (which still sets the fallback DSM's averaging)
+    musgsAvgType = c->opt.closurePointFeatures ? Options::NONE : c->opt.musgsAvgType;
     mu_sgs = c->alloc_field();
     k_sgs = c->alloc_field();
     if (musgsAvgType != Options::NONE) {

@@ -237,6 +237,8 @@ int Options::parse(const std::string &text, Options &o, std::string &err) {
     r.opt("LES.CLOSUREFILE", o.closureFile);
     r.opt("LES.CLOSUREFALLBACK", o.closureFallback);
     r.opt("LES.CLOSUREFALLBACKFRACTION", o.closureFallbackFraction);
+    if (ini.has("LES.CLOSUREFEATURES"))
+        r.enm("LES.CLOSUREFEATURES", o.closurePointFeatures, {{"AVERAGED", false}, {"POINT", true}});
     if (!r.ok) return MC3D_ECONFIG;
     // src/Options.hpp:241 (checked after the sizes are known; with a restart
     // they come from the file header and are checked when it is read)
@@ -323,6 +325,7 @@ std::string Options::describe() const {
     put("LES.DSMCOEFFRANGE", b(dumpCoeffRange));
     put("LES.CLOSUREFILE", closureFile);
     put("LES.CLOSUREFALLBACK", b(closureFallback));
+    put("LES.CLOSUREFEATURES", closurePointFeatures ? "POINT" : "AVERAGED");
     return o;
 }
 

@@ -66,6 +66,10 @@ struct Options {
     // by mc3d_closure_save, and whether out-of-range points fall back to the DSM
     std::string closureFile;
     bool closureFallback = true;
+    // the network's inputs: AVERAGED as LESAVERAGING averages the DSM's
+    // contractions, or POINT (the features at each point, whatever the averaging
+    // of the DSM the network replaces and falls back to)
+    bool closurePointFeatures = false;
     double closureFallbackFraction = 1e-3;  // DSM stage when more points than this leave the range
 
     // Parses INI text.  Returns 0, or a negative code with a message.

@@ -929,7 +929,7 @@ void CurvilinearCSolver::sampleClosure(float *X, float *T, long count, long offs
     sa.C = les->C;
     sa.Prt = les->Prt;
     sa.vol = dom.dx * dom.dy * dom.dz;
-    if (opt.musgsAvgType != Options::NONE) {
+    if (opt.musgsAvgType != Options::NONE && !opt.closurePointFeatures) {
         // the features averaged as the DSM averaged its contractions, in DSM work fields
         DSMSGS *d = static_cast<DSMSGS *>(les.get());
         FeatureArgs fa{};

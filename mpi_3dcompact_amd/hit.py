@@ -84,7 +84,7 @@ def options_text(N, fd="PADE6", filt="COMPACT10", alpha_f=0.49, rk="TVDRK3", cfl
                  coeff_range=False, ts_type="CONST_CFL", dt=None, restart_file=None,
                  dump_step=1000000, disp=None, closure_file=None, closure_fallback=True,
                  faces=None, sponge=None, closure_fallback_fraction=None, stats=None, dims=None,
-                 sponge_avg_file=None):
+                 sponge_avg_file=None, closure_features=None):
     """MPI_3DCompact.in text for a periodic HIT box of N^3 points (restart_file:
     start from that SolutionDump, which also sets the grid; disp: 3x3
     PERIODICDISP rows, default the 2 pi box; stats: dict velocity, thermo (bools),
@@ -141,7 +141,8 @@ def options_text(N, fd="PADE6", filt="COMPACT10", alpha_f=0.49, rk="TVDRK3", cfl
         f"DSMCOEFFRANGE = {'TRUE' if coeff_range else 'FALSE'}",
         f"CLOSUREFALLBACK = {'TRUE' if closure_fallback else 'FALSE'}",
     ] + ([f"CLOSUREFILE = {closure_file}"] if closure_file else []) + (
-        [f"CLOSUREFALLBACKFRACTION = {closure_fallback_fraction!r}"] if closure_fallback_fraction is not None else [])
+        [f"CLOSUREFALLBACKFRACTION = {closure_fallback_fraction!r}"] if closure_fallback_fraction is not None else []) + (
+        [f"CLOSUREFEATURES = {closure_features}"] if closure_features else [])
     return "\n".join(lines) + "\n"
 
 

@@ -203,6 +203,27 @@ def test_solver_closure_with_averaged_inputs_matches_oracle(torch, avg):
     assert np.allclose(coef, np.take(coef, [0], axis=axis), rtol=1e-12, atol=0)
 
 
+def test_point_features_ignore_the_averaging(torch):
+    """CLOSUREFEATURES POINT: under LESAVERAGING LOCAL the network reads the
+    features at each point, so mu_sgs and k_sgs equal those of LESAVERAGING
+    NONE bit for bit (the same kernel on the same values); LESAVERAGING still
+    sets the fallback DSM's averaging."""
+    from mpi_3dcompact_amd.solver import Solver
+    N = 32
+    c, _, _ = _scaled_closure(24)
+    out = {}
+    for avg, feats in (("NONE", None), ("LOCAL", "POINT")):
+        g = Solver(hit.options_text(N, fd="PADE6", filt="COMPACT10", rk="RK4", les="LEARNED", les_avg=avg,
+                                    closure_fallback=False, closure_features=feats), (N, N, N))
+        g.set_metrics(hit.identity_metrics(N))
+        g.set_initial(hit.hit_initial_condition(N, k0=4.0, Mt=0.3, seed=5))
+        g.set_closure(c)
+        g.eval_sgs()
+        out[avg] = (g.field("mu_sgs"), g.field("k_sgs"))
+        g.close()
+    assert np.array_equal(out["NONE"][0], out["LOCAL"][0]) and np.array_equal(out["NONE"][1], out["LOCAL"][1])
+
+
 def test_solver_closure_steps_and_falls_back(torch):
     """A narrow trusted range sends the stages to the DSM fallback; with the
     range wide open no stage falls back; with a fallback fraction of 1 the

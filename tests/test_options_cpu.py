@@ -179,3 +179,13 @@ def test_options_text_of_the_hit_cases_parses():
                                                                            dt=1e-3)):
         v = options_check(hit.options_text(64, **kw))
         assert v["DOMAIN.NX"] == "64"
+
+
+def test_closure_features_option():
+    """CLOSUREFEATURES (no counterpart in the reference): AVERAGED when absent,
+    AVERAGED or POINT when given, anything else refused."""
+    d = deck() + LES.replace("NONE\nLESAV", "LEARNED\nLESAV")
+    assert options_check(d)["LES.CLOSUREFEATURES"] == "AVERAGED"
+    for v in ("AVERAGED", "POINT"):
+        assert options_check(d + f"CLOSUREFEATURES = {v}\n")["LES.CLOSUREFEATURES"] == v
+    econfig(d + "CLOSUREFEATURES = FILTERED\n", "unknown value for LES.CLOSUREFEATURES")
