@@ -6,10 +6,11 @@
 For every scheme, direction and boundary type it applies one operator to
 ``--fields`` 256^3 FP64 fields (one batched launch) and prints the mean time
 per field and the algorithmic bandwidth (16 B per point per field: the input
-read once, the output written once).  A plain device-to-device copy of one
-field is timed first: it is the practical bandwidth reference, and its
-hardware counters (rocprofv3 FETCH_SIZE / WRITE_SIZE) calibrate those of the
-operators.
+read once, the output written once).  Plain device-to-device copies of one
+field and of ``--fields`` fields are timed first: the many-field copy is the
+practical bandwidth reference for a launch of that many fields, and the
+one-field copy's hardware counters (rocprofv3 FETCH_SIZE / WRITE_SIZE)
+calibrate those of the operators.
 """
 import argparse
 import sys
@@ -60,6 +61,15 @@ def main(argv=None):
     ms = timeit(lambda: outs[0].copy_(fs[0]))
     print(f"copy {N}^3 f64 (1 field): {ms * 1e3:8.1f} us  {16 * npts / (ms * 1e-3) / 1e9:7.0f} GB/s",
           flush=True)
+    # the same with as many fields as one operator launch streams: a single
+    # 256^3 field pair (268 MB) largely stays in the 256 MB last-level cache
+    # across repetitions, so the 1-field copy overstates what HBM delivers
+    big_in = torch.rand((nf, N, N, N), dtype=torch.float64, device="cuda")
+    big_out = torch.empty_like(big_in)
+    ms = timeit(lambda: big_out.copy_(big_in))
+    print(f"copy {N}^3 f64 ({nf} fields): {ms * 1e3 / nf:8.1f} us/field  "
+          f"{16 * npts * nf / (ms * 1e-3) / 1e9:7.0f} GB/s", flush=True)
+    del big_in, big_out
     names = [s for s in args.only.split(",") if s] or list(SCHEMES)
     bcs = [b for b in args.bc.split(",") if b]
     for bname in bcs:
