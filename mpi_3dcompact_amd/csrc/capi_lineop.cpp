@@ -132,7 +132,7 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
             d.NA = dims[da];
             d.NB = dims[db];
         }
-        const bool rows_ok = p.periodic ? (p.nbl + p.nbr == 0) : p.dense_ok;
+        const bool rows_ok = p.nbl + p.nbr == 0 || p.dense_ok;
         // the strided fast kernel addresses a field through a buffer descriptor
         // with 32-bit byte offsets
         const bool offsets_ok = npts * 8 < 2147483648L;

@@ -27,7 +27,7 @@ constexpr int kSpecRows = 5;   // max special rows at each end
 constexpr int kSpecW = 16;     // max taps of a special row
 constexpr int kRowRec = 3 + kSpecW;  // {offset, step, width, w[16]}
 constexpr int kLuRec = 5;      // {l1, l2, u1, u2, invd}
-constexpr int kDRowRec = 17;   // dense special row {order, w[16]}
+constexpr int kDRowRec = 17;   // dense special row {order, w[2H+1], zeros}
 
 enum OpClass { kDeriv = 0, kFilter = 1 };
 

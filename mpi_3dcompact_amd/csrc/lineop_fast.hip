@@ -32,46 +32,64 @@ namespace mc3d {
 
 namespace {
 
+// one dense special row: taps d[1..K] over the window points v[0..K), summed
+// in increasing (d[0] > 0) or decreasing position
+template <int K>
+__device__ __forceinline__ double dense_row(const double *d, const double *v) {
+#pragma clang fp contract(off)
+    double r = 0.0;
+    if (d[0] > 0.0) {
+#pragma unroll
+        for (int t = 0; t < K; ++t) r = r + d[1 + t] * v[t];
+    } else {
+#pragma unroll
+        for (int t = K - 1; t >= 0; --t) r = r + d[1 + t] * v[t];
+    }
+    return r;
+}
+
 // explicit stage of the MM rows of one chunk from the window w[0..MM+2H)
-// (w[j] = phi[cs - H + j])
-template <int H, int MM, bool PER>
+// (w[j] = phi[cs - H + j]).  With ROWS, the first chunk's first H rows and
+// the last chunk's last H rows are special rows of K = 2H + 1 dense taps:
+// Dirichlet left rows over phi[0..K) = w[H..H+K), right rows over
+// phi[N-K..N) = w[MM+H-K..MM+H), periodic rows (CD2's wrap rows) over their
+// own stencil w[i..i+K).  Each end of the chunk is one uniform branch between
+// straight-line blocks of H rows, so the window values either block reads
+// are live only where the interior rows read them anyway.  (Per-row branches
+// between the two stencils, or special rows computed over the interior ones
+// afterwards, cost 8-30 vector registers and a wave per SIMD.)
+template <int H, int MM, bool PER, bool ROWS>
 __device__ __forceinline__ void chunk_rhs(const OpDev &op, const double *w, double *y, bool first,
                                           bool last) {
 #pragma clang fp contract(off)
-#pragma unroll
-    for (int i = 0; i < MM; ++i) {
+    constexpr int K = 2 * H + 1;
+    auto scale = [&](double r) {
+        if (op.inv_dd != 0.0) return r * op.inv_dd;
+        if (op.divide) return r / op.dd;
+        return r;
+    };
+    auto interior = [&](int i) {
         double r = 0.0;
-        bool special = false;
-        if (!PER) {
-            if (first && i < kSpecRows && i < op.nbl) {
-                const double *d = op.drows + i * kDRowRec;
-                if (d[0] > 0.0) {
 #pragma unroll
-                    for (int t = 0; t < 16; ++t) r = r + d[1 + t] * w[H + t];
-                } else {
+        for (int k = 0; k < K; ++k) r = r + op.int_c[k] * w[i + k];
+        return scale(r);
+    };
+    if (ROWS && first) {
 #pragma unroll
-                    for (int t = 15; t >= 0; --t) r = r + d[1 + t] * w[H + t];
-                }
-                special = true;
-            } else if (last && i >= MM - kSpecRows && i >= MM - op.nbr) {
-                const double *d = op.drows + (op.nbl + i - (MM - op.nbr)) * kDRowRec;
-                if (d[0] > 0.0) {
+        for (int i = 0; i < H; ++i) y[i] = scale(dense_row<K>(op.drows + i * kDRowRec, w + (PER ? i : H)));
+    } else {
 #pragma unroll
-                    for (int t = 0; t < 16; ++t) r = r + d[1 + t] * w[H + MM - 16 + t];
-                } else {
+        for (int i = 0; i < H; ++i) y[i] = interior(i);
+    }
 #pragma unroll
-                    for (int t = 15; t >= 0; --t) r = r + d[1 + t] * w[H + MM - 16 + t];
-                }
-                special = true;
-            }
-        }
-        if (!special) {
+    for (int i = H; i < MM - H; ++i) y[i] = interior(i);
+    if (ROWS && last) {
 #pragma unroll
-            for (int k = 0; k < 2 * H + 1; ++k) r = r + op.int_c[k] * w[i + k];
-        }
-        if (op.inv_dd != 0.0) r = r * op.inv_dd;
-        else if (op.divide) r = r / op.dd;
-        y[i] = r;
+        for (int i = MM - H; i < MM; ++i)
+            y[i] = scale(dense_row<K>(op.drows + (i - MM + 2 * H) * kDRowRec, w + (PER ? i : MM + H - K)));
+    } else {
+#pragma unroll
+        for (int i = MM - H; i < MM; ++i) y[i] = interior(i);
     }
 }
 
@@ -169,6 +187,11 @@ __device__ __forceinline__ bool pass_line(const OpDev &op, int dir, int q, int N
     return (op.passA && (ia == 0 || ia == op.NA - 1)) || (op.passB && (ib == 0 || ib == op.NB - 1));
 }
 
+// special end rows: every Dirichlet line; of the periodic operators only CD2
+// (W = 0, H = 1), whose wrap rows carry the opposite sign of its interior
+template <int W, int H, bool PER>
+constexpr bool has_rows() { return !PER || (W == 0 && H == 1); }
+
 template <int W, int H, int MM, bool PER>
 __global__ __launch_bounds__(1024) void lineop_strided(OpDev op, LaunchArgs a) {
     extern __shared__ double red[];
@@ -223,7 +246,7 @@ __global__ __launch_bounds__(1024) void lineop_strided(OpDev op, LaunchArgs a) {
     // issue the window loads before any table load or arithmetic
     __builtin_amdgcn_sched_barrier(0);
     double y[MM];
-    chunk_rhs<H, MM, PER>(op, w, y, c == 0, c == P - 1);
+    chunk_rhs<H, MM, PER, has_rows<W, H, PER>()>(op, w, y, c == 0, c == P - 1);
     if (W > 0) {
         chunk_solve<W, MM>(op.lu + (PER ? 0 : (size_t)c * MM * kLuRec), y);
         double *myred = red + l * (NI + 1);
@@ -374,7 +397,7 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
     for (int j = 0; j < MM + 2 * H; ++j) w[j] = trow[j * LS];
     if (kShare) __syncthreads();  // every window is read before the interface values overwrite the tile
     double y[MM];
-    chunk_rhs<H, MM, PER>(op, w, y, c == 0, c == P - 1);
+    chunk_rhs<H, MM, PER, has_rows<W, H, PER>()>(op, w, y, c == 0, c == P - 1);
     if (W > 0) {
         chunk_solve<W, MM>(op.lu + (PER ? 0 : (size_t)c * MM * kLuRec), y);
         double *myred = red + l * (NI + 1);
@@ -471,8 +494,8 @@ hipError_t launch_fast_mm(const OpDev &op, const LaunchArgs &a, int nfields, hip
         case 24: return launch_fast_bc<W, H, 24>(op, a, nfields, st);
         case 20: return launch_fast_bc<W, H, 20>(op, a, nfields, st);
         case 16: return launch_fast_bc<W, H, 16>(op, a, nfields, st);
-        // 8- and 12-row chunks: periodic lines only (the end rows of a
-        // Dirichlet line read 16 window points)
+        // 8- and 12-row chunks: periodic lines only (not instantiated for
+        // Dirichlet lines, which the plan never splits that way)
         case 12: return launch_fast<W, H, 12, true>(op, a, nfields, st);
         case 8: return launch_fast<W, H, 8, true>(op, a, nfields, st);
         default: return hipErrorInvalidValue;
@@ -497,8 +520,10 @@ bool lineop_fast_supported(const OpDev &op, size_t lds_limit) {
     // workgroup size as launched: (lines per workgroup) x P
     const int lines = op.dir != 0 ? 64 : op.periodic || op.W == 0 ? 16 : contig_lines<false, 1>();
     const int threads = lines * op.P;
-    // Dirichlet interior rows must not reach past the line ends
-    const bool ends = op.periodic || (op.nbl >= h && op.nbr >= h && op.nbl <= kSpecRows && op.nbr <= kSpecRows);
+    // special rows: H at each end of a Dirichlet line (every scheme has that
+    // many); on periodic lines only where the kernels compile them in (CD2)
+    const bool ends = op.periodic ? (op.nbl + op.nbr == 0 || (w == 0 && h == 1 && op.nbl == h && op.nbr == h))
+                                  : (op.nbl == h && op.nbr == h);
     const bool rows = chunk_rows_ok(op.MM) && (op.MM >= 16 || op.periodic) && op.MM >= 2 * h;
     return pair && ends && rows && op.N == op.P * op.MM && threads <= 1024 && lineop_fast_lds_bytes(op) <= lds_limit;
 }

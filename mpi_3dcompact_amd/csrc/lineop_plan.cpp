@@ -463,20 +463,27 @@ int build_plan(const PlanRequest &rq, Plan &plan, std::string &err) {
     };
     for (int r = 0; r < plan.nbl; ++r) put_row(r, s.left[r]);
     for (int r = 0; r < plan.nbr; ++r) put_row(plan.nbl + r, s.right[r]);
-    // dense form: absolute tap positions must fall inside the 16 points at the end
+    // dense form: every nonzero tap must fall inside the K = 2H + 1 points the
+    // row's window gives it, and the taps must be summed in increasing or
+    // decreasing position (zero taps, kept or dropped, add exact zeros)
+    const int K = 2 * s.H + 1;
     plan.drows.assign((size_t)(plan.nbl + plan.nbr) * kDRowRec, 0.0);
-    plan.dense_ok = n >= 16;
+    plan.dense_ok = n >= K && K < kDRowRec && plan.nbl <= kSpecRows && plan.nbr <= kSpecRows;
     auto put_dense = [&](int idx, const Row &r, int row, int origin) {
         double *d = &plan.drows[(size_t)idx * kDRowRec];
         d[0] = r.step > 0 ? 1.0 : -1.0;
         for (size_t t = 0; t < r.c.size(); ++t) {
             const int pos = row + r.off + (int)t * r.step - origin;
-            if (pos < 0 || pos >= 16 || (r.step != 1 && r.step != -1)) { plan.dense_ok = false; return; }
+            if (r.c[t] == 0.0) continue;
+            if (pos < 0 || pos >= K || (r.c.size() > 1 && r.step == 0)) { plan.dense_ok = false; return; }
             d[1 + pos] = r.c[t];
         }
     };
-    for (int r = 0; r < plan.nbl; ++r) put_dense(r, s.left[r], r, 0);
-    for (int r = 0; r < plan.nbr; ++r) put_dense(plan.nbl + r, s.right[r], n - plan.nbr + r, n - 16);
+    for (int r = 0; r < plan.nbl; ++r) put_dense(r, s.left[r], r, per ? r - s.H : 0);
+    for (int r = 0; r < plan.nbr; ++r) {
+        const int row = n - plan.nbr + r;
+        put_dense(plan.nbl + r, s.right[r], row, per ? row - s.H : n - K);
+    }
     plan.cstart.resize(P);
     plan.clen.resize(P);
     for (int c = 0; c < P; ++c) {

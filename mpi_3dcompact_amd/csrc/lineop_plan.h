@@ -28,7 +28,7 @@ struct Plan {
     int N = 0, W = 0, H = 0, P = 0, L = 0, NI = 0, ext = 0;
     int MM = 16;         // table row stride (rows per chunk capacity)
     bool uniform = false;  // every chunk has MM rows
-    bool dense_ok = false;  // special rows fit the 16-wide dense form
+    bool dense_ok = false;  // special rows fit the (2H+1)-wide dense form
     bool periodic = false, divide = false;
     double dd = 1.0;
     int int_off = 0;
@@ -39,8 +39,11 @@ struct Plan {
     // c - gb .. c + gb (cyclically on periodic lines) is below 1e-17 of the
     // largest entry of G_c, so the fast kernels skip those blocks
     int gb = 0;
-    // special rows as dense 16-wide rows: {order (+1 increasing, -1 decreasing), w[16]};
-    // left rows over phi[0..15], right rows over phi[N-16..N-1]
+    // special rows as dense rows of K = 2H + 1 taps (the fast kernels' window
+    // holds every point they read): {order (+1 increasing, -1 decreasing),
+    // w[K], zeros}.  Dirichlet left rows read phi[0..K), right rows
+    // phi[N-K..N); periodic rows (the wrap rows of CD2) phi[p-H..p+H] around
+    // their own point p
     std::vector<double> drows;
     std::vector<int> cstart, clen;
 };

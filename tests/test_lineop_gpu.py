@@ -134,9 +134,9 @@ def test_inplace_filter_fast_kernels(torch_cuda, kind, af, bc3):
 
 
 def test_kernel_choice(torch_cuda):
-    """Evenly chunked lines take the fast kernels; ragged lines, CD2 with its
-    periodic wrap rows and TPB take the LDS-tiled kernel.  x-lines take 16-row
-    chunks."""
+    """Evenly chunked lines take the fast kernels, periodic CD2 (whose wrap
+    rows flip the sign of its interior) included; ragged lines and TPB take
+    the LDS-tiled kernel.  x-lines take 16-row chunks."""
     from mpi_3dcompact_amd.ops import LineOp
     dims = (64, 17, 48)
     assert LineOp.deriv(P.PENTA10, 0, P.PERIODIC, dims, 1 / 64).kernel() == "lineop_contig<2, 3, 16, true>"
@@ -146,7 +146,9 @@ def test_kernel_choice(torch_cuda):
     assert LineOp.deriv(P.PENTA10, 1, P.PERIODIC, (17, 256, 48), 1 / 256).kernel() == "lineop_strided<2, 3, 32, true>"
     assert LineOp.deriv(P.PENTA10, 2, P.DIRICHLET, dims, 1 / 47).kernel() == "lineop_strided<2, 3, 16, false>"
     assert LineOp.deriv(P.PADE6, 1, P.PERIODIC, dims, 1 / 17).kernel().startswith("lineop_kernel<")
-    assert LineOp.deriv(P.CD2, 0, P.PERIODIC, dims, 1 / 64).kernel().startswith("lineop_kernel<")
+    assert LineOp.deriv(P.CD2, 0, P.PERIODIC, dims, 1 / 64).kernel() == "lineop_contig<0, 1, 16, true>"
+    assert LineOp.deriv(P.CD2, 2, P.PERIODIC, dims, 1 / 48).kernel() == "lineop_strided<0, 1, 16, true>"
+    assert LineOp.deriv(P.CD2, 1, P.PERIODIC, dims, 1 / 17).kernel().startswith("lineop_kernel<")
 
 
 @pytest.mark.parametrize("scheme", [P.PADE6, P.PENTA10, P.CD2])

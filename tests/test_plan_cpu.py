@@ -108,17 +108,20 @@ def _lu_solve(lu, y, W):
 
 
 def _dense_row(d, seg):
-    """Dense special row d = {order, w[16]} over the 16 end points seg."""
+    """Dense special row d = {order, w[K]} over the K = len(seg) points seg."""
     r = 0.0
-    order = range(16) if d[0] > 0 else range(15, -1, -1)
+    K = len(seg)
+    order = range(K) if d[0] > 0 else range(K - 1, -1, -1)
     for k in order:
         r = r + d[1 + k] * seg[k]
     return r
 
 
 def fast_eligible(t):
-    rows_ok = (t["nbl"] + t["nbr"] == 0) if t["periodic"] else (
-        t["dense_ok"] and t["H"] <= t["nbl"] <= 5 and t["H"] <= t["nbr"] <= 5)
+    if t["periodic"]:
+        rows_ok = t["nbl"] + t["nbr"] == 0 or (t["dense_ok"] and t["W"] == 0 and t["H"] == 1)
+    else:
+        rows_ok = t["dense_ok"] and t["nbl"] == t["H"] == t["nbr"]
     return t["N"] == t["P"] * t["MM"] and rows_ok
 
 
@@ -141,11 +144,12 @@ def emulate_fast(t, f):
         else:
             w = np.where((idx >= 0) & (idx < N), f[np.clip(idx, 0, N - 1)], 0.0)
         y = np.zeros(MM)
+        K = 2 * H + 1
         for i in range(MM):
-            if not per and c == 0 and i < nbl:
-                r = _dense_row(t["drows"][i], w[H:H + 16])
-            elif not per and c == Pn - 1 and i >= MM - nbr:
-                r = _dense_row(t["drows"][nbl + i - (MM - nbr)], w[H + MM - 16:H + MM])
+            if c == 0 and i < nbl:
+                r = _dense_row(t["drows"][i], w[i:i + K] if per else w[H:H + K])
+            elif c == Pn - 1 and i >= MM - nbr:
+                r = _dense_row(t["drows"][nbl + i - (MM - nbr)], w[i:i + K] if per else w[H + MM - K:H + MM])
             else:
                 r = 0.0
                 for k in range(2 * H + 1):
@@ -260,24 +264,37 @@ def test_chunking_choice():
         tables(0, P.PENTA10, P.PERIODIC, 100, 1 / 100, mm=10)
 
 
-@pytest.mark.parametrize("kind", [P.PADE6, P.PENTA10, P.CD2])
+DENSE_CASES = [(0, P.PADE6, P.DIRICHLET), (0, P.PENTA10, P.DIRICHLET), (0, P.CD2, P.DIRICHLET),
+               (0, P.CD2, P.PERIODIC), (1, P.COMPACT8, P.DIRICHLET), (1, P.COMPACT10, P.DIRICHLET),
+               (1, P.COMMN5PADE, P.DIRICHLET), (1, P.COMMN5EXP, P.DIRICHLET)]
+
+
+@pytest.mark.parametrize("cls,kind,bc", DENSE_CASES)
 @pytest.mark.parametrize("n", [32, 64, 256])
-def test_dense_rows_match_row_records_bitwise(kind, n):
-    """The dense end rows sum the same taps in the same order as the row
+def test_dense_rows_match_row_records_bitwise(cls, kind, bc, n):
+    """The dense special rows (K = 2H + 1 taps over the window points the
+    row's position gives it) sum the same taps in the same order as the row
     records (zero taps add exact zeros), so both kernel families produce the
-    same bits for the explicit stage at the line ends."""
-    t = tables(0, kind, P.DIRICHLET, n, 1.0 / (n - 1))
+    same bits for the explicit stage at the line ends and, for periodic CD2,
+    at its sign-flipped wrap rows."""
+    t = tables(cls, kind, bc, n, 1.0 / (n - 1), 0.49 if cls == 1 else 0.0)
     assert t["dense_ok"]
     f = RNG.normal(size=n)
-    rows, nbl, nbr = t["rows"], t["nbl"], t["nbr"]
+    rows, nbl, nbr, H = t["rows"], t["nbl"], t["nbr"], t["H"]
+    K = 2 * H + 1
+    assert nbl + nbr > 0
     for ridx in range(nbl + nbr):
         p = ridx if ridx < nbl else n - nbr + (ridx - nbl)
         off, step, wd = int(rows[ridx, 0]), int(rows[ridx, 1]), int(rows[ridx, 2])
         r = 0.0
         for k in range(wd):
-            r = r + rows[ridx, 3 + k] * f[p + off + k * step]
-        seg = f[:16] if ridx < nbl else f[n - 16:]
+            r = r + rows[ridx, 3 + k] * f[(p + off + k * step) % n]
+        if bc == P.PERIODIC:
+            seg = f[(p - H + np.arange(K)) % n]
+        else:
+            seg = f[:K] if ridx < nbl else f[n - K:]
         assert _dense_row(t["drows"][ridx], seg) == r
+        assert not np.any(t["drows"][ridx][1 + K:])
 
 
 @pytest.mark.parametrize("kind,cls,af", [(P.PENTA10, 0, 0.0), (P.PADE6, 0, 0.0), (P.COMPACT8, 1, 0.49),
