@@ -119,6 +119,7 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
         d.N = p.N; d.dir = dir; d.W = p.W; d.H = p.H; d.P = p.P; d.L = p.L; d.NI = p.NI;
         d.ext = p.ext; d.MM = p.MM; d.periodic = p.periodic; d.nbl = p.nbl; d.nbr = p.nbr;
         d.int_off = p.int_off; d.int_w = (int)p.int_c.size(); d.gb = p.gb;
+        d.cshort = p.cshort; d.lshort = p.lshort;
         for (int k = 0; k < d.int_w; ++k) d.int_c[k] = p.int_c[k];
         d.dd = p.dd; d.divide = p.divide ? 1 : 0;
         // 1/dd is exact when dd is a power of two; then r * (1/dd) == r / dd bitwise
@@ -136,7 +137,8 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
         // the strided fast kernel addresses a field through a buffer descriptor
         // with 32-bit byte offsets
         const bool offsets_ok = npts * 8 < 2147483648L;
-        op->fast = p.uniform && rows_ok && offsets_ok && mc3d::lineop_fast_supported(d, (size_t)lds_limit);
+        op->fast = (p.uniform || p.cshort >= 0) && rows_ok && offsets_ok &&
+                   mc3d::lineop_fast_supported(d, (size_t)lds_limit);
         if (tune.family == 2) op->fast = false;
         return MC3D_OK;
     };
@@ -186,7 +188,8 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
     const std::string per = p.periodic ? "true" : "false";
     if (op->fast)
         op->kname = std::string(dir == 0 ? "lineop_contig<" : "lineop_strided<") + std::to_string(p.W) +
-                    ", " + std::to_string(p.H) + ", " + std::to_string(p.MM) + ", " + per + ">";
+                    ", " + std::to_string(p.H) + ", " + std::to_string(p.MM) + ", " + per +
+                    (p.cshort >= 0 ? ", ragged>" : ">");
     else
         op->kname = "lineop_kernel<" + std::to_string(p.W) + ", " + std::to_string(dir) + ", " +
                     std::to_string(bc == MC3D_PERIODIC ? 0 : 1) + ", " + std::to_string(p.MM) + ">";

@@ -57,6 +57,7 @@ struct OpDev {
     const double *sp;     // P * MM * 2W
     const double *G;      // P * 2W * NI
     int gb;               // interface band in chunks (Plan::gb)
+    int cshort, lshort;   // ragged lines: chunk cshort has lshort rows (else cshort = -1)
     const int *cstart;    // P
     const int *clen;      // P
 };

@@ -118,11 +118,12 @@ __device__ __forceinline__ void chunk_solve(const double *lu, double *y) {
 }
 
 // s[j] = sum over the chunks u of the band around chunk c of G_c[j][u's 2W
-// columns] . yhat[u's 2W values].  Periodic lines read chunk 0's rows G_0
-// rotated by c chunks (G_c[j][u] = G_0[j][u - 2Wc]); chunk offsets -gb..gb, or
-// every chunk once when the band covers the line.  Dirichlet lines read G_c
-// over chunks max(0, c - gb) .. min(P - 1, c + gb).
-template <int W, bool PER>
+// columns] . yhat[u's 2W values].  Evenly chunked periodic lines read chunk
+// 0's rows G_0 rotated by c chunks (G_c[j][u] = G_0[j][u - 2Wc]), ragged ones
+// (RG) their own rows; chunk offsets -gb..gb, or every chunk once when the
+// band covers the line.  Dirichlet lines read G_c over chunks
+// max(0, c - gb) .. min(P - 1, c + gb).
+template <int W, bool PER, bool RG>
 __device__ __forceinline__ void interface_sums(const OpDev &op, int c, const double *yhat, double *s) {
     constexpr int W2 = 2 * W;
     const int P = op.P, NI = op.NI, gb = op.gb;
@@ -136,7 +137,7 @@ __device__ __forceinline__ void interface_sums(const OpDev &op, int c, const dou
             const int cb = d < 0 ? d + P : d;  // column block of G_0
             int ub = c + d;                    // chunk whose interface values it multiplies
             ub = ub < 0 ? ub + P : (ub >= P ? ub - P : ub);
-            const double *g = op.G + cb * W2;
+            const double *g = RG ? op.G + (size_t)c * W2 * NI + ub * W2 : op.G + cb * W2;
             const double *yv = yhat + ub * W2;
 #pragma unroll
             for (int k = 0; k < W2; ++k) {
@@ -187,12 +188,41 @@ __device__ __forceinline__ bool pass_line(const OpDev &op, int dir, int q, int N
     return (op.passA && (ia == 0 || ia == op.NA - 1)) || (op.passB && (ib == 0 || ib == op.NB - 1));
 }
 
+// Ragged lines (RG): chunk cshort (1) has lshort < MM rows, so later chunks
+// start MM - lshort rows earlier.  The chunk's rows beyond its length are
+// computed and never stored; its factor rows there are identity rows.
+template <int MM, bool RG>
+__device__ __forceinline__ int chunk_start(const OpDev &op, int c) {
+    return RG && c > op.cshort ? c * MM - (MM - op.lshort) : c * MM;
+}
+
+// publishes a chunk's first and last W values; the last W of the short chunk
+// of a ragged line sit at rows len - W .. len - 1, picked by a select over the
+// rows (registers cannot be indexed at run time)
+template <int W, int MM, bool RG>
+__device__ __forceinline__ void publish(double *myred, int c, bool short_chunk, int len, const double *y) {
+#pragma unroll
+    for (int k = 0; k < W; ++k) myred[c * 2 * W + k] = y[k];
+    if (RG && short_chunk) {
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            double v = 0.0;
+#pragma unroll
+            for (int i = 0; i < MM; ++i) v = (i == len - W + k) ? y[i] : v;
+            myred[c * 2 * W + W + k] = v;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < W; ++k) myred[c * 2 * W + W + k] = y[MM - W + k];
+    }
+}
+
 // special end rows: every Dirichlet line; of the periodic operators only CD2
 // (W = 0, H = 1), whose wrap rows carry the opposite sign of its interior
 template <int W, int H, bool PER>
 constexpr bool has_rows() { return !PER || (W == 0 && H == 1); }
 
-template <int W, int H, int MM, bool PER>
+template <int W, int H, int MM, bool PER, bool RG>
 __global__ __launch_bounds__(1024) void lineop_strided(OpDev op, LaunchArgs a) {
     extern __shared__ double red[];
     const int l = threadIdx.x;
@@ -216,7 +246,9 @@ __global__ __launch_bounds__(1024) void lineop_strided(OpDev op, LaunchArgs a) {
         base = qq;
         ps = (long)a.Nx * a.Ny;
     }
-    const int cs = c * MM;
+    const int cs = chunk_start<MM, RG>(op, c);
+    const bool short_chunk = RG && c == op.cshort;
+    const int len = short_chunk ? op.lshort : MM;
     // window loads through a buffer descriptor (fields below 2^28 points)
     const __amdgpu_buffer_rsrc_t rin =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(in), (short)0, (int)fbytes, kRawBufferWord3);
@@ -248,21 +280,16 @@ __global__ __launch_bounds__(1024) void lineop_strided(OpDev op, LaunchArgs a) {
     double y[MM];
     chunk_rhs<H, MM, PER, has_rows<W, H, PER>()>(op, w, y, c == 0, c == P - 1);
     if (W > 0) {
-        chunk_solve<W, MM>(op.lu + (PER ? 0 : (size_t)c * MM * kLuRec), y);
-        double *myred = red + l * (NI + 1);
-#pragma unroll
-        for (int k = 0; k < W; ++k) {
-            myred[c * 2 * W + k] = y[k];
-            myred[c * 2 * W + W + k] = y[MM - W + k];
-        }
+        chunk_solve<W, MM>(op.lu + (PER && !RG ? 0 : (size_t)c * MM * kLuRec), y);
+        publish<W, MM, RG>(red + l * (NI + 1), c, short_chunk, len, y);
     }
     // every lane has read its window (in may equal out) and published its interface
     __syncthreads();
     if (W > 0) {
         const double *myred = red + l * (NI + 1);
         double s[2 * W > 0 ? 2 * W : 1];
-        interface_sums<W, PER>(op, c, myred, s);
-        const double *sp = op.sp + (PER ? 0 : (size_t)c * MM * 2 * W);
+        interface_sums<W, PER, RG>(op, c, myred, s);
+        const double *sp = op.sp + (PER && !RG ? 0 : (size_t)c * MM * 2 * W);
 #pragma unroll
         for (int i = 0; i < MM; ++i) {
             if (i % 8 == 0) __builtin_amdgcn_sched_barrier(0);
@@ -277,14 +304,17 @@ __global__ __launch_bounds__(1024) void lineop_strided(OpDev op, LaunchArgs a) {
         __builtin_amdgcn_make_buffer_rsrc(out, (short)0, (int)fbytes, kRawBufferWord3);
     if (pass_line(op, op.dir, q, a.Nx, a.Ny)) {
 #pragma unroll
-        for (int i = 0; i < MM; ++i) y[i] = bload(rin, boff, (unsigned)(cs + i) * rowb);
+        for (int i = 0; i < MM; ++i)
+            if (!RG || i < len) y[i] = bload(rin, boff, (unsigned)(cs + i) * rowb);
     }
     if (a.accumulate) {
 #pragma unroll
-        for (int i = 0; i < MM; ++i) y[i] = bload_nt(rout, boff, (unsigned)(cs + i) * rowb) + y[i];
+        for (int i = 0; i < MM; ++i)
+            if (!RG || i < len) y[i] = bload_nt(rout, boff, (unsigned)(cs + i) * rowb) + y[i];
     }
 #pragma unroll
-    for (int i = 0; i < MM; ++i) bstore(y[i], rout, boff, (unsigned)(cs + i) * rowb);
+    for (int i = 0; i < MM; ++i)
+        if (!RG || i < len) bstore(y[i], rout, boff, (unsigned)(cs + i) * rowb);
 }
 
 // lines per workgroup of the contiguous kernel: 16 for periodic lines (every
@@ -295,9 +325,11 @@ __global__ __launch_bounds__(1024) void lineop_strided(OpDev op, LaunchArgs a) {
 // CommN5Exp x 4.7 -> 5.7-5.8 TB/s with 16 lines), 64 for banded Dirichlet
 // ones (a wave is then 64 lines x one chunk, so the chunk's factor tables are
 // scalar loads and the end-row branches uniform; with 16 lines the per-lane
-// table loads made Penta10 x twice slower)
-template <bool PER, int W>
-constexpr int contig_lines() { return PER || W == 0 ? 16 : 64; }
+// table loads made Penta10 x twice slower).  Banded operators on ragged
+// lines (RG) read chunk-own tables, periodic ones included: 64 lines.
+__host__ __device__ constexpr int contig_lines(bool per, int w, bool rg) {
+    return w == 0 || (per && !rg) ? 16 : 64;
+}
 
 // loads in flight per thread while staging.  The tile caps a CU at one or two
 // workgroups, so the fill is latency bound: 16 at a time where the registers
@@ -309,10 +341,10 @@ constexpr int contig_batch() {
     return MM % 16 ? (MM <= 12 ? MM : MM / 2) : (MM == 32 || (PER && H < 5) ? 16 : 8);
 }
 
-template <int W, int H, int MM, bool PER>
+template <int W, int H, int MM, bool PER, bool RG>
 __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
     extern __shared__ double smem[];
-    constexpr int L = contig_lines<PER, W>();
+    constexpr int L = contig_lines(PER, W, RG);
     constexpr int LS = L + 1;  // row stride of the transposed tile
     const int N = op.N, P = op.P, NI = op.NI;
     double *tile = smem;  // [N + 2H][LS], row e holds phi[e - H]
@@ -323,7 +355,8 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
     constexpr bool kShare = W == 2;
     double *red = kShare ? smem : smem + (size_t)(N + 2 * H) * LS;
     const int tid = threadIdx.y * L + threadIdx.x;
-    const int nthr = L * P;  // L * N = nthr * MM elements
+    const int nthr = L * P;  // L * N = nthr * MM elements (ragged lines: fewer)
+    const int LN = L * N;
     const int q0 = blockIdx.x * L;
     // The workgroup's lines are consecutive rows of the field: one contiguous
     // block of L * N doubles, element e at point e % N of line e / N.  Buffer
@@ -338,12 +371,12 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
     // k: by (nthr / N lines, nthr % N points), wrapping the point.  When the
     // chunk rows divide the lines per workgroup, nthr is a whole number of
     // lines and the slot advances by a constant.
-    constexpr bool kWhole = L % MM == 0;
+    constexpr bool kWhole = !RG && L % MM == 0;
     const int ll0 = tid / N, p0 = tid - ll0 * N;
     const int slot0 = (p0 + H) * LS + ll0;
     const int qd = nthr / N, rd = nthr - qd * N;
     const int step = rd * LS + qd, wrapfix = 1 - N * LS;
-    constexpr int kB = contig_batch<H, MM, PER>();
+    constexpr int kB = contig_batch<H, MM, PER && !RG>();
     // fill: batches of coalesced loads, then their LDS writes
     {
         // periodic halo rows: phi[N - H .. N) above row H, phi[0 .. H) below row N + H
@@ -362,7 +395,7 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
             for (int k = 0; k < kB; ++k) v[k] = bload_nt(rin, (unsigned)(tid + (k0 + k) * nthr) * 8u, 0);
 #pragma unroll
             for (int k = 0; k < kB; ++k) {
-                tile[slot] = v[k];
+                if (!RG || tid + (k0 + k) * nthr < LN) tile[slot] = v[k];
                 if (kWhole) {
                     slot += L / MM;
                 } else {
@@ -390,7 +423,9 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
     const int c = L < 64 ? (int)threadIdx.y : __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int q = q0 + l;
     const bool live = q < a.nlines;
-    const int cs = c * MM;
+    const int cs = chunk_start<MM, RG>(op, c);
+    const bool short_chunk = RG && c == op.cshort;
+    const int len = short_chunk ? op.lshort : MM;
     double w[MM + 2 * H];
     const double *trow = tile + (size_t)cs * LS + l;  // row cs holds phi[cs - H]
 #pragma unroll
@@ -399,21 +434,16 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
     double y[MM];
     chunk_rhs<H, MM, PER, has_rows<W, H, PER>()>(op, w, y, c == 0, c == P - 1);
     if (W > 0) {
-        chunk_solve<W, MM>(op.lu + (PER ? 0 : (size_t)c * MM * kLuRec), y);
-        double *myred = red + l * (NI + 1);
-#pragma unroll
-        for (int k = 0; k < W; ++k) {
-            myred[c * 2 * W + k] = y[k];
-            myred[c * 2 * W + W + k] = y[MM - W + k];
-        }
+        chunk_solve<W, MM>(op.lu + (PER && !RG ? 0 : (size_t)c * MM * kLuRec), y);
+        publish<W, MM, RG>(red + l * (NI + 1), c, short_chunk, len, y);
     }
     // every lane has read its window and published its interface values
     __syncthreads();
     if (W > 0) {
         const double *myred = red + l * (NI + 1);
         double s[2 * W > 0 ? 2 * W : 1];
-        interface_sums<W, PER>(op, c, myred, s);
-        const double *sp = op.sp + (PER ? 0 : (size_t)c * MM * 2 * W);
+        interface_sums<W, PER, RG>(op, c, myred, s);
+        const double *sp = op.sp + (PER && !RG ? 0 : (size_t)c * MM * 2 * W);
 #pragma unroll
         for (int i = 0; i < MM; ++i) {
             if (i % 8 == 0) __builtin_amdgcn_sched_barrier(0);
@@ -431,7 +461,8 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
     }
     if (live && (kShare || !pass)) {
 #pragma unroll
-        for (int i = 0; i < MM; ++i) tile[(cs + H + i) * LS + l] = y[i];
+        for (int i = 0; i < MM; ++i)
+            if (!RG || i < len) tile[(cs + H + i) * LS + l] = y[i];
     }
     __syncthreads();
     // store: batches of coalesced stores (and loads when accumulating)
@@ -444,7 +475,7 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
             double v[kB];
 #pragma unroll
             for (int k = 0; k < kB; ++k) {
-                v[k] = tile[slot];
+                v[k] = !RG || tid + (k0 + k) * nthr < LN ? tile[slot] : 0.0;
                 if (kWhole) {
                     slot += L / MM;
                 } else {
@@ -466,28 +497,33 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
     }
 }
 
-template <int W, int H, int MM, bool PER>
+template <int W, int H, int MM, bool PER, bool RG>
 hipError_t launch_fast(const OpDev &op, const LaunchArgs &a, int nfields, hipStream_t st) {
     if (op.dir == 0) {
-        constexpr int L = contig_lines<PER, W>();
+        constexpr int L = contig_lines(PER, W, RG);
         dim3 grid((a.nlines + L - 1) / L, nfields), block(L, op.P);
-        hipLaunchKernelGGL((lineop_contig<W, H, MM, PER>), grid, block, lineop_fast_lds_bytes(op), st, op, a);
+        hipLaunchKernelGGL((lineop_contig<W, H, MM, PER, RG>), grid, block, lineop_fast_lds_bytes(op), st, op, a);
     } else {
         dim3 grid((a.nlines + 63) / 64, nfields, op.dir == 2 ? a.nbatch : 1), block(64, op.P);
         const size_t lds = sizeof(double) * (size_t)64 * (op.NI + 1);
-        hipLaunchKernelGGL((lineop_strided<W, H, MM, PER>), grid, block, lds, st, op, a);
+        hipLaunchKernelGGL((lineop_strided<W, H, MM, PER, RG>), grid, block, lds, st, op, a);
     }
     return hipGetLastError();
 }
 
-template <int W, int H, int MM>
+template <int W, int H, int MM, bool RG = false>
 hipError_t launch_fast_bc(const OpDev &op, const LaunchArgs &a, int nfields, hipStream_t st) {
-    return op.periodic ? launch_fast<W, H, MM, true>(op, a, nfields, st)
-                       : launch_fast<W, H, MM, false>(op, a, nfields, st);
+    return op.periodic ? launch_fast<W, H, MM, true, RG>(op, a, nfields, st)
+                       : launch_fast<W, H, MM, false, RG>(op, a, nfields, st);
 }
 
 template <int W, int H>
 hipError_t launch_fast_mm(const OpDev &op, const LaunchArgs &a, int nfields, hipStream_t st) {
+    if (op.cshort >= 0) {  // ragged lines: 16- or 32-row chunks and one short chunk
+        if (op.MM == 16) return launch_fast_bc<W, H, 16, true>(op, a, nfields, st);
+        if (op.MM == 32) return launch_fast_bc<W, H, 32, true>(op, a, nfields, st);
+        return hipErrorInvalidValue;
+    }
     switch (op.MM) {
         case 32: return launch_fast_bc<W, H, 32>(op, a, nfields, st);
         case 28: return launch_fast_bc<W, H, 28>(op, a, nfields, st);
@@ -496,8 +532,8 @@ hipError_t launch_fast_mm(const OpDev &op, const LaunchArgs &a, int nfields, hip
         case 16: return launch_fast_bc<W, H, 16>(op, a, nfields, st);
         // 8- and 12-row chunks: periodic lines only (not instantiated for
         // Dirichlet lines, which the plan never splits that way)
-        case 12: return launch_fast<W, H, 12, true>(op, a, nfields, st);
-        case 8: return launch_fast<W, H, 8, true>(op, a, nfields, st);
+        case 12: return launch_fast<W, H, 12, true, false>(op, a, nfields, st);
+        case 8: return launch_fast<W, H, 8, true, false>(op, a, nfields, st);
         default: return hipErrorInvalidValue;
     }
 }
@@ -506,7 +542,7 @@ hipError_t launch_fast_mm(const OpDev &op, const LaunchArgs &a, int nfields, hip
 
 size_t lineop_fast_lds_bytes(const OpDev &op) {
     if (op.dir == 0) {  // the tile, with the interface values inside it (W = 2) or after it
-        const size_t L = op.periodic || op.W == 0 ? 16 : contig_lines<false, 1>();
+        const size_t L = contig_lines(op.periodic, op.W, op.cshort >= 0);
         const size_t tile = (size_t)(op.N + 2 * op.H) * (L + 1), red = L * (op.NI + 1);
         return sizeof(double) * (op.W == 2 ? std::max(tile, red) : tile + red);
     }
@@ -518,14 +554,19 @@ bool lineop_fast_supported(const OpDev &op, size_t lds_limit) {
     const bool pair = (w == 0 && (h == 1 || h == 3)) || (w == 1 && (h == 2 || h == 4 || h == 5)) ||
                       (w == 2 && (h == 3 || h == 5));
     // workgroup size as launched: (lines per workgroup) x P
-    const int lines = op.dir != 0 ? 64 : op.periodic || op.W == 0 ? 16 : contig_lines<false, 1>();
+    const int lines = op.dir != 0 ? 64 : contig_lines(op.periodic, op.W, op.cshort >= 0);
     const int threads = lines * op.P;
     // special rows: H at each end of a Dirichlet line (every scheme has that
     // many); on periodic lines only where the kernels compile them in (CD2)
     const bool ends = op.periodic ? (op.nbl + op.nbr == 0 || (w == 0 && h == 1 && op.nbl == h && op.nbr == h))
                                   : (op.nbl == h && op.nbr == h);
     const bool rows = chunk_rows_ok(op.MM) && (op.MM >= 16 || op.periodic) && op.MM >= 2 * h;
-    return pair && ends && rows && op.N == op.P * op.MM && threads <= 1024 && lineop_fast_lds_bytes(op) <= lds_limit;
+    // evenly chunked, or ragged: 16- or 32-row chunks and one short second chunk
+    const bool chunks = op.cshort < 0 ? op.N == op.P * op.MM
+                                      : (op.MM == 16 || op.MM == 32) && op.cshort == 1 && op.P >= 3 &&
+                                            op.lshort >= std::max(2 * w, 1) && op.lshort < op.MM &&
+                                            op.N == (op.P - 1) * op.MM + op.lshort;
+    return pair && ends && rows && chunks && threads <= 1024 && lineop_fast_lds_bytes(op) <= lds_limit;
 }
 
 hipError_t lineop_launch_fast(const OpDev &op, const LaunchArgs &a, int nfields, hipStream_t st) {

@@ -416,16 +416,17 @@ int build_plan(const PlanRequest &rq, Plan &plan, std::string &err) {
     // chunking: 32-row chunks when the line splits evenly into at least two,
     // 16-row ones when it splits evenly into those, else the largest of 28, 24,
     // 20 (and, periodic lines only, 12 and 8) rows that splits it evenly into
-    // 2..16 chunks, so that the fast kernels take it; else 16-row chunks, the
-    // last one shorter (the LDS-tiled kernel)
+    // 2..16 chunks, so that the fast kernels take it; else 16-row chunks (32
+    // above 256 points, so that at most 16 chunks reach 512 points) and one
+    // shorter chunk (below)
     int mm = rq.mm;
     if (mm == 0) {
         if (n % 32 == 0 && n / 32 >= 2) mm = 32;
-        else if (n % 16 == 0) mm = 16;
+        else if (n % 16 == 0 && n / 16 <= 16) mm = 16;
         else {
             for (int c : {28, 24, 20, 12, 8})
                 if (n % c == 0 && n / c >= 2 && n / c <= 16 && (per || c >= 16)) { mm = c; break; }
-            if (!mm) mm = 16;
+            if (!mm) mm = n > 256 ? 32 : 16;
         }
     }
     if (!chunk_rows_ok(mm)) { err = "chunk rows must be 8, 12, 16, 20, 24, 28 or 32"; return -1; }
@@ -434,6 +435,18 @@ int build_plan(const PlanRequest &rq, Plan &plan, std::string &err) {
     int M = (n + P - 1) / P;
     int last = n - (P - 1) * M;
     if (last < std::max(2 * W, 1) || M > mm) { err = "cannot partition line"; return -1; }
+    // a line that mm-row chunks do not split evenly: mm-row chunks and one
+    // short chunk of the remaining rows, placed second (neither end chunk, which
+    // hold the special rows and the ends' halo), when there are at least three
+    // chunks and the short one holds both its interface blocks.  Else P chunks
+    // of M rows and a shorter last one (the LDS-tiled kernel only).
+    const int rem = n - (P - 1) * mm;
+    plan.cshort = -1;
+    plan.lshort = 0;
+    if (n % mm != 0 && P >= 3 && rem >= std::max(2 * W, 1)) {
+        plan.cshort = 1;
+        plan.lshort = rem;
+    }
     plan.N = n; plan.W = W; plan.H = s.H; plan.P = P; plan.periodic = per;
     plan.MM = mm;
     plan.uniform = (M == mm && last == mm);
@@ -486,9 +499,10 @@ int build_plan(const PlanRequest &rq, Plan &plan, std::string &err) {
     }
     plan.cstart.resize(P);
     plan.clen.resize(P);
-    for (int c = 0; c < P; ++c) {
-        plan.cstart[c] = c * M;
-        plan.clen[c] = (c == P - 1) ? last : M;
+    for (int c = 0, cs = 0; c < P; ++c) {
+        plan.cstart[c] = plan.cshort >= 0 ? cs : c * M;
+        plan.clen[c] = plan.cshort >= 0 ? (c == plan.cshort ? plan.lshort : mm) : (c == P - 1) ? last : M;
+        cs += plan.clen[c];
     }
     plan.lu.assign((size_t)P * mm * kLuRec, 0.0);
     plan.sp.assign((size_t)P * mm * std::max(2 * W, 1), 0.0);

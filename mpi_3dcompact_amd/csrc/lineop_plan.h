@@ -28,6 +28,9 @@ struct Plan {
     int N = 0, W = 0, H = 0, P = 0, L = 0, NI = 0, ext = 0;
     int MM = 16;         // table row stride (rows per chunk capacity)
     bool uniform = false;  // every chunk has MM rows
+    // ragged lines: every chunk has MM rows but chunk cshort (1), which has
+    // lshort < MM; cshort = -1 otherwise
+    int cshort = -1, lshort = 0;
     bool dense_ok = false;  // special rows fit the (2H+1)-wide dense form
     bool periodic = false, divide = false;
     double dd = 1.0;

@@ -89,6 +89,67 @@ def test_filter_all_directions(torch_cuda, kind, af, bc3, shape):
         assert relerr(out.cpu().numpy(), ref) <= TOL, (kind, bc3, d)
 
 
+# ragged lengths: x 150 = 16 + 6 + 8 x 16 (periodic) / 150; y 36 = 16 + 4 + 16;
+# z 131 = 16 + 3 + 7 x 16 (three-row short chunk: W <= 1 only); second shape
+# x 310 = 32 + 22 + 8 x 32
+RAGGED_SHAPES = [(131, 36, 150), (12, 20, 310)]
+
+
+@pytest.mark.parametrize("shape", RAGGED_SHAPES)
+@pytest.mark.parametrize("scheme,bc", list(itertools.product([P.PADE6, P.PENTA10, P.CD2],
+                                                               [P.PERIODIC, P.DIRICHLET])))
+def test_deriv_ragged_lines(torch_cuda, scheme, bc, shape):
+    """Lines that no chunk length splits evenly: 16- or 32-row chunks and one
+    short second chunk on the fast kernels, against the oracle at 1e-12."""
+    from mpi_3dcompact_amd.ops import LineOp, plan_tables
+    torch = torch_cuda
+    Nz, Ny, Nx = shape
+    dims = (Nx, Ny, Nz)
+    f = field(shape, 3)
+    fin = torch.from_numpy(f).cuda()
+    for d in range(3):
+        n = dims[d]
+        dd = 1.0 / n if bc == P.PERIODIC else 1.0 / (n - 1)
+        op = LineOp.deriv(scheme, d, bc, dims, dd)
+        kname = op.kernel()
+        _say(f"ragged deriv scheme={scheme} bc={bc} shape={shape} dir={d} kernel={kname}")
+        t = plan_tables(0, scheme, bc, n, dd)
+        # (x-lines: a 64-line LDS tile of the whole line fits 160 KB up to about 300 points)
+        if (n % t["MM"] and t["P"] >= 3 and n - (t["P"] - 1) * t["MM"] >= max(2 * t["W"], 1) and n <= 512
+                and (d != 0 or n <= 256)):
+            assert kname.startswith("lineop_contig" if d == 0 else "lineop_strided"), kname
+        out = torch.empty_like(fin)
+        op.apply([fin], [out])
+        torch.cuda.synchronize()
+        ref = P.deriv_field(scheme, d, bc, f, dd)
+        assert relerr(out.cpu().numpy(), ref) <= TOL, (scheme, bc, d)
+
+
+@pytest.mark.parametrize("shape", RAGGED_SHAPES)
+@pytest.mark.parametrize("kind,af", [(P.COMPACT8, 0.49), (P.COMPACT10, -0.49), (P.COMMN5PADE, 0.0),
+                                     (P.COMMN5EXP, 0.0)])
+@pytest.mark.parametrize("bc3", [(0, 0, 0), (1, 1, 1)])
+def test_filter_ragged_lines(torch_cuda, kind, af, bc3, shape):
+    from mpi_3dcompact_amd.ops import LineOp
+    torch = torch_cuda
+    Nz, Ny, Nx = shape
+    dims = (Nx, Ny, Nz)
+    f = field(shape, 4)
+    fin = torch.from_numpy(f).cuda()
+    for d in range(3):
+        op = LineOp.filter(kind, af, d, bc3, dims)
+        _say(f"ragged filter kind={kind} bc3={bc3} shape={shape} dir={d} kernel={op.kernel()}")
+        out = torch.empty_like(fin)
+        op.apply([fin], [out])
+        torch.cuda.synchronize()
+        ref = P.filter_field(kind, af, d, list(bc3), f)
+        assert relerr(out.cpu().numpy(), ref) <= TOL, (kind, bc3, d)
+        fl = torch.from_numpy(f).cuda()  # in place
+        op.apply([fl], [fl])
+        torch.cuda.synchronize()
+        assert relerr(fl.cpu().numpy(), ref) <= TOL, (kind, bc3, d, "in place")
+
+
 def test_batched_fields_and_inplace_filter(torch_cuda):
     from mpi_3dcompact_amd.ops import LineOp
     torch = torch_cuda
@@ -135,8 +196,9 @@ def test_inplace_filter_fast_kernels(torch_cuda, kind, af, bc3):
 
 def test_kernel_choice(torch_cuda):
     """Evenly chunked lines take the fast kernels, periodic CD2 (whose wrap
-    rows flip the sign of its interior) included; ragged lines and TPB take
-    the LDS-tiled kernel.  x-lines take 16-row chunks."""
+    rows flip the sign of its interior) included, and so do ragged lines of at
+    least three chunks whose short chunk holds 2W rows; other ragged lines and
+    TPB take the LDS-tiled kernel.  x-lines take 16-row chunks."""
     from mpi_3dcompact_amd.ops import LineOp
     dims = (64, 17, 48)
     assert LineOp.deriv(P.PENTA10, 0, P.PERIODIC, dims, 1 / 64).kernel() == "lineop_contig<2, 3, 16, true>"
@@ -149,6 +211,11 @@ def test_kernel_choice(torch_cuda):
     assert LineOp.deriv(P.CD2, 0, P.PERIODIC, dims, 1 / 64).kernel() == "lineop_contig<0, 1, 16, true>"
     assert LineOp.deriv(P.CD2, 2, P.PERIODIC, dims, 1 / 48).kernel() == "lineop_strided<0, 1, 16, true>"
     assert LineOp.deriv(P.CD2, 1, P.PERIODIC, dims, 1 / 17).kernel().startswith("lineop_kernel<")
+    assert LineOp.deriv(P.PENTA10, 0, P.PERIODIC, (150, 17, 48), 1 / 150).kernel() == \
+        "lineop_contig<2, 3, 16, true, ragged>"
+    assert LineOp.deriv(P.PENTA10, 2, P.DIRICHLET, (16, 17, 310), 1 / 309).kernel() == \
+        "lineop_strided<2, 3, 32, false, ragged>"
+    assert LineOp.deriv(P.PENTA10, 2, P.DIRICHLET, (16, 17, 131), 1 / 130).kernel().startswith("lineop_kernel<")
 
 
 @pytest.mark.parametrize("scheme", [P.PADE6, P.PENTA10, P.CD2])
@@ -268,9 +335,9 @@ def test_uneven_sizes_take_the_fast_kernels(torch_cuda, shape, mm):
 def test_eight_row_chunks(torch_cuda):
     """88-point periodic lines split into eleven 8-row chunks: the fast kernels
     take derivatives and the Compact8 filter (half-width 4 fits 8 rows);
-    Compact10 and CommN5Pade (half-width 5) take the LDS-tiled kernel.  24-point
-    z-lines split into 12-row chunks.  Dirichlet lines of these lengths have no
-    even split the fast kernels take."""
+    Compact10 and CommN5Pade (half-width 5) take the fast kernels' ragged
+    layout (five 16-row chunks and one of 8 rows), as do Dirichlet lines of
+    88 points.  24-point z-lines split into 12-row chunks."""
     from mpi_3dcompact_amd.ops import LineOp
     torch = torch_cuda
     shape = (24, 88, 88)
@@ -291,6 +358,8 @@ def test_eight_row_chunks(torch_cuda):
                 _say(f"deriv scheme={scheme} bc={bc} dir={d} kernel={kname}")
                 if bc == P.PERIODIC:
                     assert kname.startswith(fam) and kname.endswith(f", {mm[d]}, true>"), kname
+                elif mm[d] == 8:
+                    assert kname.startswith(fam) and kname.endswith(", 16, false, ragged>"), kname
                 op.apply([fin], [out])
                 torch.cuda.synchronize()
                 assert relerr(out.cpu().numpy(), P.deriv_field(scheme, d, bc, f, dd)) <= TOL, (scheme, bc, d)
@@ -301,7 +370,7 @@ def test_eight_row_chunks(torch_cuda):
             if kind == P.COMPACT8:
                 assert kname.startswith(fam) and kname.endswith(f", {mm[d]}, true>"), kname
             elif mm[d] == 8:
-                assert kname.startswith("lineop_kernel<"), kname
+                assert kname.startswith(fam) and kname.endswith(", 16, true, ragged>"), kname
             op.apply([fin], [out])
             torch.cuda.synchronize()
             assert relerr(out.cpu().numpy(), P.filter_field(kind, af, d, [0, 0, 0], f)) <= TOL, (kind, d)

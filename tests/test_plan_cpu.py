@@ -117,27 +117,40 @@ def _dense_row(d, seg):
     return r
 
 
+def ragged(t):
+    """MM-row chunks (16 or 32) and one short second chunk: the fast kernels'
+    ragged layout."""
+    clen, MM, P = list(t["clen"]), t["MM"], t["P"]
+    return (t["N"] != P * MM and MM in (16, 32) and P >= 3 and clen[1] < MM
+            and all(c == MM for i, c in enumerate(clen) if i != 1))
+
+
 def fast_eligible(t):
     if t["periodic"]:
         rows_ok = t["nbl"] + t["nbr"] == 0 or (t["dense_ok"] and t["W"] == 0 and t["H"] == 1)
     else:
         rows_ok = t["dense_ok"] and t["nbl"] == t["H"] == t["nbr"]
-    return t["N"] == t["P"] * t["MM"] and rows_ok
+    return (t["N"] == t["P"] * t["MM"] or ragged(t)) and rows_ok
 
 
 def emulate_fast(t, f):
-    """Replay the fast kernels (evenly chunked lines) on one line f: register
-    window per chunk, dense end rows, multiplication by an exact 1/dd, chunk 0
-    factors for every periodic chunk, chunk 0's interface rows rotated, and
-    only the interface blocks within gb chunks of each chunk."""
+    """Replay the fast kernels (evenly chunked lines, or ragged ones: MM-row
+    chunks and a short second chunk) on one line f: an MM + 2H register window
+    per chunk, dense end rows, multiplication by an exact 1/dd, chunk 0 factors
+    for every chunk of an evenly chunked periodic line and chunk 0's interface
+    rows rotated (chunk-own rows otherwise), the short chunk's last interface
+    values from its own last rows, and only the interface blocks within gb
+    chunks of each chunk."""
     N, W, Pn, H, MM, NI = t["N"], t["W"], t["P"], t["H"], t["MM"], t["NI"]
     nbl, nbr, per = t["nbl"], t["nbr"], t["periodic"]
+    rg = ragged(t)
+    own = not per or rg  # chunk-own factor and interface rows
     dd = t["dd"]
     m, ex = np.frexp(dd)
     inv = 1.0 / dd if (t["divide"] and m == 0.5) else 0.0
     ys = []
     for c in range(Pn):
-        cs = c * MM
+        cs, ln = int(t["cstart"][c]), int(t["clen"][c])
         idx = np.arange(cs - H, cs + MM + H)
         if per:
             w = f[idx % N]
@@ -160,22 +173,23 @@ def emulate_fast(t, f):
                 r = r / dd
             y[i] = r
         if W > 0:
-            _lu_solve(t["lu"][0 if per else c], y, W)
+            _lu_solve(t["lu"][c if own else 0], y, W)
         ys.append(y)
+    lens = [int(v) for v in t["clen"]]
     if W > 0:
-        yhat = np.concatenate([np.concatenate([y[:W], y[MM - W:]]) for y in ys])
+        yhat = np.concatenate([np.concatenate([y[:W], y[ln - W:ln]]) for y, ln in zip(ys, lens)])
         for c in range(Pn):
-            if per:
-                G = np.roll(t["G"][0], c * 2 * W, axis=1)
-            else:
+            if own:
                 G = t["G"][c].copy()
+            else:
+                G = np.roll(t["G"][0], c * 2 * W, axis=1)
             dist = np.abs(np.arange(Pn) - c)
             if per:
                 dist = np.minimum(dist, Pn - dist)
             G = G * np.repeat(dist <= t["gb"], 2 * W)[None, :]
             s = G @ yhat
-            ys[c] = ys[c] - t["sp"][0 if per else c] @ s
-    return np.concatenate(ys)
+            ys[c] = ys[c] - t["sp"][c if own else 0] @ s
+    return np.concatenate([y[:ln] for y, ln in zip(ys, lens)])
 
 
 def tables(cls, kind, bc, n, dd=1.0, af=0.0, mm=0):
@@ -205,7 +219,7 @@ def _check(t, ref_fn, n):
 
 @pytest.mark.parametrize("mm", [0, 16])
 @pytest.mark.parametrize("bc", [P.PERIODIC, P.DIRICHLET])
-@pytest.mark.parametrize("n", [11, 17, 32, 36, 40, 48, 56, 64, 72, 96, 200, 256])
+@pytest.mark.parametrize("n", [11, 17, 32, 36, 40, 48, 56, 64, 72, 96, 131, 150, 200, 256, 310])
 @pytest.mark.parametrize("scheme", DERIVS)
 def test_deriv_plan_matches_oracle(scheme, n, bc, mm):
     dd = 1.0 / n if bc == P.PERIODIC else 1.0 / (n - 1)
@@ -215,7 +229,7 @@ def test_deriv_plan_matches_oracle(scheme, n, bc, mm):
 
 @pytest.mark.parametrize("mm", [0, 16])
 @pytest.mark.parametrize("bc", [P.PERIODIC, P.DIRICHLET])
-@pytest.mark.parametrize("n", [11, 17, 32, 36, 40, 48, 56, 64, 72, 96, 200, 256])
+@pytest.mark.parametrize("n", [11, 17, 32, 36, 40, 48, 56, 64, 72, 96, 131, 150, 200, 256, 310])
 @pytest.mark.parametrize("kind,af", FILTS)
 def test_filter_plan_matches_oracle(kind, af, n, bc, mm):
     t = tables(1, kind, bc, n, 1.0, af, mm=mm)
@@ -256,9 +270,19 @@ def test_chunking_choice():
                               (88, P.PERIODIC, 8, 11), (104, P.PERIODIC, 8, 13)):
         t = tables(0, P.PENTA10, bc, n, 1.0 / n)
         assert (t["MM"], t["P"]) == (mm, chunks), (n, bc)
-    # no such divisor, or a Dirichlet line that only 12 rows would split: 16-row chunks, the last shorter
-    assert tables(0, P.PENTA10, P.PERIODIC, 150, 1 / 150)["MM"] == 16
-    assert tables(0, P.PENTA10, P.DIRICHLET, 36, 1 / 35)["MM"] == 16
+    # no such divisor, or a Dirichlet line that only 12 rows would split: 16-row
+    # chunks (32 above 256 points) and one short second chunk (the ragged
+    # layout of the fast kernels) when there are at least three chunks and the
+    # short one holds 2W rows; else chunks of equal length and a shorter last one
+    for n, bc, mm, lens in ((150, P.PERIODIC, 16, [16, 6] + [16] * 8), (36, P.DIRICHLET, 16, [16, 4, 16]),
+                            (131, P.DIRICHLET, 16, [16, 3] + [16] * 7), (310, P.PERIODIC, 32, [32, 22] + [32] * 8),
+                            (272, P.DIRICHLET, 32, [32, 16] + [32] * 7)):
+        t = tables(0, P.PENTA10 if n != 131 else P.PADE6, bc, n, 1.0 / n)
+        assert t["MM"] == mm and list(t["clen"]) == lens, (n, list(t["clen"]))
+        assert list(t["cstart"]) == list(np.cumsum([0] + lens[:-1])) and ragged(t) and fast_eligible(t)
+    t = tables(0, P.PENTA10, P.DIRICHLET, 131, 1.0 / 131)  # a 3-row short chunk cannot hold 2W = 4 rows
+    assert t["MM"] == 16 and list(t["clen"]) == [15] * 8 + [11] and not fast_eligible(t)
+    assert not ragged(tables(0, P.PENTA10, P.PERIODIC, 20, 1 / 20))  # two chunks only
     from mpi_3dcompact_amd._lib import Mc3dError
     with pytest.raises(Mc3dError):
         tables(0, P.PENTA10, P.PERIODIC, 100, 1 / 100, mm=10)

@@ -90,6 +90,20 @@ def test_scheme_combinations_32(gpu, case):
     check_pair(g, o, extra=extra)
 
 
+@pytest.mark.parametrize("case", [
+    dict(fd="PENTA10", filt="COMPACT8", rk="KENRK4", les="VREMAN", les_avg="LOCAL", mu_ref=1e-3),
+    dict(fd="PADE6", filt="COMPACT10", rk="RK4", mu_ref=1e-3),
+], ids=["penta10-c8-kenrk4-vreman-local", "pade6-c10-rk4"])
+def test_ragged_box_36(gpu, case):
+    """A 36^3 box: 16 + 4 + 16-row chunks in every direction, i.e. the fast
+    kernels' ragged layout for the derivatives, the filter and the divergence's
+    accumulating launches."""
+    _say(f"case: 36^3 {case}, 6 steps")
+    g, o = run_pair(36, 6, seed=5, k0=3.0, **case)
+    extra = ("mu_sgs",) if case.get("les") == "VREMAN" else ()
+    check_pair(g, o, extra=extra)
+
+
 # Conditioning of the DSM, measured on the oracle itself by perturbing the
 # initial velocity by 1e-15 (relative): with GLOBAL or XI averaging every
 # output moves by <= 3e-12 after 6 steps at 32^3 (k0 = 3, M_t = 0.3).  LOCAL
