@@ -137,6 +137,8 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
         // the strided fast kernel addresses a field through a buffer descriptor
         // with 32-bit byte offsets
         const bool offsets_ok = npts * 8 < 2147483648L;
+        d.phases = 1;
+        d.phases = mc3d::lineop_fast_phases(d, (size_t)lds_limit);
         op->fast = (p.uniform || p.cshort >= 0) && rows_ok && offsets_ok &&
                    mc3d::lineop_fast_supported(d, (size_t)lds_limit);
         if (tune.family == 2) op->fast = false;
@@ -187,7 +189,8 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
     std::memcpy(op->dims, dims, sizeof(op->dims));
     const std::string per = p.periodic ? "true" : "false";
     if (op->fast)
-        op->kname = std::string(dir == 0 ? "lineop_contig<" : "lineop_strided<") + std::to_string(p.W) +
+        op->kname = std::string(dir != 0 ? "lineop_strided<" : op->dev.phases > 1 ? "lineop_contig_ph<" : "lineop_contig<") +
+                    std::to_string(p.W) +
                     ", " + std::to_string(p.H) + ", " + std::to_string(p.MM) + ", " + per +
                     (p.cshort >= 0 ? ", ragged>" : ">");
     else

@@ -58,6 +58,7 @@ struct OpDev {
     const double *G;      // P * 2W * NI
     int gb;               // interface band in chunks (Plan::gb)
     int cshort, lshort;   // ragged lines: chunk cshort has lshort rows (else cshort = -1)
+    int phases;           // x-lines in 64-line tiles: groups of chunks staged one after another (1: all)
     const int *cstart;    // P
     const int *clen;      // P
 };

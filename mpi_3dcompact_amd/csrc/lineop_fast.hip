@@ -497,11 +497,157 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
     }
 }
 
+// x-lines in 64-line tiles (banded Dirichlet or ragged operators, a wave per
+// chunk) whose whole-line tile exceeds LDS: the chunks are staged in
+// op.phases groups of cpp chunks, one after another through one tile of
+// cpp * MM + 2H rows.  Phase k loads the rows its chunks' windows span (zero or
+// wrapped outside the line), its waves read their windows, and the next phase
+// reuses the tile; the results go back the same way, a group at a time.  The
+// tile holds the rows of a group of lines [row][line] (row stride 65), filled
+// and drained as 64 contiguous segments of the workgroup's block with a
+// running (line, row) walk.
+template <int W, int H, int MM, bool PER, bool RG>
+__global__ __launch_bounds__(1024) void lineop_contig_ph(OpDev op, LaunchArgs a) {
+    extern __shared__ double smem[];
+    constexpr int L = 64, LS = L + 1, kB = 8;
+    const int N = op.N, P = op.P, NI = op.NI, K = op.phases;
+    const int cpp = (P + K - 1) / K;
+    double *tile = smem;
+    constexpr bool kShare = W == 2;  // interface values inside the tile (dead by then)
+    double *red = kShare ? smem : smem + (size_t)(cpp * MM + 2 * H) * LS;
+    const int l = threadIdx.x;
+    const int c = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int tid = threadIdx.y * L + l;
+    const int nthr = L * P;
+    const int q0 = blockIdx.x * L;
+    const int rem = a.nlines - q0;
+    const unsigned nrec = (unsigned)(rem < L ? rem : L) * (unsigned)N * 8u;
+    const __amdgpu_buffer_rsrc_t rin = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<double *>(a.in[blockIdx.y]) + (long)q0 * N, (short)0, (int)nrec, kRawBufferWord3);
+    const int q = q0 + l;
+    const bool live = q < a.nlines;
+    const int cs = chunk_start<MM, RG>(op, c);
+    const bool short_chunk = RG && c == op.cshort;
+    const int len = short_chunk ? op.lshort : MM;
+    const int myk = c / cpp;
+    double w[MM + 2 * H];
+    for (int k = 0; k < K; ++k) {
+        const int c0 = k * cpp;
+        if (c0 >= P) break;
+        const int c1 = c0 + cpp < P ? c0 + cpp : P;
+        const int r0 = chunk_start<MM, RG>(op, c0) - H;  // line row held by tile row 0
+        const int R = chunk_start<MM, RG>(op, c1 - 1) + MM + H - r0;
+        if (k > 0) __syncthreads();  // the previous group has read its windows
+        {
+            int ll = tid / R, rr = tid - ll * R;
+            const int qd = nthr / R, rd = nthr - qd * R;
+            for (int e0 = tid; e0 < L * R; e0 += kB * nthr) {
+                double v[kB];
+                int slot[kB];
+#pragma unroll
+                for (int b = 0; b < kB; ++b) {
+                    const int p = r0 + rr;
+                    const int pg = PER ? (p < 0 ? p + N : (p >= N ? p - N : p)) : p;
+                    const bool in_line = PER || (p >= 0 && p < N);
+                    v[b] = (ll < L && in_line) ? bload_nt(rin, (unsigned)(ll * N + pg) * 8u, 0) : 0.0;
+                    slot[b] = ll < L ? rr * LS + ll : -1;
+                    ll += qd;
+                    rr += rd;
+                    if (rr >= R) { rr -= R; ++ll; }
+                }
+#pragma unroll
+                for (int b = 0; b < kB; ++b)
+                    if (slot[b] >= 0) tile[slot[b]] = v[b];
+            }
+        }
+        __syncthreads();
+        if (myk == k) {
+            const double *trow = tile + (size_t)(cs - H - r0) * LS + l;
+#pragma unroll
+            for (int j = 0; j < MM + 2 * H; ++j) w[j] = trow[j * LS];
+        }
+    }
+    if (kShare) __syncthreads();  // every window is read before the interface values overwrite the tile
+    double y[MM];
+    chunk_rhs<H, MM, PER, has_rows<W, H, PER>()>(op, w, y, c == 0, c == P - 1);
+    if (W > 0) {
+        chunk_solve<W, MM>(op.lu + (PER && !RG ? 0 : (size_t)c * MM * kLuRec), y);
+        publish<W, MM, RG>(red + l * (NI + 1), c, short_chunk, len, y);
+    }
+    __syncthreads();
+    if (W > 0) {
+        const double *myred = red + l * (NI + 1);
+        double s[2 * W > 0 ? 2 * W : 1];
+        interface_sums<W, PER, RG>(op, c, myred, s);
+        const double *sp = op.sp + (PER && !RG ? 0 : (size_t)c * MM * 2 * W);
+#pragma unroll
+        for (int i = 0; i < MM; ++i) {
+            if (i % 8 == 0) __builtin_amdgcn_sched_barrier(0);
+            double x = y[i];
+#pragma unroll
+            for (int j = 0; j < 2 * W; ++j) x -= sp[i * 2 * W + j] * s[j];
+            y[i] = x;
+        }
+    }
+    // pass-through lines keep their input (nothing is stored yet)
+    if (live && pass_line(op, 0, q, a.Nx, a.Ny)) {
+#pragma unroll
+        for (int i = 0; i < MM; ++i) y[i] = bload(rin, (unsigned)(l * N + cs + i) * 8u, 0);
+    }
+    const __amdgpu_buffer_rsrc_t rout = __builtin_amdgcn_make_buffer_rsrc(
+        a.out[blockIdx.y] + (long)q0 * N, (short)0, (int)nrec, kRawBufferWord3);
+    for (int k = 0; k < K; ++k) {
+        const int c0 = k * cpp;
+        if (c0 >= P) break;
+        const int c1 = c0 + cpp < P ? c0 + cpp : P;
+        const int s0 = chunk_start<MM, RG>(op, c0);  // line row held by tile row 0
+        const int R = chunk_start<MM, RG>(op, c1 - 1) + (RG && c1 - 1 == op.cshort ? op.lshort : MM) - s0;
+        __syncthreads();  // interface values read (first group) or the previous group stored
+        if (myk == k && live) {
+#pragma unroll
+            for (int i = 0; i < MM; ++i)
+                if (!RG || i < len) tile[(cs - s0 + i) * LS + l] = y[i];
+        }
+        __syncthreads();
+        int ll = tid / R, rr = tid - ll * R;
+        const int qd = nthr / R, rd = nthr - qd * R;
+        for (int e0 = tid; e0 < L * R; e0 += kB * nthr) {
+            double v[kB];
+            unsigned off[kB];
+            bool ok[kB];
+#pragma unroll
+            for (int b = 0; b < kB; ++b) {
+                ok[b] = ll < L;
+                v[b] = ok[b] ? tile[rr * LS + ll] : 0.0;
+                off[b] = (unsigned)(ll * N + s0 + rr) * 8u;
+                ll += qd;
+                rr += rd;
+                if (rr >= R) { rr -= R; ++ll; }
+            }
+            if (a.accumulate) {
+#pragma unroll
+                for (int b = 0; b < kB; ++b)
+                    if (ok[b]) v[b] = bload_nt(rout, off[b], 0) + v[b];
+            }
+#pragma unroll
+            for (int b = 0; b < kB; ++b)
+                if (ok[b]) bstore(v[b], rout, off[b], 0);
+        }
+    }
+}
+
 template <int W, int H, int MM, bool PER, bool RG>
 hipError_t launch_fast(const OpDev &op, const LaunchArgs &a, int nfields, hipStream_t st) {
     if (op.dir == 0) {
         constexpr int L = contig_lines(PER, W, RG);
         dim3 grid((a.nlines + L - 1) / L, nfields), block(L, op.P);
+        if constexpr (L == 64) {
+            if (op.phases > 1) {
+                hipLaunchKernelGGL((lineop_contig_ph<W, H, MM, PER, RG>), grid, block, lineop_fast_lds_bytes(op), st,
+                                   op, a);
+                return hipGetLastError();
+            }
+        }
         hipLaunchKernelGGL((lineop_contig<W, H, MM, PER, RG>), grid, block, lineop_fast_lds_bytes(op), st, op, a);
     } else {
         dim3 grid((a.nlines + 63) / 64, nfields, op.dir == 2 ? a.nbatch : 1), block(64, op.P);
@@ -543,10 +689,22 @@ hipError_t launch_fast_mm(const OpDev &op, const LaunchArgs &a, int nfields, hip
 size_t lineop_fast_lds_bytes(const OpDev &op) {
     if (op.dir == 0) {  // the tile, with the interface values inside it (W = 2) or after it
         const size_t L = contig_lines(op.periodic, op.W, op.cshort >= 0);
-        const size_t tile = (size_t)(op.N + 2 * op.H) * (L + 1), red = L * (op.NI + 1);
+        const int K = L == 64 && op.phases > 1 ? op.phases : 1;
+        const size_t rows = K > 1 ? (size_t)((op.P + K - 1) / K) * op.MM + 2 * op.H : (size_t)op.N + 2 * op.H;
+        const size_t tile = rows * (L + 1), red = L * (op.NI + 1);
         return sizeof(double) * (op.W == 2 ? std::max(tile, red) : tile + red);
     }
     return sizeof(double) * (size_t)64 * (op.NI + 1);
+}
+
+int lineop_fast_phases(const OpDev &op, size_t lds_limit) {
+    if (op.dir != 0 || contig_lines(op.periodic, op.W, op.cshort >= 0) != 64) return 1;
+    for (int k = 1; k <= op.P; ++k) {
+        OpDev t = op;
+        t.phases = k;
+        if (lineop_fast_lds_bytes(t) <= lds_limit) return k;
+    }
+    return 1;
 }
 
 bool lineop_fast_supported(const OpDev &op, size_t lds_limit) {

@@ -31,6 +31,9 @@ hipError_t lineop_launch(const OpDev &op, const LaunchArgs &a, int nfields, int 
 // rows) or Dirichlet with dense special rows; no TPB
 size_t lineop_fast_lds_bytes(const OpDev &op);
 bool lineop_fast_supported(const OpDev &op, size_t lds_limit);
+// groups of chunks the 64-line x-line kernel stages one after another so that
+// its tile fits lds_limit (1: the whole line at once)
+int lineop_fast_phases(const OpDev &op, size_t lds_limit);
 hipError_t lineop_launch_fast(const OpDev &op, const LaunchArgs &a, int nfields, hipStream_t st);
 
 }  // namespace mc3d

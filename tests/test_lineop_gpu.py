@@ -114,9 +114,7 @@ def test_deriv_ragged_lines(torch_cuda, scheme, bc, shape):
         kname = op.kernel()
         _say(f"ragged deriv scheme={scheme} bc={bc} shape={shape} dir={d} kernel={kname}")
         t = plan_tables(0, scheme, bc, n, dd)
-        # (x-lines: a 64-line LDS tile of the whole line fits 160 KB up to about 300 points)
-        if (n % t["MM"] and t["P"] >= 3 and n - (t["P"] - 1) * t["MM"] >= max(2 * t["W"], 1) and n <= 512
-                and (d != 0 or n <= 256)):
+        if n % t["MM"] and t["P"] >= 3 and n - (t["P"] - 1) * t["MM"] >= max(2 * t["W"], 1) and n <= 512:
             assert kname.startswith("lineop_contig" if d == 0 else "lineop_strided"), kname
         out = torch.empty_like(fin)
         op.apply([fin], [out])
@@ -148,6 +146,49 @@ def test_filter_ragged_lines(torch_cuda, kind, af, bc3, shape):
         op.apply([fl], [fl])
         torch.cuda.synchronize()
         assert relerr(fl.cpu().numpy(), ref) <= TOL, (kind, bc3, d, "in place")
+
+
+@pytest.mark.parametrize("nx", [310, 384, 512])
+@pytest.mark.parametrize("bc", [P.PERIODIC, P.DIRICHLET])
+def test_long_x_lines_in_phases(torch_cuda, nx, bc):
+    """x-lines whose 64-line tile exceeds LDS (banded Dirichlet operators, and
+    ragged periodic ones): the chunks are staged in groups through one tile
+    (lineop_contig_ph); derivatives and filters against the oracle, pass-through
+    lines and in-place filtering included."""
+    from mpi_3dcompact_amd.ops import LineOp
+    torch = torch_cuda
+    shape = (12, 18, nx)
+    Nz, Ny, Nx = shape
+    dims = (Nx, Ny, Nz)
+    f = field(shape, 9)
+    fin = torch.from_numpy(f).cuda()
+    out = torch.empty_like(fin)
+    dd = 1.0 / nx if bc == P.PERIODIC else 1.0 / (nx - 1)
+    phased = bc == P.DIRICHLET or nx == 310
+    for scheme in (P.PADE6, P.PENTA10):
+        op = LineOp.deriv(scheme, 0, bc, dims, dd)
+        kname = op.kernel()
+        _say(f"long x deriv nx={nx} bc={bc} scheme={scheme} kernel={kname}")
+        if phased:
+            assert kname.startswith("lineop_contig_ph<"), kname
+        op.apply([fin], [out])
+        torch.cuda.synchronize()
+        assert relerr(out.cpu().numpy(), P.deriv_field(scheme, 0, bc, f, dd)) <= TOL, (scheme, nx, bc)
+    bc3 = (bc, 1, 1)
+    for kind, af in ((P.COMPACT8, 0.49), (P.COMPACT10, -0.49), (P.COMMN5PADE, 0.0)):
+        op = LineOp.filter(kind, af, 0, bc3, dims)
+        kname = op.kernel()
+        _say(f"long x filter nx={nx} bc3={bc3} kind={kind} kernel={kname}")
+        if phased:
+            assert kname.startswith("lineop_contig_ph<"), kname
+        ref = P.filter_field(kind, af, 0, list(bc3), f)
+        op.apply([fin], [out])
+        torch.cuda.synchronize()
+        assert relerr(out.cpu().numpy(), ref) <= TOL, (kind, nx, bc)
+        fl = torch.from_numpy(f).cuda()
+        op.apply([fl], [fl])
+        torch.cuda.synchronize()
+        assert relerr(fl.cpu().numpy(), ref) <= TOL, (kind, nx, bc, "in place")
 
 
 def test_batched_fields_and_inplace_filter(torch_cuda):
