@@ -189,7 +189,8 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
     std::memcpy(op->dims, dims, sizeof(op->dims));
     const std::string per = p.periodic ? "true" : "false";
     if (op->fast)
-        op->kname = std::string(dir != 0 ? "lineop_strided<" : op->dev.phases > 1 ? "lineop_contig_ph<" : "lineop_contig<") +
+        op->kname = std::string(dir != 0 ? (p.P > 16 ? "lineop_strided_multi<" : "lineop_strided<")
+                                         : op->dev.phases > 1 ? "lineop_contig_ph<" : "lineop_contig<") +
                     std::to_string(p.W) +
                     ", " + std::to_string(p.H) + ", " + std::to_string(p.MM) + ", " + per +
                     (p.cshort >= 0 ? ", ragged>" : ">");

@@ -317,6 +317,157 @@ __global__ __launch_bounds__(1024) void lineop_strided(OpDev op, LaunchArgs a) {
         if (!RG || i < len) bstore(y[i], rout, boff, (unsigned)(cs + i) * rowb);
 }
 
+// y- and z-lines of more than 16 chunks of 16 rows (640 or 1024 points, say): a
+// workgroup of 64 lines x 16 chunk waves, each wave taking chunks ty, ty + 16,
+// ... in rounds.  Pass 1 computes every chunk's explicit stage and block solve
+// and publishes its interface values; pass 2 recomputes each chunk (its window
+// read again) and applies the interface correction.  Nothing is held across
+// chunks but the interface values, so the registers stay those of one chunk;
+// the price is a second read of the input and a second explicit stage and
+// solve.  Within a round every window is read before any of the round's
+// stores; the rows a window needs from a chunk of an earlier round (the H rows
+// below each round's first chunk, and the line's first H rows for the
+// periodic wrap of the last chunk) are stashed in LDS in pass 1, so that the
+// operator may run in place.
+template <int W, int H, int MM, bool PER, bool RG>
+__global__ __launch_bounds__(1024) void lineop_strided_multi(OpDev op, LaunchArgs a) {
+    extern __shared__ double smem[];
+    constexpr int CW = 16;  // chunk waves per workgroup
+    const int l = threadIdx.x;
+    const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int P = op.P, NI = op.NI, N = op.N;
+    const int rounds = (P + CW - 1) / CW;
+    double *red = smem;                               // [64][NI + 1]
+    double *stash = smem + (size_t)64 * (NI + 1);     // [64][rounds][H]
+    const int q = blockIdx.x * 64 + l;
+    const bool live = q < a.nlines;
+    const int qq = live ? q : 0;
+    const double *__restrict__ in = a.in[blockIdx.y];
+    double *__restrict__ out = a.out[blockIdx.y];
+    long base, ps;
+    unsigned fbytes = (unsigned)a.Nx * (unsigned)a.Ny * (unsigned)a.Nz * 8u;  // one member
+    if (op.dir == 1) {
+        base = (long)(qq / a.Nx) * a.Nx * a.Ny + qq % a.Nx;
+        ps = a.Nx;
+        fbytes *= (unsigned)a.nbatch;
+    } else {
+        const long member = (long)blockIdx.z * a.Nx * a.Ny * a.Nz;
+        in += member;
+        out += member;
+        base = qq;
+        ps = (long)a.Nx * a.Ny;
+    }
+    const __amdgpu_buffer_rsrc_t rin =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(in), (short)0, (int)fbytes, kRawBufferWord3);
+    const __amdgpu_buffer_rsrc_t rout =
+        __builtin_amdgcn_make_buffer_rsrc(out, (short)0, (int)fbytes, kRawBufferWord3);
+    const unsigned boff = (unsigned)base * 8u, rowb = (unsigned)ps * 8u;
+    double *mystash = stash + (size_t)l * rounds * H;
+    // the window of chunk c: rows cs - H .. cs + MM + H - 1 (pass 2 takes the
+    // rows stored in an earlier round from the stash)
+    auto window = [&](int c, bool pass2, double *w) {
+        const int s0 = chunk_start<MM, RG>(op, c) - H;
+        const bool first = c == 0, last = c == P - 1;
+        const bool below = pass2 && c > 0 && c % CW == 0;        // rows of chunk c - 1, an earlier round
+        const bool wrap = pass2 && PER && last && rounds > 1;    // rows 0 .. H - 1, stored in round 0
+#pragma unroll
+        for (int j = 0; j < MM + 2 * H; ++j) {
+            if (j < H) {
+                if (below) {
+                    w[j] = mystash[(c / CW) * H + j];
+                } else if (first) {
+                    w[j] = PER ? bload(rin, boff, (unsigned)(s0 + j + N) * rowb) : 0.0;
+                } else {
+                    w[j] = bload(rin, boff, (unsigned)(s0 + j) * rowb);
+                }
+            } else if (j >= MM + H) {
+                if (wrap) {
+                    w[j] = mystash[j - MM - H];
+                } else if (last) {
+                    w[j] = PER ? bload(rin, boff, (unsigned)(s0 + j - N) * rowb) : 0.0;
+                } else {
+                    w[j] = bload(rin, boff, (unsigned)(s0 + j) * rowb);
+                }
+            } else {
+                w[j] = bload(rin, boff, (unsigned)(s0 + j) * rowb);
+            }
+        }
+    };
+    // pass 1: interface values of every chunk, and the stash
+#pragma unroll 1
+    for (int r = 0; r < rounds; ++r) {
+        const int c = ty + CW * r;
+        // (explicit operators read a window here only for the stash)
+        if (c < P && (W > 0 || c % CW == CW - 1 || (PER && c == 0))) {
+            double w[MM + 2 * H];
+            window(c, false, w);
+            if (c % CW == CW - 1 && c + 1 < P) {  // the H rows below chunk c + 1 (c is a full chunk)
+#pragma unroll
+                for (int j = 0; j < H; ++j) mystash[((c + 1) / CW) * H + j] = w[MM + j];
+            }
+            if (PER && c == 0) {  // the line's first H rows
+#pragma unroll
+                for (int j = 0; j < H; ++j) mystash[j] = w[H + j];
+            }
+            if (W > 0) {
+                double y[MM];
+                chunk_rhs<H, MM, PER, has_rows<W, H, PER>()>(op, w, y, c == 0, c == P - 1);
+                chunk_solve<W, MM>(op.lu + (PER && !RG ? 0 : (size_t)c * MM * kLuRec), y);
+                publish<W, MM, RG>(red + l * (NI + 1), c, RG && c == op.cshort, RG && c == op.cshort ? op.lshort : MM,
+                                   y);
+            }
+        }
+    }
+    __syncthreads();
+    // pass 2: a round at a time, every window read before the round's stores.
+    // The interface sums come first: computed after the block solve, they made
+    // the compiler spill the chunk's registers to scratch.
+#pragma unroll 1
+    for (int r = 0; r < rounds; ++r) {
+        const int c = ty + CW * r;
+        const bool has = c < P;
+        const int cs = has ? chunk_start<MM, RG>(op, c) : 0;
+        const int len = RG && c == op.cshort ? op.lshort : MM;
+        double y[MM];
+        double s[2 * W > 0 ? 2 * W : 1];
+        if (has && W > 0) interface_sums<W, PER, RG>(op, c, red + l * (NI + 1), s);
+        if (has) {
+            double w[MM + 2 * H];
+            window(c, true, w);
+            __builtin_amdgcn_sched_barrier(0);
+            chunk_rhs<H, MM, PER, has_rows<W, H, PER>()>(op, w, y, c == 0, c == P - 1);
+            if (W > 0) {
+                chunk_solve<W, MM>(op.lu + (PER && !RG ? 0 : (size_t)c * MM * kLuRec), y);
+                const double *sp = op.sp + (PER && !RG ? 0 : (size_t)c * MM * 2 * W);
+#pragma unroll
+                for (int i = 0; i < MM; ++i) {
+                    if (i % 8 == 0) __builtin_amdgcn_sched_barrier(0);
+                    double x = y[i];
+#pragma unroll
+                    for (int j = 0; j < 2 * W; ++j) x -= sp[i * 2 * W + j] * s[j];
+                    y[i] = x;
+                }
+            }
+        }
+        __syncthreads();  // the round's windows are read (in may equal out)
+        if (has && live) {
+            if (pass_line(op, op.dir, q, a.Nx, a.Ny)) {
+#pragma unroll
+                for (int i = 0; i < MM; ++i)
+                    if (!RG || i < len) y[i] = bload(rin, boff, (unsigned)(cs + i) * rowb);
+            }
+            if (a.accumulate) {
+#pragma unroll
+                for (int i = 0; i < MM; ++i)
+                    if (!RG || i < len) y[i] = bload_nt(rout, boff, (unsigned)(cs + i) * rowb) + y[i];
+            }
+#pragma unroll
+            for (int i = 0; i < MM; ++i)
+                if (!RG || i < len) bstore(y[i], rout, boff, (unsigned)(cs + i) * rowb);
+        }
+    }
+}
+
 // lines per workgroup of the contiguous kernel: 16 for periodic lines (every
 // table read is chunk independent, so a wave may span chunks; the smaller tile
 // lets four workgroups of 256-point lines share a CU, which overlaps one
@@ -650,9 +801,16 @@ hipError_t launch_fast(const OpDev &op, const LaunchArgs &a, int nfields, hipStr
         }
         hipLaunchKernelGGL((lineop_contig<W, H, MM, PER, RG>), grid, block, lineop_fast_lds_bytes(op), st, op, a);
     } else {
+        if constexpr (MM == 16) {
+            if (op.P > 16) {
+                dim3 grid((a.nlines + 63) / 64, nfields, op.dir == 2 ? a.nbatch : 1), block(64, 16);
+                hipLaunchKernelGGL((lineop_strided_multi<W, H, MM, PER, RG>), grid, block, lineop_fast_lds_bytes(op),
+                                   st, op, a);
+                return hipGetLastError();
+            }
+        }
         dim3 grid((a.nlines + 63) / 64, nfields, op.dir == 2 ? a.nbatch : 1), block(64, op.P);
-        const size_t lds = sizeof(double) * (size_t)64 * (op.NI + 1);
-        hipLaunchKernelGGL((lineop_strided<W, H, MM, PER, RG>), grid, block, lds, st, op, a);
+        hipLaunchKernelGGL((lineop_strided<W, H, MM, PER, RG>), grid, block, lineop_fast_lds_bytes(op), st, op, a);
     }
     return hipGetLastError();
 }
@@ -694,7 +852,9 @@ size_t lineop_fast_lds_bytes(const OpDev &op) {
         const size_t tile = rows * (L + 1), red = L * (op.NI + 1);
         return sizeof(double) * (op.W == 2 ? std::max(tile, red) : tile + red);
     }
-    return sizeof(double) * (size_t)64 * (op.NI + 1);
+    // the interface values, and beyond 16 chunks the stash of rows between rounds
+    const size_t stash = op.P > 16 ? (size_t)64 * ((op.P + 15) / 16) * op.H : 0;
+    return sizeof(double) * ((size_t)64 * (op.NI + 1) + stash);
 }
 
 int lineop_fast_phases(const OpDev &op, size_t lds_limit) {
@@ -713,7 +873,10 @@ bool lineop_fast_supported(const OpDev &op, size_t lds_limit) {
                       (w == 2 && (h == 3 || h == 5));
     // workgroup size as launched: (lines per workgroup) x P
     const int lines = op.dir != 0 ? 64 : contig_lines(op.periodic, op.W, op.cshort >= 0);
-    const int threads = lines * op.P;
+    // y- and z-lines of more than 16 chunks: 16-row chunks, 16 chunk waves in
+    // rounds (up to 64 chunks); the 32-row chunk state spills in that kernel
+    const bool multi = op.dir != 0 && op.P > 16;
+    const int threads = multi ? (op.MM == 16 && op.P <= 64 ? 1024 : 2048) : lines * op.P;
     // special rows: H at each end of a Dirichlet line (every scheme has that
     // many); on periodic lines only where the kernels compile them in (CD2)
     const bool ends = op.periodic ? (op.nbl + op.nbr == 0 || (w == 0 && h == 1 && op.nbl == h && op.nbr == h))

@@ -191,6 +191,45 @@ def test_long_x_lines_in_phases(torch_cuda, nx, bc):
         assert relerr(fl.cpu().numpy(), ref) <= TOL, (kind, nx, bc, "in place")
 
 
+@pytest.mark.parametrize("shape", [(576, 40, 12), (650, 12, 16)])
+def test_long_yz_lines_in_rounds(torch_cuda, shape):
+    """y- and z-lines of more than sixteen 16-row chunks (576 = 36 x 16,
+    650 = 16 + 10 + 39 x 16): 16 chunk waves take the chunks in rounds
+    (lineop_strided_multi); every operator against the oracle, periodic and
+    Dirichlet, in place too."""
+    from mpi_3dcompact_amd.ops import LineOp
+    torch = torch_cuda
+    Nz, Ny, Nx = shape
+    dims = (Nx, Ny, Nz)
+    f = field(shape, 12)
+    fin = torch.from_numpy(f).cuda()
+    out = torch.empty_like(fin)
+    for bc in (P.PERIODIC, P.DIRICHLET):
+        dd = 1.0 / Nz if bc == P.PERIODIC else 1.0 / (Nz - 1)
+        for scheme in (P.PADE6, P.PENTA10, P.CD2):
+            op = LineOp.deriv(scheme, 2, bc, dims, dd)
+            kname = op.kernel()
+            _say(f"long z deriv shape={shape} bc={bc} scheme={scheme} kernel={kname}")
+            assert kname.startswith("lineop_strided_multi<2" if scheme == P.PENTA10 else "lineop_strided_multi<"), kname
+            op.apply([fin], [out])
+            torch.cuda.synchronize()
+            assert relerr(out.cpu().numpy(), P.deriv_field(scheme, 2, bc, f, dd)) <= TOL, (scheme, bc)
+        bc3 = (0, bc, bc)
+        for kind, af in ((P.COMPACT8, 0.49), (P.COMPACT10, -0.49), (P.COMMN5PADE, 0.0), (P.COMMN5EXP, 0.0)):
+            op = LineOp.filter(kind, af, 2, bc3, dims)
+            kname = op.kernel()
+            _say(f"long z filter shape={shape} bc3={bc3} kind={kind} kernel={kname}")
+            assert kname.startswith("lineop_strided_multi<"), kname
+            ref = P.filter_field(kind, af, 2, list(bc3), f)
+            op.apply([fin], [out])
+            torch.cuda.synchronize()
+            assert relerr(out.cpu().numpy(), ref) <= TOL, (kind, bc)
+            fl = torch.from_numpy(f).cuda()
+            op.apply([fl], [fl])
+            torch.cuda.synchronize()
+            assert relerr(fl.cpu().numpy(), ref) <= TOL, (kind, bc, "in place")
+
+
 def test_batched_fields_and_inplace_filter(torch_cuda):
     from mpi_3dcompact_amd.ops import LineOp
     torch = torch_cuda

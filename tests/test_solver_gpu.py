@@ -104,6 +104,30 @@ def test_ragged_box_36(gpu, case):
     check_pair(g, o, extra=extra)
 
 
+def test_long_z_box(gpu):
+    """A 12 x 12 x 576 box: z-lines of 36 16-row chunks take the multi-round
+    strided kernel, the zeta divergence's accumulating launches included;
+    against the oracle after four steps."""
+    from mpi_3dcompact_amd.solver import Solver
+    Nx, Ny, Nz = 12, 12, 576
+    z, y, x = np.meshgrid(np.arange(Nz) / Nz, np.arange(Ny) / Ny, np.arange(Nx) / Nx, indexing="ij")
+    tp = 2 * np.pi
+    ic = dict(rho=1.0 + 0.02 * np.sin(tp * (x + 3 * z)), U=0.2 * np.sin(tp * (y + 5 * z)),
+              V=0.2 * np.cos(tp * (x - 4 * z)), W=0.2 * np.sin(tp * (x + y + 7 * z)),
+              p=(1.0 + 0.03 * np.cos(tp * (2 * z - y))) / 1.4)
+    L = 2 * np.pi
+    one, zero = np.ones((Nz, Ny, Nx)), np.zeros((Nz, Ny, Nx))
+    met = [one / L ** 3, one / L, zero, zero, zero, one / L, zero, zero, zero, one / L]
+    case = dict(fd="PENTA10", filt="COMPACT8", rk="RK4", mu_ref=1e-3, dims=(Nx, Ny, Nz))
+    g = Solver(hit.options_text(Nz, **case), (Nz, Ny, Nx))
+    g.set_metrics(met)
+    g.set_initial(ic)
+    g.step(4)
+    o = P.Solver(hit.oracle_config(Nz, **case), met, ic)
+    o.steps(4)
+    check_pair(g, o)
+
+
 # Conditioning of the DSM, measured on the oracle itself by perturbing the
 # initial velocity by 1e-15 (relative): with GLOBAL or XI averaging every
 # output moves by <= 3e-12 after 6 steps at 32^3 (k0 = 3, M_t = 0.3).  LOCAL
