@@ -302,18 +302,41 @@ __global__ __launch_bounds__(256) void k_mlp_grad(const float *__restrict__ P, c
     }
 }
 
-__global__ void k_mlp_reduce(const float *__restrict__ partial, const float *__restrict__ lossp, int nb,
-                             float inv_norm, float *__restrict__ grad, float *__restrict__ loss) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+// Sum of the nb partial gradients of every parameter and of the partial
+// losses, in a fixed order (the result does not depend on scheduling): a
+// workgroup takes 64 parameters, its four waves each sum a quarter of the
+// partials in order (rows of 64 consecutive floats), and the four sums are
+// added in wave order.  Workgroup 0 also sums the partial losses: strided
+// per thread, then a fixed tree.  (One thread per parameter summing all nb
+// partials kept 35 workgroups busy and took 35 us per batch.)
+constexpr int kReduceCols = 64;
+__global__ __launch_bounds__(256) void k_mlp_reduce(const float *__restrict__ partial,
+                                                    const float *__restrict__ lossp, int nb, float inv_norm,
+                                                    float *__restrict__ grad, float *__restrict__ loss) {
+    __shared__ float part[4][kReduceCols];
+    __shared__ float lsum[256];
+    const int t = threadIdx.x % kReduceCols, g = threadIdx.x / kReduceCols;
+    const int p = blockIdx.x * kReduceCols + t;
+    const int per = (nb + 3) / 4, b0 = g * per, b1 = b0 + per < nb ? b0 + per : nb;
+    float acc = 0.0f;
     if (p < kMlpParams) {
-        float acc = 0.0f;
-        for (int b = 0; b < nb; ++b) acc = acc + partial[(size_t)b * kMlpParams + p];
-        grad[p] = acc;
+#pragma unroll 8
+        for (int b = b0; b < b1; ++b) acc = acc + partial[(size_t)b * kMlpParams + p];
     }
-    if (p == 0) {
-        float acc = 0.0f;
-        for (int b = 0; b < nb; ++b) acc = acc + lossp[b];
-        loss[0] = acc * inv_norm;
+    part[g][t] = acc;
+    if (blockIdx.x == 0) {
+        float l = 0.0f;
+        for (int b = threadIdx.x; b < nb; b += 256) l = l + lossp[b];
+        lsum[threadIdx.x] = l;
+    }
+    __syncthreads();
+    if (g == 0 && p < kMlpParams) grad[p] = ((part[0][t] + part[1][t]) + part[2][t]) + part[3][t];
+    if (blockIdx.x == 0) {
+        for (int h = 128; h > 0; h >>= 1) {
+            if (threadIdx.x < h) lsum[threadIdx.x] = lsum[threadIdx.x] + lsum[threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) loss[0] = lsum[0] * inv_norm;
     }
 }
 
@@ -833,8 +856,8 @@ hipError_t launch_mlp_grad_partials(const float *params, const float *X, const f
 hipError_t launch_mlp_reduce(const float *partial, const float *lossp, int nblocks, long n_total, float *grad,
                              float *loss, hipStream_t st) {
     const float inv_norm = (float)(1.0 / ((double)n_total * kMlpOut));
-    hipLaunchKernelGGL(k_mlp_reduce, dim3((kMlpParams + 255) / 256), dim3(256), 0, st, partial, lossp, nblocks,
-                       inv_norm, grad, loss);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3((kMlpParams + kReduceCols - 1) / kReduceCols), dim3(256), 0, st, partial,
+                       lossp, nblocks, inv_norm, grad, loss);
     return hipGetLastError();
 }
 

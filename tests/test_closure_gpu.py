@@ -111,6 +111,14 @@ def test_training_fits_a_smooth_map(torch):
     assert held < 2.0 * losses[-1] + 1e-3
 
 
+def test_closure_bench_runs(torch, capsys):
+    """The training/inference throughput tool runs and reports both rates."""
+    from mpi_3dcompact_amd import closure_bench
+    assert closure_bench.main(["--samples", "65536", "--epochs", "1", "--batch", "8192"]) == 0
+    out = capsys.readouterr().out
+    assert "us per batch" in out and "G pairs/s" in out
+
+
 def test_save_and_load(torch, tmp_path):
     c, p = _net(5)
     c.set_scaling(np.arange(7) * 0.1, np.arange(1, 8) * 0.5, [0.1, 0.2], [0.3, 0.4], -np.arange(1, 8),
