@@ -2,6 +2,7 @@
 
     python -m mpi_3dcompact_amd.opbench [--n 256] [--fields 5] [--reps 20]
                                         [--only penta10] [--bc periodic]
+    python -m mpi_3dcompact_amd.opbench --kernel-level [--n 256]
 
 For every scheme, direction and boundary type it applies one operator to
 ``--fields`` 256^3 FP64 fields (one batched launch) and prints the mean time
@@ -11,6 +12,12 @@ field and of ``--fields`` fields are timed first: the many-field copy is the
 practical bandwidth reference for a launch of that many fields, and the
 one-field copy's hardware counters (rocprofv3 FETCH_SIZE / WRITE_SIZE)
 calibrate those of the operators.
+
+--kernel-level times the kernel-level workload as one pass: Penta10 first
+derivatives in xi, eta and zeta and the Compact8 filter (alpha_f 0.49) in all
+three directions, each on five fields, periodic in xi and with the boundary
+closures in eta and zeta (six launches), and prints the pass time and its
+aggregate algorithmic bandwidth (6 x 16 B per point per field).
 """
 import argparse
 import sys
@@ -33,6 +40,7 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--only", default="", help="comma-separated scheme names")
     ap.add_argument("--bc", default="periodic,dirichlet")
+    ap.add_argument("--kernel-level", action="store_true", help="the six-launch kernel-level pass only")
     args = ap.parse_args(argv)
 
     import torch
@@ -58,6 +66,26 @@ def main(argv=None):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / args.reps
 
+    if args.kernel_level:
+        bc3 = (0, 1, 1)  # periodic xi, closures in eta and zeta
+        ops = [LineOp.deriv(2, d, bc3[d], dims, 1.0 / N if bc3[d] == 0 else 1.0 / (N - 1)) for d in range(3)]
+        ops += [LineOp.filter(0, 0.49, d, bc3, dims) for d in range(3)]
+        derivs = [[torch.empty_like(f) for f in fs] for _ in range(3)]
+        filt = [torch.empty_like(f) for f in fs]
+
+        def one_pass():
+            for d in range(3):
+                ops[d].apply(fs, derivs[d])
+            ops[3].apply(fs, filt)
+            ops[4].apply(filt, filt)
+            ops[5].apply(filt, filt)
+
+        ms = timeit(one_pass)
+        for op in ops:
+            print(f"  {op.kernel()}", flush=True)
+        print(f"kernel-level pass at {N}^3, {nf} fields: {ms:.3f} ms, {6 * 16 * npts * nf / (ms * 1e-3) / 1e9:.0f} GB/s "
+              f"algorithmic, {npts / (ms * 1e-3) / 1e6:.0f} Mpt/s", flush=True)
+        return 0
     ms = timeit(lambda: outs[0].copy_(fs[0]))
     print(f"copy {N}^3 f64 (1 field): {ms * 1e3:8.1f} us  {16 * npts / (ms * 1e-3) / 1e9:7.0f} GB/s",
           flush=True)
