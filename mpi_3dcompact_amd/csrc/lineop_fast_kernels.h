@@ -488,14 +488,13 @@ __host__ __device__ constexpr int contig_lines(bool per, int w, bool rg) {
     return w == 0 || (per && !rg) ? 16 : 64;
 }
 
-// loads in flight per thread while staging.  The tile caps a CU at one or two
-// workgroups, so the fill is latency bound: 16 at a time where the registers
-// allow it (8 at a time left Penta10 x at 3.4 TB/s); the 16-row Dirichlet
-// kernels (64-line tiles of 1024 threads, 128 VGPRs) and the 16-row H = 5
-// periodic ones spill with 16
-template <int H, int MM, bool PER>
+// loads in flight per thread while staging.  The tile caps a CU at one to four
+// workgroups, so the fill is latency bound: 16 at a time (8 at a time left
+// Penta10 x at 3.4 TB/s; the 16-row Dirichlet kernels and the H = 5 periodic
+// ones now fit 16 in 78-104 VGPRs, 1-3% faster than 8)
+template <int MM>
 constexpr int contig_batch() {
-    return MM % 16 ? (MM <= 12 ? MM : MM / 2) : (MM == 32 || (PER && H < 5) ? 16 : 8);
+    return MM % 16 ? (MM <= 12 ? MM : MM / 2) : 16;
 }
 
 template <int W, int H, int MM, bool PER, bool RG>
@@ -533,7 +532,7 @@ __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
     const int slot0 = (p0 + H) * LS + ll0;
     const int qd = nthr / N, rd = nthr - qd * N;
     const int step = rd * LS + qd, wrapfix = 1 - N * LS;
-    constexpr int kB = contig_batch<H, MM, PER && !RG>();
+    constexpr int kB = contig_batch<MM>();
     // fill: batches of coalesced loads, then their LDS writes
     {
         // periodic halo rows: phi[N - H .. N) above row H, phi[0 .. H) below row N + H
