@@ -38,10 +38,14 @@ constexpr float kG0 = 0.7978845608028654f;  // sqrt(2 / pi)
 constexpr float kG1 = 0.044715f;
 
 // tanh-form GELU 0.5 z (1 + tanh u), u = k0 (z + k1 z^3), written as z s with
-// s = 1 / (1 + exp(-2u)) = (1 + tanh u) / 2
+// s = 1 / (1 + exp(-2u)) = (1 + tanh u) / 2 and exp(-2u) = 2^(z (c1 + c2 z^2)),
+// c1 = -2 k0 log2(e), c2 = c1 k1: three FP32 operations, the hardware 2^x and
+// reciprocal (the network's evaluation is bound by these vector operations)
+constexpr float kGc1 = (float)(-2.0 * 0.7978845608028654 * 1.4426950408889634);
+constexpr float kGc2 = (float)(-2.0 * 0.7978845608028654 * 1.4426950408889634 * 0.044715);
 __device__ __forceinline__ float gelu_s(float z) {
-    const float u = kG0 * (z + kG1 * z * z * z);
-    return __builtin_amdgcn_rcpf(1.0f + __expf(-2.0f * u));
+    const float z2 = z * z;
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * (kGc1 + kGc2 * z2)));
 }
 __device__ __forceinline__ float gelu(float z) { return z * gelu_s(z); }
 // d/dz: s + z 2 s (1 - s) k0 (1 + 3 k1 z^2)
@@ -690,6 +694,11 @@ __global__ __launch_bounds__(256) void k_closure_features(FeatureArgs a, long n)
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_learned_sgs(ClosureArgs a, long n) {
     __shared__ InferLds net;
+    // the input range and standardisation in LDS: read per group instead of
+    // held in scalar registers, which spilled to vector-register lanes
+    __shared__ MlpScaling sc;
+    if (threadIdx.x < sizeof(MlpScaling) / sizeof(float))
+        reinterpret_cast<float *>(&sc)[threadIdx.x] = reinterpret_cast<const float *>(&a.sc)[threadIdx.x];
     stage_infer(a.params, net);
     __shared__ int wcount[4];
     const int l = lane_id(), w = threadIdx.x >> 6;
@@ -721,11 +730,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             }
 #pragma unroll
             for (int k = 0; k < kMlpIn; ++k) {
-                const bool in = f[k] >= a.sc.lo[k] && f[k] <= a.sc.hi[k];
+                const bool in = f[k] >= sc.lo[k] && f[k] <= sc.hi[k];
                 out = out || !in;
                 // outside the trusted range the input is clamped to it
-                const float fc = in ? f[k] : (f[k] < a.sc.lo[k] ? a.sc.lo[k] : a.sc.hi[k]);
-                x[k] = (fc - a.sc.in_mean[k]) / a.sc.in_std[k];
+                const float fc = in ? f[k] : (f[k] < sc.lo[k] ? sc.lo[k] : sc.hi[k]);
+                x[k] = (fc - sc.in_mean[k]) / sc.in_std[k];
             }
             base = (a.rho[p] * d2) * Smag;
         } else {
@@ -735,8 +744,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         float y0 = 0.0f, y1 = 0.0f;
         infer64_lds(net, x, y0, y1);
         if (live) {
-            const float C = y0 * a.sc.out_std[0] + a.sc.out_mean[0];
-            const float D = y1 * a.sc.out_std[1] + a.sc.out_mean[1];
+            const float C = y0 * sc.out_std[0] + sc.out_mean[0];
+            const float D = y1 * sc.out_std[1] + sc.out_mean[1];
             a.mu_sgs[p] = base * (double)C;
             a.k_sgs[p] = (a.cp * base) * (double)D;
             a.flag[p] = out ? 1 : 0;
