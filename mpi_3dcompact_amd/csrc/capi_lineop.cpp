@@ -188,8 +188,9 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
     op->nbatch = nbatch;
     std::memcpy(op->dims, dims, sizeof(op->dims));
     const std::string per = p.periodic ? "true" : "false";
+    const bool pair = p.periodic && p.cshort < 0 && p.nbl + p.nbr == 0 && p.P > 16 && p.P <= 32;
     if (op->fast)
-        op->kname = std::string(dir != 0 ? (p.P > 16 ? "lineop_strided_multi<" : "lineop_strided<")
+        op->kname = std::string(dir != 0 ? (pair ? "lineop_strided_pair<" : p.P > 16 ? "lineop_strided_multi<" : "lineop_strided<")
                                          : op->dev.phases > 1 ? "lineop_contig_ph<" : "lineop_contig<") +
                     std::to_string(p.W) +
                     ", " + std::to_string(p.H) + ", " + std::to_string(p.MM) + ", " + per +

@@ -5,6 +5,14 @@
 
 namespace mc3d {
 
+namespace {
+// y- and z-lines that lineop_strided_pair takes: evenly chunked periodic lines
+// of 17 to 32 chunks with no special rows
+bool strided_pair(const OpDev &op) {
+    return op.dir != 0 && op.periodic && op.cshort < 0 && op.nbl + op.nbr == 0 && op.P > 16 && op.P <= 32;
+}
+}  // namespace
+
 size_t lineop_fast_lds_bytes(const OpDev &op) {
     if (op.dir == 0) {  // the tile, with the interface values inside it (W = 2) or after it
         const size_t L = contig_lines(op.periodic, op.W, op.cshort >= 0);
@@ -13,7 +21,9 @@ size_t lineop_fast_lds_bytes(const OpDev &op) {
         const size_t tile = rows * (L + 1), red = L * (op.NI + 1);
         return sizeof(double) * (op.W == 2 ? std::max(tile, red) : tile + red);
     }
-    // the interface values, and beyond 16 chunks the stash of rows between rounds
+    // the interface values (32 lines when two chunks share a wave), and beyond
+    // 16 chunks in rounds the stash of rows between rounds
+    if (strided_pair(op)) return sizeof(double) * (size_t)32 * (op.NI + 1);
     const size_t stash = op.P > 16 ? (size_t)64 * ((op.P + 15) / 16) * op.H : 0;
     return sizeof(double) * ((size_t)64 * (op.NI + 1) + stash);
 }
@@ -34,10 +44,13 @@ bool lineop_fast_supported(const OpDev &op, size_t lds_limit) {
                       (w == 2 && (h == 3 || h == 5));
     // workgroup size as launched: (lines per workgroup) x P
     const int lines = op.dir != 0 ? 64 : contig_lines(op.periodic, op.W, op.cshort >= 0);
-    // y- and z-lines of more than 16 chunks: 16-row chunks, 16 chunk waves in
-    // rounds (up to 64 chunks); the 32-row chunk state spills in that kernel
+    // y- and z-lines of more than 16 chunks: evenly chunked periodic ones up to
+    // 32 chunks two to a wave; else 16-row chunks, 16 chunk waves in rounds (up
+    // to 64 chunks; the 32-row chunk state spills in that kernel)
     const bool multi = op.dir != 0 && op.P > 16;
-    const int threads = multi ? (op.MM == 16 && op.P <= 64 ? 1024 : 2048) : lines * op.P;
+    const int threads = strided_pair(op) ? 64 * ((op.P + 1) / 2)
+                        : multi          ? (op.MM == 16 && op.P <= 64 ? 1024 : 2048)
+                                         : lines * op.P;
     // special rows: H at each end of a Dirichlet line (every scheme has that
     // many); on periodic lines only where the kernels compile them in (CD2)
     const bool ends = op.periodic ? (op.nbl + op.nbr == 0 || (w == 0 && h == 1 && op.nbl == h && op.nbr == h))

@@ -474,6 +474,102 @@ __global__ __launch_bounds__(1024) void lineop_strided_multi(OpDev op, LaunchArg
     }
 }
 
+// Evenly chunked periodic y- and z-lines of 17 to 32 chunks (576 or 1024
+// points, say): every chunk reads chunk 0's factor rows and the rotation of
+// its interface rows, so a wave may hold two chunks, one per half-wave:
+// lanes 0-31 chunk ty, lanes 32-63 chunk ty + P2 (P2 = P/2 rounded up), each
+// over 32 consecutive lines (256-byte runs).  The window's row offset within
+// the chunk stays a scalar register; the chunk's start moves into the lane's
+// offset.  Not for CD2 (its wrap rows are per chunk) or ragged lines (their
+// tables are per chunk).
+template <int W, int H, int MM>
+__global__ __launch_bounds__(1024) void lineop_strided_pair(OpDev op, LaunchArgs a) {
+    extern __shared__ double red[];  // [32 lines][NI + 1]
+    const int l = threadIdx.x, ll = l & 31, half = l >> 5;
+    const int P = op.P, N = op.N, NI = op.NI;
+    const int P2 = (P + 1) / 2;
+    const int c = (int)threadIdx.y + half * P2;  // this lane's chunk
+    const bool has = c < P;                      // (the last wave's upper half when P is odd)
+    const int cc = has ? c : 0;
+    const int q = blockIdx.x * 32 + ll;
+    const bool live = q < a.nlines;
+    const int qq = live ? q : 0;
+    const double *__restrict__ in = a.in[blockIdx.y];
+    double *__restrict__ out = a.out[blockIdx.y];
+    long base, ps;
+    unsigned fbytes = (unsigned)a.Nx * (unsigned)a.Ny * (unsigned)a.Nz * 8u;  // one member
+    if (op.dir == 1) {
+        base = (long)(qq / a.Nx) * a.Nx * a.Ny + qq % a.Nx;
+        ps = a.Nx;
+        fbytes *= (unsigned)a.nbatch;
+    } else {
+        const long member = (long)blockIdx.z * a.Nx * a.Ny * a.Nz;
+        in += member;
+        out += member;
+        base = qq;
+        ps = (long)a.Nx * a.Ny;
+    }
+    const int cs = cc * MM;
+    const __amdgpu_buffer_rsrc_t rin =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(in), (short)0, (int)fbytes, kRawBufferWord3);
+    const unsigned rowb = (unsigned)ps * 8u;
+    // The lane's offset is the first row of each of the window's three parts
+    // (the H rows below the chunk, wrapped to N - H for chunk 0; the chunk's
+    // rows; the H rows above it, wrapped to 0 for chunk P - 1), the row within
+    // the part the scalar offset; every lane offset is a row of the field.
+    const unsigned b8 = (unsigned)base * 8u;
+    const unsigned vlo = b8 + (unsigned)(cc == 0 ? N - H : cs - H) * rowb;
+    const unsigned vmid = b8 + (unsigned)cs * rowb;
+    const unsigned vhi = b8 + (unsigned)(cc == P - 1 ? 0 : cs + MM) * rowb;
+    double w[MM + 2 * H];
+#pragma unroll
+    for (int j = 0; j < MM + 2 * H; ++j) {
+        if (j < H) {
+            w[j] = bload(rin, vlo, (unsigned)j * rowb);
+        } else if (j >= MM + H) {
+            w[j] = bload(rin, vhi, (unsigned)(j - MM - H) * rowb);
+        } else if (j >= 2 * H && j < MM) {
+            w[j] = bload_nt(rin, vmid, (unsigned)(j - H) * rowb);
+        } else {
+            w[j] = bload(rin, vmid, (unsigned)(j - H) * rowb);
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    double y[MM];
+    chunk_rhs<H, MM, true, false>(op, w, y, false, false);
+    double *myred = red + ll * (NI + 1);
+    if (W > 0) {
+        chunk_solve<W, MM>(op.lu, y);
+        if (has) publish<W, MM, false>(myred, c, false, MM, y);
+    }
+    __syncthreads();
+    if (W > 0) {
+        double s[2 * W > 0 ? 2 * W : 1];
+        interface_sums<W, true, false>(op, cc, myred, s);
+#pragma unroll
+        for (int i = 0; i < MM; ++i) {
+            if (i % 8 == 0) __builtin_amdgcn_sched_barrier(0);
+            double x = y[i];
+#pragma unroll
+            for (int j = 0; j < 2 * W; ++j) x -= op.sp[i * 2 * W + j] * s[j];
+            y[i] = x;
+        }
+    }
+    if (!live || !has) return;
+    const __amdgpu_buffer_rsrc_t rout =
+        __builtin_amdgcn_make_buffer_rsrc(out, (short)0, (int)fbytes, kRawBufferWord3);
+    if (pass_line(op, op.dir, q, a.Nx, a.Ny)) {
+#pragma unroll
+        for (int i = 0; i < MM; ++i) y[i] = bload(rin, vmid, (unsigned)i * rowb);
+    }
+    if (a.accumulate) {
+#pragma unroll
+        for (int i = 0; i < MM; ++i) y[i] = bload_nt(rout, vmid, (unsigned)i * rowb) + y[i];
+    }
+#pragma unroll
+    for (int i = 0; i < MM; ++i) bstore(y[i], rout, vmid, (unsigned)i * rowb);
+}
+
 // lines per workgroup of the contiguous kernel: 16 for periodic lines (every
 // table read is chunk independent, so a wave may span chunks; the smaller tile
 // lets four workgroups of 256-point lines share a CU, which overlaps one
@@ -806,6 +902,14 @@ hipError_t launch_fast(const OpDev &op, const LaunchArgs &a, int nfields, hipStr
         }
         hipLaunchKernelGGL((lineop_contig<W, H, MM, PER, RG>), grid, block, lineop_fast_lds_bytes(op), st, op, a);
     } else {
+        if constexpr (PER && !RG && !has_rows<W, H, PER>()) {
+            if (op.P > 16 && op.P <= 32) {
+                dim3 grid((a.nlines + 31) / 32, nfields, op.dir == 2 ? a.nbatch : 1), block(64, (op.P + 1) / 2);
+                hipLaunchKernelGGL((lineop_strided_pair<W, H, MM>), grid, block, lineop_fast_lds_bytes(op), st, op,
+                                   a);
+                return hipGetLastError();
+            }
+        }
         if constexpr (MM == 16) {
             if (op.P > 16) {
                 dim3 grid((a.nlines + 63) / 64, nfields, op.dir == 2 ? a.nbatch : 1), block(64, 16);

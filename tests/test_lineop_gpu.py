@@ -191,12 +191,13 @@ def test_long_x_lines_in_phases(torch_cuda, nx, bc):
         assert relerr(fl.cpu().numpy(), ref) <= TOL, (kind, nx, bc, "in place")
 
 
-@pytest.mark.parametrize("shape", [(576, 40, 12), (650, 12, 16)])
+@pytest.mark.parametrize("shape", [(576, 40, 12), (650, 12, 16), (1024, 12, 16)])
 def test_long_yz_lines_in_rounds(torch_cuda, shape):
-    """y- and z-lines of more than sixteen 16-row chunks (576 = 36 x 16,
-    650 = 16 + 10 + 39 x 16): 16 chunk waves take the chunks in rounds
-    (lineop_strided_multi); every operator against the oracle, periodic and
-    Dirichlet, in place too."""
+    """z-lines of more than 16 chunks: evenly chunked periodic lines (576 =
+    18 x 32, 1024 = 32 x 32) two chunks to a wave (lineop_strided_pair); the
+    rest (Dirichlet, CD2, ragged 650 = 16 + 10 + 39 x 16) in 16-row chunks,
+    16 chunk waves taking them in rounds (lineop_strided_multi); every
+    operator against the oracle, in place too."""
     from mpi_3dcompact_amd.ops import LineOp
     torch = torch_cuda
     Nz, Ny, Nx = shape
@@ -210,7 +211,8 @@ def test_long_yz_lines_in_rounds(torch_cuda, shape):
             op = LineOp.deriv(scheme, 2, bc, dims, dd)
             kname = op.kernel()
             _say(f"long z deriv shape={shape} bc={bc} scheme={scheme} kernel={kname}")
-            assert kname.startswith("lineop_strided_multi<2" if scheme == P.PENTA10 else "lineop_strided_multi<"), kname
+            pair = bc == P.PERIODIC and scheme != P.CD2 and Nz % 32 == 0
+            assert kname.startswith("lineop_strided_pair<" if pair else "lineop_strided_multi<"), kname
             op.apply([fin], [out])
             torch.cuda.synchronize()
             assert relerr(out.cpu().numpy(), P.deriv_field(scheme, 2, bc, f, dd)) <= TOL, (scheme, bc)
@@ -219,7 +221,8 @@ def test_long_yz_lines_in_rounds(torch_cuda, shape):
             op = LineOp.filter(kind, af, 2, bc3, dims)
             kname = op.kernel()
             _say(f"long z filter shape={shape} bc3={bc3} kind={kind} kernel={kname}")
-            assert kname.startswith("lineop_strided_multi<"), kname
+            pair = bc == P.PERIODIC and Nz % 32 == 0
+            assert kname.startswith("lineop_strided_pair<" if pair else "lineop_strided_multi<"), kname
             ref = P.filter_field(kind, af, 2, list(bc3), f)
             op.apply([fin], [out])
             torch.cuda.synchronize()

@@ -105,9 +105,9 @@ def test_ragged_box_36(gpu, case):
 
 
 def test_long_z_box(gpu):
-    """A 12 x 12 x 576 box: z-lines of 36 16-row chunks take the multi-round
-    strided kernel, the zeta divergence's accumulating launches included;
-    against the oracle after four steps."""
+    """A 12 x 12 x 576 box: z-lines of 18 32-row chunks, two to a wave
+    (lineop_strided_pair), the zeta divergence's accumulating launches
+    included; against the oracle after four steps."""
     from mpi_3dcompact_amd.solver import Solver
     Nx, Ny, Nz = 12, 12, 576
     z, y, x = np.meshgrid(np.arange(Nz) / Nz, np.arange(Ny) / Ny, np.arange(Nx) / Nx, indexing="ij")
