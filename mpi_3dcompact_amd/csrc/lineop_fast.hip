@@ -15,7 +15,7 @@ bool strided_pair(const OpDev &op) {
 
 size_t lineop_fast_lds_bytes(const OpDev &op) {
     if (op.dir == 0) {  // the tile, with the interface values inside it (W = 2) or after it
-        const size_t L = contig_lines(op.periodic, op.W, op.cshort >= 0);
+        const size_t L = contig_lines(op.periodic, op.W, op.cshort >= 0, op.N);
         const int K = L == 64 && op.phases > 1 ? op.phases : 1;
         const size_t rows = K > 1 ? (size_t)((op.P + K - 1) / K) * op.MM + 2 * op.H : (size_t)op.N + 2 * op.H;
         const size_t tile = rows * (L + 1), red = L * (op.NI + 1);
@@ -43,7 +43,7 @@ bool lineop_fast_supported(const OpDev &op, size_t lds_limit) {
     const bool pair = (w == 0 && (h == 1 || h == 3)) || (w == 1 && (h == 2 || h == 4 || h == 5)) ||
                       (w == 2 && (h == 3 || h == 5));
     // workgroup size as launched: (lines per workgroup) x P
-    const int lines = op.dir != 0 ? 64 : contig_lines(op.periodic, op.W, op.cshort >= 0);
+    const int lines = op.dir != 0 ? 64 : contig_lines(op.periodic, op.W, op.cshort >= 0, op.N);
     // y- and z-lines of more than 16 chunks: evenly chunked periodic ones up to
     // 32 chunks two to a wave; else 16-row chunks, 16 chunk waves in rounds (up
     // to 64 chunks; the 32-row chunk state spills in that kernel)

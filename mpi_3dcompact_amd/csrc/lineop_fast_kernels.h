@@ -580,8 +580,11 @@ __global__ __launch_bounds__(1024) void lineop_strided_pair(OpDev op, LaunchArgs
 // scalar loads and the end-row branches uniform; with 16 lines the per-lane
 // table loads made Penta10 x twice slower).  Banded operators on ragged
 // lines (RG) read chunk-own tables, periodic ones included: 64 lines.
-__host__ __device__ constexpr int contig_lines(bool per, int w, bool rg) {
-    return w == 0 || (per && !rg) ? 16 : 64;
+// Banded periodic lines longer than 256 points: 8 lines (the 16-line tile of
+// 576-point lines, 79 KB, let two workgroups share a CU; 8 lines measured
+// 13-49% faster there, and mixed, -5% to +8%, at 256 points).
+__host__ __device__ constexpr int contig_lines(bool per, int w, bool rg, int n = 0) {
+    return w == 0 ? 16 : (per && !rg) ? (n > 256 ? 8 : 16) : 64;
 }
 
 // loads in flight per thread while staging.  The tile caps a CU at one to four
@@ -593,10 +596,9 @@ constexpr int contig_batch() {
     return MM % 16 ? (MM <= 12 ? MM : MM / 2) : 16;
 }
 
-template <int W, int H, int MM, bool PER, bool RG>
+template <int W, int H, int MM, bool PER, bool RG, int L>
 __global__ __launch_bounds__(1024) void lineop_contig(OpDev op, LaunchArgs a) {
     extern __shared__ double smem[];
-    constexpr int L = contig_lines(PER, W, RG);
     constexpr int LS = L + 1;  // row stride of the transposed tile
     const int N = op.N, P = op.P, NI = op.NI;
     double *tile = smem;  // [N + 2H][LS], row e holds phi[e - H]
@@ -892,6 +894,14 @@ template <int W, int H, int MM, bool PER, bool RG>
 hipError_t launch_fast(const OpDev &op, const LaunchArgs &a, int nfields, hipStream_t st) {
     if (op.dir == 0) {
         constexpr int L = contig_lines(PER, W, RG);
+        if constexpr (PER && !RG && W > 0) {
+            if (contig_lines(PER, W, RG, op.N) == 8) {
+                dim3 grid((a.nlines + 7) / 8, nfields), block(8, op.P);
+                hipLaunchKernelGGL((lineop_contig<W, H, MM, PER, RG, 8>), grid, block, lineop_fast_lds_bytes(op), st,
+                                   op, a);
+                return hipGetLastError();
+            }
+        }
         dim3 grid((a.nlines + L - 1) / L, nfields), block(L, op.P);
         if constexpr (L == 64) {
             if (op.phases > 1) {
@@ -900,7 +910,7 @@ hipError_t launch_fast(const OpDev &op, const LaunchArgs &a, int nfields, hipStr
                 return hipGetLastError();
             }
         }
-        hipLaunchKernelGGL((lineop_contig<W, H, MM, PER, RG>), grid, block, lineop_fast_lds_bytes(op), st, op, a);
+        hipLaunchKernelGGL((lineop_contig<W, H, MM, PER, RG, L>), grid, block, lineop_fast_lds_bytes(op), st, op, a);
     } else {
         if constexpr (PER && !RG && !has_rows<W, H, PER>()) {
             if (op.P > 16 && op.P <= 32) {
