@@ -11,13 +11,17 @@
 // registers; LDS is only used for the interface exchange.  The window loads
 // and the stores go through a buffer descriptor (one 32-bit lane offset, the
 // row offset in a scalar register), and all window loads are issued before
-// any other work of the lane.
+// any other work of the lane.  Lines of more than 16 chunks: periodic ones
+// two chunks to a wave (lineop_strided_pair), the others in rounds of 16
+// chunk waves with a second, recomputing pass (lineop_strided_multi).
 //
-// Contiguous direction (x): the workgroup stages 16 (periodic) or 64 lines
-// into LDS in a transposed [point][line] layout (row stride lines + 1) with
-// halo rows, so the coalesced fill and the per-lane chunk reads are both free
-// of bank conflicts; each thread issues batches of loads before its LDS
-// writes.
+// Contiguous direction (x): the workgroup stages 8, 16 (periodic, explicit)
+// or 64 lines (banded Dirichlet or ragged: a wave per chunk) into LDS in a
+// transposed [point][line] layout (row stride lines + 1) with halo rows, so
+// the coalesced fill and the per-lane chunk reads are both free of bank
+// conflicts; each thread issues batches of loads before its LDS writes.
+// 64-line tiles too large for LDS stage their chunks in groups
+// (lineop_contig_ph).
 //
 // Per chunk: explicit stencil (taps in the reference's summation order, no
 // FMA contraction, so the explicit stage is bitwise that of the reference),
