@@ -191,10 +191,11 @@ def test_long_x_lines_in_phases(torch_cuda, nx, bc):
         assert relerr(fl.cpu().numpy(), ref) <= TOL, (kind, nx, bc, "in place")
 
 
-@pytest.mark.parametrize("shape", [(576, 40, 12), (650, 12, 16), (1024, 12, 16)])
+@pytest.mark.parametrize("shape", [(576, 40, 12), (544, 13, 11), (650, 12, 16), (1024, 12, 16)])
 def test_long_yz_lines_in_rounds(torch_cuda, shape):
     """z-lines of more than 16 chunks: evenly chunked periodic lines (576 =
-    18 x 32, 1024 = 32 x 32) two chunks to a wave (lineop_strided_pair); the
+    18 x 32, 544 = 17 x 32 with the last wave's upper half idle, 1024 =
+    32 x 32) two chunks to a wave (lineop_strided_pair); the
     rest (Dirichlet, CD2, ragged 650 = 16 + 10 + 39 x 16) in 16-row chunks,
     16 chunk waves taking them in rounds (lineop_strided_multi); every
     operator against the oracle, in place too."""
@@ -459,15 +460,16 @@ def test_eight_row_chunks(torch_cuda):
             assert relerr(out.cpu().numpy(), P.filter_field(kind, af, d, [0, 0, 0], f)) <= TOL, (kind, d)
 
 
-@pytest.mark.parametrize("nx", [384, 512])
-def test_long_periodic_x_lines(torch_cuda, nx):
+@pytest.mark.parametrize("nx,ny,nz", [(384, 20, 4), (512, 20, 4), (576, 13, 11)])
+def test_long_periodic_x_lines(torch_cuda, nx, ny, nz):
     """x-lines longer than sixteen 16-row chunks: the contiguous kernel's
-    workgroup is 16 lines x P chunks (384 or 512 lanes), so it still takes
-    them with 16-row chunks."""
+    workgroup is 8 lines x P chunks above 256 points (192 to 288 lanes), so
+    it still takes them with 16-row chunks; 143 lines leave the last
+    workgroup of 8 partly empty."""
     from mpi_3dcompact_amd.ops import LineOp
     torch = torch_cuda
-    shape = (4, 20, nx)
-    dims = (nx, 20, 4)
+    shape = (nz, ny, nx)
+    dims = (nx, ny, nz)
     f = field(shape, 12)
     fin = torch.from_numpy(f).cuda()
     out = torch.empty_like(fin)
