@@ -198,6 +198,33 @@ __device__ __forceinline__ bool pass_line(const OpDev &op, int dir, int q, int N
     return (op.passA && (ia == 0 || ia == op.NA - 1)) || (op.passB && (ib == 0 || ib == op.NB - 1));
 }
 
+// A y- or z-line qq of this launch row (field blockIdx.y): y-lines run x
+// fastest within a z-plane over every stacked member, z-lines belong to member
+// blockIdx.z.  The field pointers (member included), the line's first point,
+// the point stride, and the bytes a buffer descriptor over the field covers.
+struct StridedLine {
+    const double *in;
+    double *out;
+    long base, ps;
+    unsigned fbytes;
+};
+__device__ __forceinline__ StridedLine strided_line(const OpDev &op, const LaunchArgs &a, int qq) {
+    StridedLine r{a.in[blockIdx.y], a.out[blockIdx.y], 0, 0,
+                  (unsigned)a.Nx * (unsigned)a.Ny * (unsigned)a.Nz * 8u};  // one member
+    if (op.dir == 1) {
+        r.base = (long)(qq / a.Nx) * a.Nx * a.Ny + qq % a.Nx;
+        r.ps = a.Nx;
+        r.fbytes *= (unsigned)a.nbatch;
+    } else {
+        const long member = (long)blockIdx.z * a.Nx * a.Ny * a.Nz;
+        r.in += member;
+        r.out += member;
+        r.base = qq;
+        r.ps = (long)a.Nx * a.Ny;
+    }
+    return r;
+}
+
 // Ragged lines (RG): chunk cshort (1) has lshort < MM rows, so later chunks
 // start MM - lshort rows earlier.  The chunk's rows beyond its length are
 // computed and never stored; its factor rows there are identity rows.
@@ -241,21 +268,11 @@ __global__ __launch_bounds__(1024) void lineop_strided(OpDev op, LaunchArgs a) {
     const int q = blockIdx.x * 64 + l;
     const bool live = q < a.nlines;
     const int qq = live ? q : 0;
-    const double *__restrict__ in = a.in[blockIdx.y];
-    double *__restrict__ out = a.out[blockIdx.y];
-    long base, ps;
-    unsigned fbytes = (unsigned)a.Nx * (unsigned)a.Ny * (unsigned)a.Nz * 8u;  // one member
-    if (op.dir == 1) {
-        base = (long)(qq / a.Nx) * a.Nx * a.Ny + qq % a.Nx;
-        ps = a.Nx;
-        fbytes *= (unsigned)a.nbatch;
-    } else {
-        const long member = (long)blockIdx.z * a.Nx * a.Ny * a.Nz;
-        in += member;
-        out += member;
-        base = qq;
-        ps = (long)a.Nx * a.Ny;
-    }
+    const StridedLine ln = strided_line(op, a, qq);
+    const double *__restrict__ in = ln.in;
+    double *__restrict__ out = ln.out;
+    const long base = ln.base, ps = ln.ps;
+    const unsigned fbytes = ln.fbytes;
     const int cs = chunk_start<MM, RG>(op, c);
     const bool short_chunk = RG && c == op.cshort;
     const int len = short_chunk ? op.lshort : MM;
@@ -352,21 +369,11 @@ __global__ __launch_bounds__(1024) void lineop_strided_multi(OpDev op, LaunchArg
     const int q = blockIdx.x * 64 + l;
     const bool live = q < a.nlines;
     const int qq = live ? q : 0;
-    const double *__restrict__ in = a.in[blockIdx.y];
-    double *__restrict__ out = a.out[blockIdx.y];
-    long base, ps;
-    unsigned fbytes = (unsigned)a.Nx * (unsigned)a.Ny * (unsigned)a.Nz * 8u;  // one member
-    if (op.dir == 1) {
-        base = (long)(qq / a.Nx) * a.Nx * a.Ny + qq % a.Nx;
-        ps = a.Nx;
-        fbytes *= (unsigned)a.nbatch;
-    } else {
-        const long member = (long)blockIdx.z * a.Nx * a.Ny * a.Nz;
-        in += member;
-        out += member;
-        base = qq;
-        ps = (long)a.Nx * a.Ny;
-    }
+    const StridedLine ln = strided_line(op, a, qq);
+    const double *__restrict__ in = ln.in;
+    double *__restrict__ out = ln.out;
+    const long base = ln.base, ps = ln.ps;
+    const unsigned fbytes = ln.fbytes;
     const __amdgpu_buffer_rsrc_t rin =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(in), (short)0, (int)fbytes, kRawBufferWord3);
     const __amdgpu_buffer_rsrc_t rout =
@@ -498,6 +505,8 @@ __global__ __launch_bounds__(1024) void lineop_strided_pair(OpDev op, LaunchArgs
     const int q = blockIdx.x * 32 + ll;
     const bool live = q < a.nlines;
     const int qq = live ? q : 0;
+    // (the addressing of strided_line, written out: through the helper this
+    // kernel took two more registers and a wave fewer per SIMD)
     const double *__restrict__ in = a.in[blockIdx.y];
     double *__restrict__ out = a.out[blockIdx.y];
     long base, ps;
