@@ -120,6 +120,7 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
         d.ext = p.ext; d.MM = p.MM; d.periodic = p.periodic; d.nbl = p.nbl; d.nbr = p.nbr;
         d.int_off = p.int_off; d.int_w = (int)p.int_c.size(); d.gb = p.gb;
         d.cshort = p.cshort; d.lshort = p.lshort;
+        d.same_interior = p.same_interior ? 1 : 0;
         for (int k = 0; k < d.int_w; ++k) d.int_c[k] = p.int_c[k];
         d.dd = p.dd; d.divide = p.divide ? 1 : 0;
         // 1/dd is exact when dd is a power of two; then r * (1/dd) == r / dd bitwise
@@ -188,7 +189,7 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
     op->nbatch = nbatch;
     std::memcpy(op->dims, dims, sizeof(op->dims));
     const std::string per = p.periodic ? "true" : "false";
-    const bool pair = p.periodic && p.cshort < 0 && p.nbl + p.nbr == 0 && p.P > 16 && p.P <= 32;
+    const bool pair = mc3d::lineop_fast_pair(op->dev);
     if (op->fast)
         op->kname = std::string(dir != 0 ? (pair ? "lineop_strided_pair<" : p.P > 16 ? "lineop_strided_multi<" : "lineop_strided<")
                                          : op->dev.phases > 1 ? "lineop_contig_ph<" : "lineop_contig<") +

@@ -5,13 +5,17 @@
 
 namespace mc3d {
 
-namespace {
-// y- and z-lines that lineop_strided_pair takes: evenly chunked periodic lines
-// of 17 to 32 chunks with no special rows
-bool strided_pair(const OpDev &op) {
-    return op.dir != 0 && op.periodic && op.cshort < 0 && op.nbl + op.nbr == 0 && op.P > 16 && op.P <= 32;
+bool lineop_fast_pair(const OpDev &op) {
+    if (op.dir == 0 || op.cshort >= 0 || op.N != op.P * op.MM || op.P <= 16 || op.P > 32) return false;
+    if (op.periodic && op.nbl + op.nbr == 0) return true;  // chunk 0's tables serve every chunk
+    // special rows: 16- or 32-row chunks, a wave for each end chunk (30 chunks
+    // in 16 waves), with no tables or interior ones shared
+    return (op.MM == 16 || op.MM == 32) && op.P <= 30 && (op.W == 0 || (!op.periodic && op.same_interior));
 }
-}  // namespace
+
+int lineop_pair_waves(const OpDev &op) {
+    return op.periodic && op.nbl + op.nbr == 0 ? (op.P + 1) / 2 : 2 + (op.P - 1) / 2;
+}
 
 size_t lineop_fast_lds_bytes(const OpDev &op) {
     if (op.dir == 0) {  // the tile, with the interface values inside it (W = 2) or after it
@@ -23,7 +27,7 @@ size_t lineop_fast_lds_bytes(const OpDev &op) {
     }
     // the interface values (32 lines when two chunks share a wave), and beyond
     // 16 chunks in rounds the stash of rows between rounds
-    if (strided_pair(op)) return sizeof(double) * (size_t)32 * (op.NI + 1);
+    if (lineop_fast_pair(op)) return sizeof(double) * (size_t)32 * (op.NI + 1);
     const size_t stash = op.P > 16 ? (size_t)64 * ((op.P + 15) / 16) * op.H : 0;
     return sizeof(double) * ((size_t)64 * (op.NI + 1) + stash);
 }
@@ -44,11 +48,11 @@ bool lineop_fast_supported(const OpDev &op, size_t lds_limit) {
                       (w == 2 && (h == 3 || h == 5));
     // workgroup size as launched: (lines per workgroup) x P
     const int lines = op.dir != 0 ? 64 : contig_lines(op.periodic, op.W, op.cshort >= 0, op.N);
-    // y- and z-lines of more than 16 chunks: evenly chunked periodic ones up to
-    // 32 chunks two to a wave; else 16-row chunks, 16 chunk waves in rounds (up
-    // to 64 chunks; the 32-row chunk state spills in that kernel)
+    // y- and z-lines of more than 16 chunks: evenly chunked ones up to 32
+    // chunks two to a wave; else 16-row chunks, 16 chunk waves in rounds (up to
+    // 64 chunks; the 32-row chunk state spills in that kernel)
     const bool multi = op.dir != 0 && op.P > 16;
-    const int threads = strided_pair(op) ? 64 * ((op.P + 1) / 2)
+    const int threads = lineop_fast_pair(op) ? 64 * lineop_pair_waves(op)
                         : multi          ? (op.MM == 16 && op.P <= 64 ? 1024 : 2048)
                                          : lines * op.P;
     // special rows: H at each end of a Dirichlet line (every scheme has that

@@ -11,9 +11,10 @@
 // registers; LDS is only used for the interface exchange.  The window loads
 // and the stores go through a buffer descriptor (one 32-bit lane offset, the
 // row offset in a scalar register), and all window loads are issued before
-// any other work of the lane.  Lines of more than 16 chunks: periodic ones
-// two chunks to a wave (lineop_strided_pair), the others in rounds of 16
-// chunk waves with a second, recomputing pass (lineop_strided_multi).
+// any other work of the lane.  Lines of more than 16 chunks: evenly chunked
+// ones up to 32 chunks two chunks to a wave (lineop_strided_pair), the others
+// in rounds of 16 chunk waves with a second, recomputing pass
+// (lineop_strided_multi).
 //
 // Contiguous direction (x): the workgroup stages 8, 16 (periodic, explicit)
 // or 64 lines (banded Dirichlet or ragged: a wave per chunk) into LDS in a
@@ -104,26 +105,45 @@ __device__ __forceinline__ void chunk_rhs(const OpDev &op, const double *w, doub
 }
 
 // banded LU solve of one chunk block with the factor rows lu[MM][kLuRec]
-// The factor rows are read with scalar loads; a scheduling fence every eight
-// rows keeps the scheduler from hoisting every row's loads to the top, where
-// they outgrow the scalar registers and are spilled to vector-register lanes
-// (one v_writelane per dword, one v_readlane per dword at each use).
-template <int W, int MM>
-__device__ __forceinline__ void chunk_solve(const double *lu, double *y) {
+// (lu(e): entry e of the rows).  The factor rows are read with scalar loads; a
+// scheduling fence every eight rows keeps the scheduler from hoisting every
+// row's loads to the top, where they outgrow the scalar registers and are
+// spilled to vector-register lanes (one v_writelane per dword, one v_readlane
+// per dword at each use).
+template <int W, int MM, class LU>
+__device__ __forceinline__ void chunk_solve_by(LU lu, double *y) {
 #pragma unroll
     for (int i = 1; i < MM; ++i) {
         if (i % 8 == 0) __builtin_amdgcn_sched_barrier(0);
-        double z = y[i] - lu[i * kLuRec + 0] * y[i - 1];
-        if (W == 2 && i >= 2) z -= lu[i * kLuRec + 1] * y[i - 2];
+        double z = y[i] - lu(i * kLuRec + 0) * y[i - 1];
+        if (W == 2 && i >= 2) z -= lu(i * kLuRec + 1) * y[i - 2];
         y[i] = z;
     }
 #pragma unroll
     for (int i = MM - 1; i >= 0; --i) {
         if (i % 8 == 7) __builtin_amdgcn_sched_barrier(0);
         double x = y[i];
-        if (i + 1 < MM) x -= lu[i * kLuRec + 2] * y[i + 1];
-        if (W == 2 && i + 2 < MM) x -= lu[i * kLuRec + 3] * y[i + 2];
-        y[i] = x * lu[i * kLuRec + 4];
+        if (i + 1 < MM) x -= lu(i * kLuRec + 2) * y[i + 1];
+        if (W == 2 && i + 2 < MM) x -= lu(i * kLuRec + 3) * y[i + 2];
+        y[i] = x * lu(i * kLuRec + 4);
+    }
+}
+template <int W, int MM>
+__device__ __forceinline__ void chunk_solve(const double *lu, double *y) {
+    chunk_solve_by<W, MM>([lu](int e) { return lu[e]; }, y);
+}
+
+// interface correction y = y - S s of one chunk with the spike rows
+// sp[MM][2W] (sp(e): entry e)
+template <int W, int MM, class SP>
+__device__ __forceinline__ void chunk_correct_by(SP sp, const double *s, double *y) {
+#pragma unroll
+    for (int i = 0; i < MM; ++i) {
+        if (i % 8 == 0) __builtin_amdgcn_sched_barrier(0);
+        double x = y[i];
+#pragma unroll
+        for (int j = 0; j < 2 * W; ++j) x -= sp(i * 2 * W + j) * s[j];
+        y[i] = x;
     }
 }
 
@@ -170,6 +190,49 @@ __device__ __forceinline__ void interface_sums(const OpDev &op, int c, const dou
             }
         }
     }
+}
+
+// interface_sums of Dirichlet lines for a wave of two chunks, cl in the lower
+// half-wave and ch in the upper (hi): every lane forms both chunks' sums with
+// the rows G_cl and G_ch as scalar operands, over its own line's interface
+// values, and keeps its half's (rows selected per lane instead took 28 more
+// registers).  The terms come in the order of interface_sums (increasing
+// chunk); a chunk offset d = -gb .. gb that falls off the line for one chunk
+// adds zero terms there.
+template <int W>
+__device__ __forceinline__ void interface_sums_pair(const OpDev &op, int cl, int ch, bool hi, const double *yhat,
+                                                    double *s) {
+    constexpr int W2 = 2 * W;
+    const int P = op.P, NI = op.NI, gb = op.gb;
+    const double *Gl = op.G + (size_t)cl * W2 * NI, *Gh = op.G + (size_t)ch * W2 * NI;
+    double sl[W2], sh[W2];
+#pragma unroll
+    for (int j = 0; j < W2; ++j) sl[j] = sh[j] = 0.0;
+    for (int d = -gb; d <= gb; ++d) {
+        const int ul = cl + d, uh = ch + d;
+        const bool vl = ul >= 0 && ul < P, vh = uh >= 0 && uh < P;
+        if (vl) {
+            const double *g = Gl + ul * W2, *yv = yhat + ul * W2;
+#pragma unroll
+            for (int k = 0; k < W2; ++k) {
+                const double v = yv[k];
+#pragma unroll
+                for (int j = 0; j < W2; ++j) sl[j] += g[j * NI + k] * v;
+            }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (vh) {
+            const double *g = Gh + uh * W2, *yv = yhat + uh * W2;
+#pragma unroll
+            for (int k = 0; k < W2; ++k) {
+                const double v = yv[k];
+#pragma unroll
+                for (int j = 0; j < W2; ++j) sh[j] += g[j * NI + k] * v;
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < W2; ++j) s[j] = hi ? sh[j] : sl[j];
 }
 
 // raw buffer access: descriptor in SGPRs, one 32-bit lane offset, row offset in an SGPR
@@ -317,14 +380,7 @@ __global__ __launch_bounds__(1024) void lineop_strided(OpDev op, LaunchArgs a) {
         double s[2 * W > 0 ? 2 * W : 1];
         interface_sums<W, PER, RG>(op, c, myred, s);
         const double *sp = op.sp + (PER && !RG ? 0 : (size_t)c * MM * 2 * W);
-#pragma unroll
-        for (int i = 0; i < MM; ++i) {
-            if (i % 8 == 0) __builtin_amdgcn_sched_barrier(0);
-            double x = y[i];
-#pragma unroll
-            for (int j = 0; j < 2 * W; ++j) x -= sp[i * 2 * W + j] * s[j];
-            y[i] = x;
-        }
+        chunk_correct_by<W, MM>([sp](int e) { return sp[e]; }, s, y);
     }
     if (!live) return;
     const __amdgpu_buffer_rsrc_t rout =
@@ -456,14 +512,7 @@ __global__ __launch_bounds__(1024) void lineop_strided_multi(OpDev op, LaunchArg
             if (W > 0) {
                 chunk_solve<W, MM>(op.lu + (PER && !RG ? 0 : (size_t)c * MM * kLuRec), y);
                 const double *sp = op.sp + (PER && !RG ? 0 : (size_t)c * MM * 2 * W);
-#pragma unroll
-                for (int i = 0; i < MM; ++i) {
-                    if (i % 8 == 0) __builtin_amdgcn_sched_barrier(0);
-                    double x = y[i];
-#pragma unroll
-                    for (int j = 0; j < 2 * W; ++j) x -= sp[i * 2 * W + j] * s[j];
-                    y[i] = x;
-                }
+                chunk_correct_by<W, MM>([sp](int e) { return sp[e]; }, s, y);
             }
         }
         __syncthreads();  // the round's windows are read (in may equal out)
@@ -485,23 +534,47 @@ __global__ __launch_bounds__(1024) void lineop_strided_multi(OpDev op, LaunchArg
     }
 }
 
-// Evenly chunked periodic y- and z-lines of 17 to 32 chunks (576 or 1024
-// points, say): every chunk reads chunk 0's factor rows and the rotation of
-// its interface rows, so a wave may hold two chunks, one per half-wave:
-// lanes 0-31 chunk ty, lanes 32-63 chunk ty + P2 (P2 = P/2 rounded up), each
-// over 32 consecutive lines (256-byte runs).  The window's row offset within
-// the chunk stays a scalar register; the chunk's start moves into the lane's
-// offset.  Not for CD2 (its wrap rows are per chunk) or ragged lines (their
-// tables are per chunk).
-template <int W, int H, int MM>
+// Evenly chunked y- and z-lines of 17 to 32 chunks (576 or 1024 points, say),
+// two chunks to a wave, one per half-wave, each over 32 consecutive lines
+// (256-byte runs).  The window's row offset within the chunk stays a scalar
+// register; the chunk's start moves into the lane's offset.
+//  - periodic lines without special rows: lanes 0-31 chunk ty, lanes 32-63
+//    chunk ty + P2 (P2 = P/2 rounded up; when P is odd the last wave's upper
+//    half is idle); every chunk reads chunk 0's factor rows and the rotation
+//    of its interface rows;
+//  - lines with special rows (Dirichlet, CD2's periodic wrap rows), up to 30
+//    chunks: chunk 0 and chunk P - 1 take a wave each (the upper half redoes
+//    the lower half's chunk and stores nothing), the interior chunks 1 .. P - 2
+//    two to a wave (1 + k and 1 + k + Q, Q = (P - 1) / 2).  The end-row flags
+//    and the factor and spike tables are then wave-uniform (the interior
+//    chunks' tables are equal bitwise; the host checks it); the interface
+//    rows G_c, different for every chunk, are selected per lane.  (Tables and
+//    end rows selected per lane in the waves holding an end chunk took the
+//    kernel past 128 registers.)
+template <int W, int H, int MM, bool PER>
 __global__ __launch_bounds__(1024) void lineop_strided_pair(OpDev op, LaunchArgs a) {
+    constexpr bool ROWS = has_rows<W, H, PER>();
+    constexpr bool OWN = !PER && W > 0;  // tables per chunk
     extern __shared__ double red[];  // [32 lines][NI + 1]
     const int l = threadIdx.x, ll = l & 31, half = l >> 5;
     const int P = op.P, N = op.N, NI = op.NI;
     const int P2 = (P + 1) / 2;
-    const int c = (int)threadIdx.y + half * P2;  // this lane's chunk
-    const bool has = c < P;                      // (the last wave's upper half when P is odd)
-    const int cc = has ? c : 0;
+    // the wave's chunks (cl, ch), wave-uniform with special rows
+    const int ty = ROWS ? __builtin_amdgcn_readfirstlane(threadIdx.y) : 0;
+    const int Q = (P - 1) / 2;
+    const int cl = ty == 0 ? 0 : ty == 1 ? P - 1 : ty - 1;
+    const bool hi_has = ty >= 2 && ty - 1 + Q <= P - 2;
+    const int ch = hi_has ? ty - 1 + Q : cl;
+    int c;     // this lane's chunk
+    bool has;  // whether the lane stores it
+    if constexpr (ROWS) {
+        c = half ? ch : cl;
+        has = !half || hi_has;
+    } else {
+        c = (int)threadIdx.y + half * P2;
+        has = c < P;  // (the last wave's upper half when P is odd)
+    }
+    const int cc = has || ROWS ? c : 0;
     const int q = blockIdx.x * 32 + ll;
     const bool live = q < a.nlines;
     const int qq = live ? q : 0;
@@ -530,6 +603,7 @@ __global__ __launch_bounds__(1024) void lineop_strided_pair(OpDev op, LaunchArgs
     // (the H rows below the chunk, wrapped to N - H for chunk 0; the chunk's
     // rows; the H rows above it, wrapped to 0 for chunk P - 1), the row within
     // the part the scalar offset; every lane offset is a row of the field.
+    // (Dirichlet lines read the wrapped rows too; their end rows never use them.)
     const unsigned b8 = (unsigned)base * 8u;
     const unsigned vlo = b8 + (unsigned)(cc == 0 ? N - H : cs - H) * rowb;
     const unsigned vmid = b8 + (unsigned)cs * rowb;
@@ -549,24 +623,20 @@ __global__ __launch_bounds__(1024) void lineop_strided_pair(OpDev op, LaunchArgs
     }
     __builtin_amdgcn_sched_barrier(0);
     double y[MM];
-    chunk_rhs<H, MM, true, false>(op, w, y, false, false);
+    chunk_rhs<H, MM, PER, ROWS>(op, w, y, cl == 0, cl == P - 1);
     double *myred = red + ll * (NI + 1);
+    const double *lu = op.lu + (OWN ? (size_t)cl * MM * kLuRec : 0);
+    const double *sp = op.sp + (OWN ? (size_t)cl * MM * 2 * W : 0);
     if (W > 0) {
-        chunk_solve<W, MM>(op.lu, y);
+        chunk_solve<W, MM>(lu, y);
         if (has) publish<W, MM, false>(myred, c, false, MM, y);
     }
     __syncthreads();
     if (W > 0) {
         double s[2 * W > 0 ? 2 * W : 1];
-        interface_sums<W, true, false>(op, cc, myred, s);
-#pragma unroll
-        for (int i = 0; i < MM; ++i) {
-            if (i % 8 == 0) __builtin_amdgcn_sched_barrier(0);
-            double x = y[i];
-#pragma unroll
-            for (int j = 0; j < 2 * W; ++j) x -= op.sp[i * 2 * W + j] * s[j];
-            y[i] = x;
-        }
+        if constexpr (OWN) interface_sums_pair<W>(op, cl, ch, half, myred, s);
+        else interface_sums<W, true, false>(op, cc, myred, s);
+        chunk_correct_by<W, MM>([=](int e) { return sp[e]; }, s, y);
     }
     if (!live || !has) return;
     const __amdgpu_buffer_rsrc_t rout =
@@ -925,11 +995,11 @@ hipError_t launch_fast(const OpDev &op, const LaunchArgs &a, int nfields, hipStr
         }
         hipLaunchKernelGGL((lineop_contig<W, H, MM, PER, RG, L>), grid, block, lineop_fast_lds_bytes(op), st, op, a);
     } else {
-        if constexpr (PER && !RG && !has_rows<W, H, PER>()) {
-            if (op.P > 16 && op.P <= 32) {
-                dim3 grid((a.nlines + 31) / 32, nfields, op.dir == 2 ? a.nbatch : 1), block(64, (op.P + 1) / 2);
-                hipLaunchKernelGGL((lineop_strided_pair<W, H, MM>), grid, block, lineop_fast_lds_bytes(op), st, op,
-                                   a);
+        if constexpr (!RG && ((PER && !has_rows<W, H, PER>()) || MM == 16 || MM == 32)) {
+            if (lineop_fast_pair(op)) {
+                dim3 grid((a.nlines + 31) / 32, nfields, op.dir == 2 ? a.nbatch : 1), block(64, lineop_pair_waves(op));
+                hipLaunchKernelGGL((lineop_strided_pair<W, H, MM, PER>), grid, block, lineop_fast_lds_bytes(op), st,
+                                   op, a);
                 return hipGetLastError();
             }
         }

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -565,6 +566,13 @@ int build_plan(const PlanRequest &rq, Plan &plan, std::string &err) {
                 plan.sp[((size_t)c * mm + i) * W2 + k] = (double)col[i];
             }
         }
+    }
+    if (plan.uniform && P >= 3) {
+        const size_t nlu = (size_t)mm * kLuRec, nsp = (size_t)mm * W2;
+        plan.same_interior = true;
+        for (int c = 2; c < P - 1 && plan.same_interior; ++c)
+            plan.same_interior = std::memcmp(&plan.lu[c * nlu], &plan.lu[nlu], nlu * sizeof(double)) == 0 &&
+                                 std::memcmp(&plan.sp[c * nsp], &plan.sp[nsp], nsp * sizeof(double)) == 0;
     }
     // interface matrix R t = yhat
     Mat R(NI);

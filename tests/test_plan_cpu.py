@@ -335,6 +335,24 @@ def test_periodic_uniform_chunks_share_factors(kind, cls, af):
         assert np.array_equal(t["G"][c], np.roll(t["G"][0], c * W2, axis=1))
 
 
+@pytest.mark.parametrize("kind,cls,af", [(P.PENTA10, 0, 0.0), (P.PADE6, 0, 0.0), (P.COMPACT8, 1, 0.49),
+                                         (P.COMPACT10, 1, -0.49), (P.COMMN5PADE, 1, 0.0)])
+@pytest.mark.parametrize("n", [544, 576, 960])
+def test_dirichlet_interior_chunks_share_factors(kind, cls, af, n):
+    """Evenly chunked Dirichlet lines of 32-row chunks: chunks 1 .. P - 2 hold
+    bitwise the same LU factors and spikes (the two-chunk y/z kernel gives
+    every interior chunk chunk 1's), the end chunks their own (the matrix's
+    boundary rows), and the interface rows differ from chunk to chunk."""
+    t = tables(cls, kind, P.DIRICHLET, n, 1 / (n - 1), af, 32)
+    assert t["P"] == n // 32
+    for c in range(2, t["P"] - 1):
+        assert np.array_equal(t["lu"][c], t["lu"][1])
+        assert np.array_equal(t["sp"][c], t["sp"][1])
+    for c in (0, t["P"] - 1):
+        assert not np.array_equal(t["lu"][c], t["lu"][1])
+    assert not np.array_equal(t["G"][2], np.roll(t["G"][1], 2 * t["W"], axis=1))
+
+
 def test_plan_rejects_short_lines():
     from mpi_3dcompact_amd import Mc3dError
     with pytest.raises(Mc3dError):

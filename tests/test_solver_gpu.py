@@ -104,10 +104,12 @@ def test_ragged_box_36(gpu, case):
     check_pair(g, o, extra=extra)
 
 
-def test_long_z_box(gpu):
+@pytest.mark.parametrize("faces", [None, {"Z0": "ADIABATIC_WALL", "Z1": "ADIABATIC_WALL"}], ids=["periodic", "z-walls"])
+def test_long_z_box(gpu, faces):
     """A 12 x 12 x 576 box: z-lines of 18 32-row chunks, two to a wave
-    (lineop_strided_pair), the zeta divergence's accumulating launches
-    included; against the oracle after four steps."""
+    (lineop_strided_pair), periodic or between adiabatic walls (Dirichlet
+    z-lines: the end chunks take a wave each), the zeta divergence's
+    accumulating launches included; against the oracle after four steps."""
     from mpi_3dcompact_amd.solver import Solver
     Nx, Ny, Nz = 12, 12, 576
     z, y, x = np.meshgrid(np.arange(Nz) / Nz, np.arange(Ny) / Ny, np.arange(Nx) / Nx, indexing="ij")
@@ -119,11 +121,12 @@ def test_long_z_box(gpu):
     one, zero = np.ones((Nz, Ny, Nx)), np.zeros((Nz, Ny, Nx))
     met = [one / L ** 3, one / L, zero, zero, zero, one / L, zero, zero, zero, one / L]
     case = dict(fd="PENTA10", filt="COMPACT8", rk="RK4", mu_ref=1e-3, dims=(Nx, Ny, Nz))
-    g = Solver(hit.options_text(Nz, **case), (Nz, Ny, Nx))
+    _say(f"case: 12 x 12 x 576, faces {faces}")
+    g = Solver(hit.options_text(Nz, faces=faces, **case), (Nz, Ny, Nx))
     g.set_metrics(met)
     g.set_initial(ic)
     g.step(4)
-    o = P.Solver(hit.oracle_config(Nz, **case), met, ic)
+    o = P.Solver(hit.oracle_config(Nz, faces=faces, **case), met, ic)
     o.steps(4)
     check_pair(g, o)
 
