@@ -192,6 +192,7 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
     const bool pair = mc3d::lineop_fast_pair(op->dev);
     if (op->fast)
         op->kname = std::string(dir != 0 ? (pair ? "lineop_strided_pair<" : p.P > 16 ? "lineop_strided_multi<" : "lineop_strided<")
+                                         : pair ? "lineop_contig_pair<"
                                          : op->dev.phases > 1 ? "lineop_contig_ph<" : "lineop_contig<") +
                     std::to_string(p.W) +
                     ", " + std::to_string(p.H) + ", " + std::to_string(p.MM) + ", " + per +

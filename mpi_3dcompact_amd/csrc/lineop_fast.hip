@@ -6,7 +6,10 @@
 namespace mc3d {
 
 bool lineop_fast_pair(const OpDev &op) {
-    if (op.dir == 0 || op.cshort >= 0 || op.N != op.P * op.MM || op.P <= 16 || op.P > 32) return false;
+    if (op.cshort >= 0 || op.N != op.P * op.MM || op.P <= 16 || op.P > 32) return false;
+    if (op.dir == 0)  // banded Dirichlet x-lines of 32-row chunks (lineop_contig_pair; shorter
+                      // lines keep the 64-line tiles, in 32-row chunks if need be)
+        return !op.periodic && op.W > 0 && op.MM == 32 && op.P <= 30 && op.same_interior;
     if (op.periodic && op.nbl + op.nbr == 0) return true;  // chunk 0's tables serve every chunk
     // special rows: 16- or 32-row chunks, a wave for each end chunk (30 chunks
     // in 16 waves), with no tables or interior ones shared
@@ -18,6 +21,12 @@ int lineop_pair_waves(const OpDev &op) {
 }
 
 size_t lineop_fast_lds_bytes(const OpDev &op) {
+    if (op.dir == 0 && lineop_fast_pair(op)) {  // 32-line tile of a group of waves, the interface values
+        const int nw = lineop_pair_waves(op), K = std::max(op.phases, 1);
+        const int wpp = (nw + K - 1) / K;
+        const size_t tile = (size_t)(2 * wpp * op.MM + 2 * op.H) * 33, red = (size_t)32 * (op.NI + 1);
+        return sizeof(double) * (op.W == 2 ? std::max(tile, red) : tile + red);
+    }
     if (op.dir == 0) {  // the tile, with the interface values inside it (W = 2) or after it
         const size_t L = contig_lines(op.periodic, op.W, op.cshort >= 0, op.N);
         const int K = L == 64 && op.phases > 1 ? op.phases : 1;
@@ -33,6 +42,14 @@ size_t lineop_fast_lds_bytes(const OpDev &op) {
 }
 
 int lineop_fast_phases(const OpDev &op, size_t lds_limit) {
+    if (op.dir == 0 && lineop_fast_pair(op)) {  // groups of waves
+        for (int k = 1; k <= lineop_pair_waves(op); ++k) {
+            OpDev t = op;
+            t.phases = k;
+            if (lineop_fast_lds_bytes(t) <= lds_limit) return k;
+        }
+        return 1;
+    }
     if (op.dir != 0 || contig_lines(op.periodic, op.W, op.cshort >= 0) != 64) return 1;
     for (int k = 1; k <= op.P; ++k) {
         OpDev t = op;

@@ -17,7 +17,8 @@
 // (lineop_strided_multi).
 //
 // Contiguous direction (x): the workgroup stages 8, 16 (periodic, explicit)
-// or 64 lines (banded Dirichlet or ragged: a wave per chunk) into LDS in a
+// or 64 lines (banded Dirichlet or ragged: a wave per chunk; 32 lines and two
+// chunks a wave for Dirichlet lines of more than 16 chunks) into LDS in a
 // transposed [point][line] layout (row stride lines + 1) with halo rows, so
 // the coalesced fill and the per-lane chunk reads are both free of bank
 // conflicts; each thread issues batches of loads before its LDS writes.
@@ -653,6 +654,144 @@ __global__ __launch_bounds__(1024) void lineop_strided_pair(OpDev op, LaunchArgs
     for (int i = 0; i < MM; ++i) bstore(y[i], rout, vmid, (unsigned)i * rowb);
 }
 
+// Dirichlet x-lines of 17 to 30 chunks of 32 rows of banded operators (544 to
+// 960 points), which the 64-line tiles cannot hold a wave per chunk:
+// 32-line tiles, two chunks to a wave, one per half-wave, laid out as in
+// lineop_strided_pair (chunk 0 and chunk P - 1 a wave each, the interior
+// chunks two to a wave, so that tables and end rows are wave-uniform), here
+// with neighbouring chunks paired: wave t = 1 .. Q holds chunks 2t - 1 and 2t
+// (Q = (P - 1) / 2; for odd P the upper half of wave Q redoes the lower
+// half's chunk and stores nothing), wave Q + 1 chunk P - 1.  The waves then
+// hold the chunks in line order, and the staging of lineop_contig_ph applies
+// to groups of waves: op.phases groups of consecutive waves, each group's rows
+// loaded into one tile [row][line] (row stride 33), its windows read, the next
+// group reusing the tile; the results drain the same way.
+template <int W, int H, int MM>
+__global__ __launch_bounds__(1024) void lineop_contig_pair(OpDev op, LaunchArgs a) {
+    extern __shared__ double smem[];
+    constexpr int L = 32, LS = L + 1, kB = 8;
+    const int N = op.N, P = op.P, NI = op.NI, K = op.phases;
+    const int Q = (P - 1) / 2, NW = Q + 2;
+    const int wpp = (NW + K - 1) / K;  // waves per group
+    // first and last chunk of wave t
+    auto wfirst = [=](int t) { return t == 0 ? 0 : t == NW - 1 ? P - 1 : 2 * t - 1; };
+    auto wlast = [=](int t) { return t == 0 ? 0 : t == NW - 1 ? P - 1 : 2 * t <= P - 2 ? 2 * t : 2 * t - 1; };
+    double *tile = smem;
+    constexpr bool kShare = W == 2;  // interface values inside the tile (dead by then)
+    // (a group spans at most 2 wpp chunks)
+    double *red = kShare ? smem : smem + (size_t)(2 * wpp * MM + 2 * H) * LS;
+    const int l = threadIdx.x, ll = l & 31, half = l >> 5;
+    const int ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int cl = wfirst(ty);
+    const bool hi_has = ty >= 1 && ty <= Q && 2 * ty <= P - 2;
+    const int ch = hi_has ? 2 * ty : cl;
+    const int c = half ? ch : cl;  // this lane's chunk
+    const bool has = !half || hi_has;
+    const int tid = threadIdx.y * 64 + l;
+    const int nthr = 64 * NW;
+    const int q0 = blockIdx.x * L;
+    const int rem = a.nlines - q0;
+    const unsigned nrec = (unsigned)(rem < L ? rem : L) * (unsigned)N * 8u;
+    const __amdgpu_buffer_rsrc_t rin = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<double *>(a.in[blockIdx.y]) + (long)q0 * N, (short)0, (int)nrec, kRawBufferWord3);
+    const int q = q0 + ll;
+    const bool live = q < a.nlines;
+    const int cs = c * MM;
+    const int myk = ty / wpp;
+    double w[MM + 2 * H];
+    for (int k = 0; k < K; ++k) {
+        const int t0 = k * wpp;
+        if (t0 >= NW) break;
+        const int t1 = t0 + wpp < NW ? t0 + wpp : NW;
+        const int r0 = wfirst(t0) * MM - H;  // line row held by tile row 0
+        const int R = (wlast(t1 - 1) + 1) * MM + H - r0;
+        if (k > 0) __syncthreads();  // the previous group has read its windows
+        {
+            int lq = tid / R, rr = tid - lq * R;
+            const int qd = nthr / R, rd = nthr - qd * R;
+            for (int e0 = tid; e0 < L * R; e0 += kB * nthr) {
+                double v[kB];
+                int slot[kB];
+#pragma unroll
+                for (int b = 0; b < kB; ++b) {
+                    const int p = r0 + rr;
+                    v[b] = (lq < L && p >= 0 && p < N) ? bload_nt(rin, (unsigned)(lq * N + p) * 8u, 0) : 0.0;
+                    slot[b] = lq < L ? rr * LS + lq : -1;
+                    lq += qd;
+                    rr += rd;
+                    if (rr >= R) { rr -= R; ++lq; }
+                }
+#pragma unroll
+                for (int b = 0; b < kB; ++b)
+                    if (slot[b] >= 0) tile[slot[b]] = v[b];
+            }
+        }
+        __syncthreads();
+        if (myk == k) {
+            const double *trow = tile + (size_t)(cs - H - r0) * LS + ll;
+#pragma unroll
+            for (int j = 0; j < MM + 2 * H; ++j) w[j] = trow[j * LS];
+        }
+    }
+    if (kShare) __syncthreads();  // every window is read before the interface values overwrite the tile
+    double y[MM];
+    chunk_rhs<H, MM, false, true>(op, w, y, cl == 0, cl == P - 1);
+    double *myred = red + ll * (NI + 1);
+    chunk_solve<W, MM>(op.lu + (size_t)cl * MM * kLuRec, y);
+    if (has) publish<W, MM, false>(myred, c, false, MM, y);
+    __syncthreads();
+    {
+        double s[2 * W];
+        interface_sums_pair<W>(op, cl, ch, half, myred, s);
+        const double *sp = op.sp + (size_t)cl * MM * 2 * W;
+        chunk_correct_by<W, MM>([=](int e) { return sp[e]; }, s, y);
+    }
+    // pass-through lines keep their input (nothing is stored yet)
+    if (live && pass_line(op, 0, q, a.Nx, a.Ny)) {
+#pragma unroll
+        for (int i = 0; i < MM; ++i) y[i] = bload(rin, (unsigned)(ll * N + cs + i) * 8u, 0);
+    }
+    const __amdgpu_buffer_rsrc_t rout = __builtin_amdgcn_make_buffer_rsrc(
+        a.out[blockIdx.y] + (long)q0 * N, (short)0, (int)nrec, kRawBufferWord3);
+    for (int k = 0; k < K; ++k) {
+        const int t0 = k * wpp;
+        if (t0 >= NW) break;
+        const int t1 = t0 + wpp < NW ? t0 + wpp : NW;
+        const int s0 = wfirst(t0) * MM;  // line row held by tile row 0
+        const int R = (wlast(t1 - 1) + 1) * MM - s0;
+        __syncthreads();  // interface values read (first group) or the previous group stored
+        if (myk == k && has && live) {
+#pragma unroll
+            for (int i = 0; i < MM; ++i) tile[(cs - s0 + i) * LS + ll] = y[i];
+        }
+        __syncthreads();
+        int lq = tid / R, rr = tid - lq * R;
+        const int qd = nthr / R, rd = nthr - qd * R;
+        for (int e0 = tid; e0 < L * R; e0 += kB * nthr) {
+            double v[kB];
+            unsigned off[kB];
+            bool ok[kB];
+#pragma unroll
+            for (int b = 0; b < kB; ++b) {
+                ok[b] = lq < L;
+                v[b] = ok[b] ? tile[rr * LS + lq] : 0.0;
+                off[b] = (unsigned)(lq * N + s0 + rr) * 8u;
+                lq += qd;
+                rr += rd;
+                if (rr >= R) { rr -= R; ++lq; }
+            }
+            if (a.accumulate) {
+#pragma unroll
+                for (int b = 0; b < kB; ++b)
+                    if (ok[b]) v[b] = bload_nt(rout, off[b], 0) + v[b];
+            }
+#pragma unroll
+            for (int b = 0; b < kB; ++b)
+                if (ok[b]) bstore(v[b], rout, off[b], 0);
+        }
+    }
+}
+
 // lines per workgroup of the contiguous kernel: 16 for periodic lines (every
 // table read is chunk independent, so a wave may span chunks; the smaller tile
 // lets four workgroups of 256-point lines share a CU, which overlaps one
@@ -976,6 +1115,13 @@ __global__ __launch_bounds__(1024) void lineop_contig_ph(OpDev op, LaunchArgs a)
 template <int W, int H, int MM, bool PER, bool RG>
 hipError_t launch_fast(const OpDev &op, const LaunchArgs &a, int nfields, hipStream_t st) {
     if (op.dir == 0) {
+        if constexpr (!PER && !RG && W > 0 && MM == 32) {
+            if (lineop_fast_pair(op)) {
+                dim3 grid((a.nlines + 31) / 32, nfields), block(64, lineop_pair_waves(op));
+                hipLaunchKernelGGL((lineop_contig_pair<W, H, MM>), grid, block, lineop_fast_lds_bytes(op), st, op, a);
+                return hipGetLastError();
+            }
+        }
         constexpr int L = contig_lines(PER, W, RG);
         if constexpr (PER && !RG && W > 0) {
             if (contig_lines(PER, W, RG, op.N) == 8) {

@@ -34,11 +34,12 @@ bool lineop_fast_supported(const OpDev &op, size_t lds_limit);
 // groups of chunks the 64-line x-line kernel stages one after another so that
 // its tile fits lds_limit (1: the whole line at once)
 int lineop_fast_phases(const OpDev &op, size_t lds_limit);
-// y- and z-lines the fast path runs two chunks to a wave (lineop_strided_pair):
-// evenly chunked lines of 17 to 32 chunks whose chunks read chunk-independent
-// tables (periodic lines without special rows), or, up to 30 chunks, need
-// none or have interior chunks sharing theirs (Dirichlet lines, CD2); and the
-// waves of its workgroup
+// Lines the fast path runs two chunks to a wave: y- and z-lines
+// (lineop_strided_pair) evenly chunked into 17 to 32 chunks whose chunks read
+// chunk-independent tables (periodic lines without special rows), or, up to
+// 30 chunks, need none or have interior chunks sharing theirs (Dirichlet
+// lines, CD2); x-lines (lineop_contig_pair) of banded Dirichlet operators in
+// 17 to 30 chunks of 32 rows.  And the waves of its workgroup.
 bool lineop_fast_pair(const OpDev &op);
 int lineop_pair_waves(const OpDev &op);
 hipError_t lineop_launch_fast(const OpDev &op, const LaunchArgs &a, int nfields, hipStream_t st);

@@ -148,13 +148,16 @@ def test_filter_ragged_lines(torch_cuda, kind, af, bc3, shape):
         assert relerr(fl.cpu().numpy(), ref) <= TOL, (kind, bc3, d, "in place")
 
 
-@pytest.mark.parametrize("nx", [310, 384, 512])
+@pytest.mark.parametrize("nx", [310, 384, 512, 544, 640, 960])
 @pytest.mark.parametrize("bc", [P.PERIODIC, P.DIRICHLET])
 def test_long_x_lines_in_phases(torch_cuda, nx, bc):
     """x-lines whose 64-line tile exceeds LDS (banded Dirichlet operators, and
     ragged periodic ones): the chunks are staged in groups through one tile
-    (lineop_contig_ph); derivatives and filters against the oracle, pass-through
-    lines and in-place filtering included."""
+    (lineop_contig_ph); banded Dirichlet lines of 17 to 30 chunks (544 = 17 x
+    32 with an idle upper half-wave, 640, 960 = 30 x 32) in 32-line tiles two
+    chunks to a wave, staged in groups of waves (lineop_contig_pair), 216
+    lines (a partial last workgroup); derivatives and filters against the
+    oracle, pass-through lines and in-place filtering included."""
     from mpi_3dcompact_amd.ops import LineOp
     torch = torch_cuda
     shape = (12, 18, nx)
@@ -165,12 +168,13 @@ def test_long_x_lines_in_phases(torch_cuda, nx, bc):
     out = torch.empty_like(fin)
     dd = 1.0 / nx if bc == P.PERIODIC else 1.0 / (nx - 1)
     phased = bc == P.DIRICHLET or nx == 310
+    expect = "lineop_contig_pair<" if bc == P.DIRICHLET and nx > 512 else "lineop_contig_ph<" if phased else None
     for scheme in (P.PADE6, P.PENTA10):
         op = LineOp.deriv(scheme, 0, bc, dims, dd)
         kname = op.kernel()
         _say(f"long x deriv nx={nx} bc={bc} scheme={scheme} kernel={kname}")
-        if phased:
-            assert kname.startswith("lineop_contig_ph<"), kname
+        if expect:
+            assert kname.startswith(expect), kname
         op.apply([fin], [out])
         torch.cuda.synchronize()
         assert relerr(out.cpu().numpy(), P.deriv_field(scheme, 0, bc, f, dd)) <= TOL, (scheme, nx, bc)
@@ -179,8 +183,8 @@ def test_long_x_lines_in_phases(torch_cuda, nx, bc):
         op = LineOp.filter(kind, af, 0, bc3, dims)
         kname = op.kernel()
         _say(f"long x filter nx={nx} bc3={bc3} kind={kind} kernel={kname}")
-        if phased:
-            assert kname.startswith("lineop_contig_ph<"), kname
+        if expect:
+            assert kname.startswith(expect), kname
         ref = P.filter_field(kind, af, 0, list(bc3), f)
         op.apply([fin], [out])
         torch.cuda.synchronize()
