@@ -12,13 +12,14 @@
 // and the stores go through a buffer descriptor (one 32-bit lane offset, the
 // row offset in a scalar register), and all window loads are issued before
 // any other work of the lane.  Lines of more than 16 chunks: evenly chunked
-// ones up to 32 chunks two chunks to a wave (lineop_strided_pair), the others
-// in rounds of 16 chunk waves with a second, recomputing pass
-// (lineop_strided_multi).
+// ones two chunks to a wave (lineop_strided_pair; up to 32 chunks, 30 with
+// special rows), the others in rounds of 16 chunk waves with a second,
+// recomputing pass (lineop_strided_multi).
 //
 // Contiguous direction (x): the workgroup stages 8, 16 (periodic, explicit)
 // or 64 lines (banded Dirichlet or ragged: a wave per chunk; 32 lines and two
-// chunks a wave for Dirichlet lines of more than 16 chunks) into LDS in a
+// chunks a wave for Dirichlet lines of 17 to 30 chunks of 32 rows,
+// lineop_contig_pair) into LDS in a
 // transposed [point][line] layout (row stride lines + 1) with halo rows, so
 // the coalesced fill and the per-lane chunk reads are both free of bank
 // conflicts; each thread issues batches of loads before its LDS writes.
