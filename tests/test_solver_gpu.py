@@ -104,14 +104,18 @@ def test_ragged_box_36(gpu, case):
     check_pair(g, o, extra=extra)
 
 
-@pytest.mark.parametrize("faces", [None, {"Z0": "ADIABATIC_WALL", "Z1": "ADIABATIC_WALL"}], ids=["periodic", "z-walls"])
-def test_long_z_box(gpu, faces):
-    """A 12 x 12 x 576 box: z-lines of 18 32-row chunks, two to a wave
-    (lineop_strided_pair), periodic or between adiabatic walls (Dirichlet
-    z-lines: the end chunks take a wave each), the zeta divergence's
-    accumulating launches included; against the oracle after four steps."""
+@pytest.mark.parametrize("long_axis,faces", [
+    ("z", None), ("z", {"Z0": "ADIABATIC_WALL", "Z1": "ADIABATIC_WALL"}),
+    ("x", {"X0": "ADIABATIC_WALL", "X1": "ADIABATIC_WALL"})], ids=["z-periodic", "z-walls", "x-walls"])
+def test_long_line_box(gpu, long_axis, faces):
+    """Boxes of 576 points along one axis and 12 along the others: z-lines of
+    18 32-row chunks, two to a wave (lineop_strided_pair), periodic or between
+    adiabatic walls (Dirichlet z-lines: the end chunks take a wave each), the
+    zeta divergence's accumulating launches included; x-lines between walls
+    the same in 32-line tiles (lineop_contig_pair); against the oracle after
+    four steps."""
     from mpi_3dcompact_amd.solver import Solver
-    Nx, Ny, Nz = 12, 12, 576
+    Nx, Ny, Nz = (12, 12, 576) if long_axis == "z" else (576, 12, 12)
     z, y, x = np.meshgrid(np.arange(Nz) / Nz, np.arange(Ny) / Ny, np.arange(Nx) / Nx, indexing="ij")
     tp = 2 * np.pi
     ic = dict(rho=1.0 + 0.02 * np.sin(tp * (x + 3 * z)), U=0.2 * np.sin(tp * (y + 5 * z)),
@@ -121,12 +125,12 @@ def test_long_z_box(gpu, faces):
     one, zero = np.ones((Nz, Ny, Nx)), np.zeros((Nz, Ny, Nx))
     met = [one / L ** 3, one / L, zero, zero, zero, one / L, zero, zero, zero, one / L]
     case = dict(fd="PENTA10", filt="COMPACT8", rk="RK4", mu_ref=1e-3, dims=(Nx, Ny, Nz))
-    _say(f"case: 12 x 12 x 576, faces {faces}")
-    g = Solver(hit.options_text(Nz, faces=faces, **case), (Nz, Ny, Nx))
+    _say(f"case: {Nx} x {Ny} x {Nz}, faces {faces}")
+    g = Solver(hit.options_text(max(Nx, Nz), faces=faces, **case), (Nz, Ny, Nx))
     g.set_metrics(met)
     g.set_initial(ic)
     g.step(4)
-    o = P.Solver(hit.oracle_config(Nz, faces=faces, **case), met, ic)
+    o = P.Solver(hit.oracle_config(max(Nx, Nz), faces=faces, **case), met, ic)
     o.steps(4)
     check_pair(g, o)
 
