@@ -329,6 +329,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256, help="grid points per direction")
+    ap.add_argument("--walls", action="store_true",
+                    help="adiabatic walls on every face (Dirichlet lines in every direction) instead of the periodic box")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=3, help="timed CPU oracle steps after one warm-up")
@@ -368,7 +370,8 @@ def main():
     log(f"[bench] generating {N}^3 HIT initial field")
     ic = hit.hit_initial_condition(N, k0=4.0, Mt=0.3, seed=0)
     met = hit.identity_metrics(N)
-    g = Solver(hit.options_text(N, **CASE), (N, N, N))
+    faces = {f: "ADIABATIC_WALL" for f in hit.FACE_NAMES} if args.walls else None
+    g = Solver(hit.options_text(N, faces=faces, **CASE), (N, N, N))
     g.set_metrics(met)
     g.set_initial(ic)
     del met
@@ -409,7 +412,7 @@ def main():
     g.close()
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and not args.walls:  # (the CPU sample is the periodic box)
         log(f"[bench] CPU oracle baseline at {args.cpu_n}^3")
         cpu = cpu_baseline(args.cpu_n, args.cpu_steps, args.cpu_one_core_n)
 
@@ -426,12 +429,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"full plain LES, decaying HIT {N}^3 periodic, PENTA10 + COMPACT8 "
+        "config": {"workload": f"full plain LES, decaying HIT {N}^3 "
+                               f"{'between adiabatic walls on every face' if args.walls else 'periodic'}, PENTA10 + COMPACT8 "
                                f"(alpha_f 0.49) + KENRK4 (CFL 0.5) + VREMAN LOCAL, mu 1/5000",
                    "grid": [N, N, N], "points": N ** 3, "rk_stages_per_step": 5},
         "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(dom["kernel"], N), "bytes_per_launch": bytes_per_launch,
+                     "traffic": None if args.walls else measured_traffic(dom["kernel"], N), "bytes_per_launch": bytes_per_launch,
                      "avg_launch_ms": round(avg_ms, 4),
                      "share_of_step": round(dom["ms"] / kern_ms, 3),
                      "events": f"HIP events around every launch over {args.steps} further steps "
