@@ -550,9 +550,10 @@ __global__ __launch_bounds__(1024) void lineop_strided_multi(OpDev op, LaunchArg
 //    two to a wave (1 + k and 1 + k + Q, Q = (P - 1) / 2).  The end-row flags
 //    and the factor and spike tables are then wave-uniform (the interior
 //    chunks' tables are equal bitwise; the host checks it); the interface
-//    rows G_c, different for every chunk, are selected per lane.  (Tables and
-//    end rows selected per lane in the waves holding an end chunk took the
-//    kernel past 128 registers.)
+//    rows G_c differ for every chunk of a Dirichlet line, so each lane forms
+//    both of its wave's sums and keeps its own (interface_sums_pair).
+//    (Tables and end rows selected per lane in the waves holding an end
+//    chunk took the kernel past 128 registers.)
 template <int W, int H, int MM, bool PER>
 __global__ __launch_bounds__(1024) void lineop_strided_pair(OpDev op, LaunchArgs a) {
     constexpr bool ROWS = has_rows<W, H, PER>();
@@ -659,7 +660,8 @@ __global__ __launch_bounds__(1024) void lineop_strided_pair(OpDev op, LaunchArgs
 // 960 points), which the 64-line tiles cannot hold a wave per chunk:
 // 32-line tiles, two chunks to a wave, one per half-wave, laid out as in
 // lineop_strided_pair (chunk 0 and chunk P - 1 a wave each, the interior
-// chunks two to a wave, so that tables and end rows are wave-uniform), here
+// chunks two to a wave, so that tables and end rows are wave-uniform and only
+// the interface sums are formed for both chunks of a wave), here
 // with neighbouring chunks paired: wave t = 1 .. Q holds chunks 2t - 1 and 2t
 // (Q = (P - 1) / 2; for odd P the upper half of wave Q redoes the lower
 // half's chunk and stores nothing), wave Q + 1 chunk P - 1.  The waves then
