@@ -59,7 +59,7 @@ struct OpDev {
     int gb;               // interface band in chunks (Plan::gb)
     int cshort, lshort;   // ragged lines: chunk cshort has lshort rows (else cshort = -1)
     int phases;           // x-lines in 64-line tiles: groups of chunks staged one after another (1: all)
-    int same_interior;    // chunks 1 .. P - 2 share bitwise their lu and sp tables (Plan::same_interior)
+    int same_interior;    // interior chunks share bitwise their lu and sp tables (Plan::same_interior)
     const int *cstart;    // P
     const int *clen;      // P
 };

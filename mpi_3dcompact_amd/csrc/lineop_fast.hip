@@ -6,7 +6,10 @@
 namespace mc3d {
 
 bool lineop_fast_pair(const OpDev &op) {
-    if (op.cshort >= 0 || op.N != op.P * op.MM || op.P <= 16 || op.P > 32) return false;
+    if (op.cshort >= 0)  // ragged Dirichlet y/z-lines, the short chunk a wave of its own (29 chunks in 16 waves)
+        return op.dir != 0 && !op.periodic && (op.MM == 16 || op.MM == 32) && op.cshort == 1 && op.P > 16 &&
+               op.P <= 29 && (op.W == 0 || op.same_interior);
+    if (op.N != op.P * op.MM || op.P <= 16 || op.P > 32) return false;
     if (op.dir == 0)  // banded Dirichlet x-lines of 32-row chunks (lineop_contig_pair; shorter
                       // lines keep the 64-line tiles, in 32-row chunks if need be)
         return !op.periodic && op.W > 0 && op.MM == 32 && op.P <= 30 && op.same_interior;
@@ -17,7 +20,9 @@ bool lineop_fast_pair(const OpDev &op) {
 }
 
 int lineop_pair_waves(const OpDev &op) {
-    return op.periodic && op.nbl + op.nbr == 0 ? (op.P + 1) / 2 : 2 + (op.P - 1) / 2;
+    if (op.periodic && op.nbl + op.nbr == 0 && op.cshort < 0) return (op.P + 1) / 2;
+    const int f = op.cshort >= 0 ? 2 : 1;  // chunks 0 .. f - 1 and P - 1 alone, the rest in pairs
+    return f + 1 + (op.P - f) / 2;
 }
 
 size_t lineop_fast_lds_bytes(const OpDev &op) {

@@ -547,15 +547,18 @@ __global__ __launch_bounds__(1024) void lineop_strided_multi(OpDev op, LaunchArg
 //  - lines with special rows (Dirichlet, CD2's periodic wrap rows), up to 30
 //    chunks: chunk 0 and chunk P - 1 take a wave each (the upper half redoes
 //    the lower half's chunk and stores nothing), the interior chunks 1 .. P - 2
-//    two to a wave (1 + k and 1 + k + Q, Q = (P - 1) / 2).  The end-row flags
+//    two to a wave (1 + k and 1 + k + Q, Q = (P - 1) / 2); ragged Dirichlet
+//    lines (RG) up to 29 chunks give the short chunk 1 a wave of its own too
+//    and pair chunks 2 .. P - 2 (2 + k and 2 + k + Q, Q = (P - 2) / 2).  The end-row flags
 //    and the factor and spike tables are then wave-uniform (the interior
 //    chunks' tables are equal bitwise; the host checks it); the interface
 //    rows G_c differ for every chunk of a Dirichlet line, so each lane forms
 //    both of its wave's sums and keeps its own (interface_sums_pair).
 //    (Tables and end rows selected per lane in the waves holding an end
 //    chunk took the kernel past 128 registers.)
-template <int W, int H, int MM, bool PER>
+template <int W, int H, int MM, bool PER, bool RG>
 __global__ __launch_bounds__(1024) void lineop_strided_pair(OpDev op, LaunchArgs a) {
+    static_assert(!(PER && RG), "ragged lines in pairs: Dirichlet only");
     constexpr bool ROWS = has_rows<W, H, PER>();
     constexpr bool OWN = !PER && W > 0;  // tables per chunk
     extern __shared__ double red[];  // [32 lines][NI + 1]
@@ -564,9 +567,10 @@ __global__ __launch_bounds__(1024) void lineop_strided_pair(OpDev op, LaunchArgs
     const int P2 = (P + 1) / 2;
     // the wave's chunks (cl, ch), wave-uniform with special rows
     const int ty = ROWS ? __builtin_amdgcn_readfirstlane(threadIdx.y) : 0;
-    const int Q = (P - 1) / 2;
+    constexpr int F = RG ? 2 : 1;  // first paired chunk (waves 0 .. F hold one chunk each)
+    const int Q = (P - F) / 2;
     const int cl = ty == 0 ? 0 : ty == 1 ? P - 1 : ty - 1;
-    const bool hi_has = ty >= 2 && ty - 1 + Q <= P - 2;
+    const bool hi_has = ty >= F + 1 && ty - 1 + Q <= P - 2;
     const int ch = hi_has ? ty - 1 + Q : cl;
     int c;     // this lane's chunk
     bool has;  // whether the lane stores it
@@ -598,7 +602,9 @@ __global__ __launch_bounds__(1024) void lineop_strided_pair(OpDev op, LaunchArgs
         base = qq;
         ps = (long)a.Nx * a.Ny;
     }
-    const int cs = cc * MM;
+    const int cs = chunk_start<MM, RG>(op, cc);
+    const bool short_chunk = RG && cl == op.cshort;  // (wave-uniform: the short chunk's wave holds it alone)
+    const int len = short_chunk ? op.lshort : MM;
     const __amdgpu_buffer_rsrc_t rin =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(in), (short)0, (int)fbytes, kRawBufferWord3);
     const unsigned rowb = (unsigned)ps * 8u;
@@ -632,7 +638,7 @@ __global__ __launch_bounds__(1024) void lineop_strided_pair(OpDev op, LaunchArgs
     const double *sp = op.sp + (OWN ? (size_t)cl * MM * 2 * W : 0);
     if (W > 0) {
         chunk_solve<W, MM>(lu, y);
-        if (has) publish<W, MM, false>(myred, c, false, MM, y);
+        if (has) publish<W, MM, RG>(myred, c, short_chunk, len, y);
     }
     __syncthreads();
     if (W > 0) {
@@ -646,14 +652,17 @@ __global__ __launch_bounds__(1024) void lineop_strided_pair(OpDev op, LaunchArgs
         __builtin_amdgcn_make_buffer_rsrc(out, (short)0, (int)fbytes, kRawBufferWord3);
     if (pass_line(op, op.dir, q, a.Nx, a.Ny)) {
 #pragma unroll
-        for (int i = 0; i < MM; ++i) y[i] = bload(rin, vmid, (unsigned)i * rowb);
+        for (int i = 0; i < MM; ++i)
+            if (!RG || i < len) y[i] = bload(rin, vmid, (unsigned)i * rowb);
     }
     if (a.accumulate) {
 #pragma unroll
-        for (int i = 0; i < MM; ++i) y[i] = bload_nt(rout, vmid, (unsigned)i * rowb) + y[i];
+        for (int i = 0; i < MM; ++i)
+            if (!RG || i < len) y[i] = bload_nt(rout, vmid, (unsigned)i * rowb) + y[i];
     }
 #pragma unroll
-    for (int i = 0; i < MM; ++i) bstore(y[i], rout, vmid, (unsigned)i * rowb);
+    for (int i = 0; i < MM; ++i)
+        if (!RG || i < len) bstore(y[i], rout, vmid, (unsigned)i * rowb);
 }
 
 // Dirichlet x-lines of 17 to 30 chunks of 32 rows of banded operators (544 to
@@ -1144,11 +1153,11 @@ hipError_t launch_fast(const OpDev &op, const LaunchArgs &a, int nfields, hipStr
         }
         hipLaunchKernelGGL((lineop_contig<W, H, MM, PER, RG, L>), grid, block, lineop_fast_lds_bytes(op), st, op, a);
     } else {
-        if constexpr (!RG && ((PER && !has_rows<W, H, PER>()) || MM == 16 || MM == 32)) {
+        if constexpr ((!RG && ((PER && !has_rows<W, H, PER>()) || MM == 16 || MM == 32)) || (RG && !PER)) {
             if (lineop_fast_pair(op)) {
                 dim3 grid((a.nlines + 31) / 32, nfields, op.dir == 2 ? a.nbatch : 1), block(64, lineop_pair_waves(op));
-                hipLaunchKernelGGL((lineop_strided_pair<W, H, MM, PER>), grid, block, lineop_fast_lds_bytes(op), st,
-                                   op, a);
+                hipLaunchKernelGGL((lineop_strided_pair<W, H, MM, PER, RG>), grid, block, lineop_fast_lds_bytes(op),
+                                   st, op, a);
                 return hipGetLastError();
             }
         }

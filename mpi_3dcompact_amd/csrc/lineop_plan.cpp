@@ -567,12 +567,14 @@ int build_plan(const PlanRequest &rq, Plan &plan, std::string &err) {
             }
         }
     }
-    if (plan.uniform && P >= 3) {
+    // (ragged lines: the full chunks 2 .. P - 2, after the short chunk 1)
+    const int c1 = plan.cshort >= 0 ? 2 : 1;
+    if ((plan.uniform || plan.cshort == 1) && P >= c1 + 2) {
         const size_t nlu = (size_t)mm * kLuRec, nsp = (size_t)mm * W2;
         plan.same_interior = true;
-        for (int c = 2; c < P - 1 && plan.same_interior; ++c)
-            plan.same_interior = std::memcmp(&plan.lu[c * nlu], &plan.lu[nlu], nlu * sizeof(double)) == 0 &&
-                                 std::memcmp(&plan.sp[c * nsp], &plan.sp[nsp], nsp * sizeof(double)) == 0;
+        for (int c = c1 + 1; c < P - 1 && plan.same_interior; ++c)
+            plan.same_interior = std::memcmp(&plan.lu[c * nlu], &plan.lu[c1 * nlu], nlu * sizeof(double)) == 0 &&
+                                 std::memcmp(&plan.sp[c * nsp], &plan.sp[c1 * nsp], nsp * sizeof(double)) == 0;
     }
     // interface matrix R t = yhat
     Mat R(NI);

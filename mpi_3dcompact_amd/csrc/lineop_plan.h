@@ -32,8 +32,9 @@ struct Plan {
     // lshort < MM; cshort = -1 otherwise
     int cshort = -1, lshort = 0;
     bool dense_ok = false;  // special rows fit the (2H+1)-wide dense form
-    // chunks 1 .. P - 2 hold bitwise the same factor and spike tables (their
-    // blocks are interior rows of the matrix); false when there are no tables
+    // chunks 1 .. P - 2 (ragged lines: 2 .. P - 2) hold bitwise the same factor
+    // and spike tables (their blocks are interior rows of the matrix); false
+    // when there are no tables
     bool same_interior = false;
     bool periodic = false, divide = false;
     double dd = 1.0;

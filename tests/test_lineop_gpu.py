@@ -195,16 +195,20 @@ def test_long_x_lines_in_phases(torch_cuda, nx, bc):
         assert relerr(fl.cpu().numpy(), ref) <= TOL, (kind, nx, bc, "in place")
 
 
-@pytest.mark.parametrize("shape", [(576, 40, 12), (544, 13, 11), (960, 12, 16), (650, 12, 16), (1024, 12, 16)])
+@pytest.mark.parametrize("shape", [(576, 40, 12), (544, 13, 11), (960, 12, 16), (650, 12, 16), (560, 12, 11),
+                                   (906, 10, 10), (1024, 12, 16)])
 def test_long_yz_lines_in_rounds(torch_cuda, shape):
     """z-lines of more than 16 chunks: evenly chunked lines two chunks to a
     wave (lineop_strided_pair): periodic ones without special rows up to 32
     chunks (576 = 18 x 32, 544 = 17 x 32 with the last wave's upper half
     idle, 1024 = 32 x 32), Dirichlet ones and CD2 up to 30 (a wave for each
-    end chunk; 960 = 30 x 32 fills 16 waves); the rest (ragged 650 = 16 + 10
-    + 39 x 16, Dirichlet and CD2 at 1024) in 16-row chunks, 16 chunk waves
-    taking them in rounds (lineop_strided_multi); every operator against the
-    oracle, in place too."""
+    end chunk; 960 = 30 x 32 fills 16 waves), ragged Dirichlet ones up to 29
+    (650 = 32 + 10 + 19 x 32, the short chunk a wave of its own; 560 = 32 +
+    16 + 16 x 32 with the last wave's upper half idle; 906 = 29 chunks in 16
+    waves); the rest
+    (periodic 650 = 16 + 10 + 39 x 16, Dirichlet and CD2 at 1024) in 16-row
+    chunks, 16 chunk waves taking them in rounds (lineop_strided_multi); every
+    operator against the oracle, in place too."""
     from mpi_3dcompact_amd.ops import LineOp
     torch = torch_cuda
     Nz, Ny, Nx = shape
@@ -218,7 +222,8 @@ def test_long_yz_lines_in_rounds(torch_cuda, shape):
             op = LineOp.deriv(scheme, 2, bc, dims, dd)
             kname = op.kernel()
             _say(f"long z deriv shape={shape} bc={bc} scheme={scheme} kernel={kname}")
-            pair = Nz % 32 == 0 and ((bc == P.PERIODIC and scheme != P.CD2) or Nz // 32 <= 30)
+            pair = ((Nz % 32 == 0 and ((bc == P.PERIODIC and scheme != P.CD2) or Nz // 32 <= 30))
+                    or (Nz % 32 != 0 and bc == P.DIRICHLET and Nz // 32 + 1 <= 29))
             assert kname.startswith("lineop_strided_pair<" if pair else "lineop_strided_multi<"), kname
             op.apply([fin], [out])
             torch.cuda.synchronize()
@@ -228,7 +233,8 @@ def test_long_yz_lines_in_rounds(torch_cuda, shape):
             op = LineOp.filter(kind, af, 2, bc3, dims)
             kname = op.kernel()
             _say(f"long z filter shape={shape} bc3={bc3} kind={kind} kernel={kname}")
-            pair = Nz % 32 == 0 and (bc == P.PERIODIC or Nz // 32 <= 30)
+            pair = ((Nz % 32 == 0 and (bc == P.PERIODIC or Nz // 32 <= 30))
+                    or (Nz % 32 != 0 and bc == P.DIRICHLET and Nz // 32 + 1 <= 29))
             assert kname.startswith("lineop_strided_pair<" if pair else "lineop_strided_multi<"), kname
             ref = P.filter_field(kind, af, 2, list(bc3), f)
             op.apply([fin], [out])

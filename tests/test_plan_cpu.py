@@ -353,6 +353,24 @@ def test_dirichlet_interior_chunks_share_factors(kind, cls, af, n):
     assert not np.array_equal(t["G"][2], np.roll(t["G"][1], 2 * t["W"], axis=1))
 
 
+@pytest.mark.parametrize("kind,cls,af", [(P.PENTA10, 0, 0.0), (P.PADE6, 0, 0.0), (P.COMPACT8, 1, 0.49),
+                                         (P.COMMN5PADE, 1, 0.0)])
+@pytest.mark.parametrize("n", [600, 650, 906])
+def test_dirichlet_ragged_interior_chunks_share_factors(kind, cls, af, n):
+    """Ragged Dirichlet lines of 32-row chunks (the short chunk second):
+    chunks 2 .. P - 2 hold bitwise chunk 2's LU factors and spikes (the
+    two-chunk y/z kernel gives them all chunk 2's); the short chunk and the
+    end chunks their own."""
+    t = tables(cls, kind, P.DIRICHLET, n, 1 / (n - 1), af, 32)
+    Pn = t["P"]
+    assert Pn == n // 32 + 1 and list(t["clen"][:3]) == [32, n % 32, 32]
+    for c in range(3, Pn - 1):
+        assert np.array_equal(t["lu"][c], t["lu"][2])
+        assert np.array_equal(t["sp"][c], t["sp"][2])
+    for c in (0, 1, Pn - 1):
+        assert not np.array_equal(t["lu"][c], t["lu"][2])
+
+
 def test_plan_rejects_short_lines():
     from mpi_3dcompact_amd import Mc3dError
     with pytest.raises(Mc3dError):
