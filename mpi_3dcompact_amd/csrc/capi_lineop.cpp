@@ -135,9 +135,10 @@ int create(int cls, int kind, int dir, int bc, const int bc3[3], const int dims[
             d.NB = dims[db];
         }
         const bool rows_ok = p.nbl + p.nbr == 0 || p.dense_ok;
-        // the strided fast kernel addresses a field through a buffer descriptor
-        // with 32-bit byte offsets
-        const bool offsets_ok = npts * 8 < 2147483648L;
+        // the strided fast kernels address a field through a buffer descriptor:
+        // unsigned 32-bit byte offsets and record count, so fields below 4 GiB
+        // (536 M points: a 640^3 field and up to about 812^3)
+        const bool offsets_ok = npts * 8 < 4294967296L;
         d.phases = 1;
         d.phases = mc3d::lineop_fast_phases(d, (size_t)lds_limit);
         op->fast = (p.uniform || p.cshort >= 0) && rows_ok && offsets_ok &&

@@ -526,3 +526,47 @@ def test_long_dirichlet_x_lines(torch_cuda):
         op.apply([fin], [out])
         torch.cuda.synchronize()
         assert relerr(out.cpu().numpy(), P.deriv_field(scheme, 0, P.DIRICHLET, f, dd)) <= TOL, scheme
+
+
+def test_fields_above_2gib(torch_cuda):
+    """A 650^3 field (2.2 GB, above 2^31 bytes: the strided kernels' buffer
+    offsets and record counts are unsigned 32-bit): Penta10 derivatives in
+    every direction, periodic and Dirichlet, y and z through the fast kernels
+    (ragged lines: lineop_strided_multi, lineop_strided_pair); lines at
+    both ends of the field (byte offsets near 2.2e9) equal bitwise the same
+    lines of small boxes cut from it, which the other tests hold to the oracle."""
+    from mpi_3dcompact_amd.ops import LineOp
+    torch = torch_cuda
+    n = 650
+    g = torch.Generator(device="cuda").manual_seed(3)
+    f = torch.rand((n, n, n), dtype=torch.float64, device="cuda", generator=g)
+    out = torch.empty_like(f)
+    ends = torch.tensor([0, 1, 2, 3, n - 4, n - 3, n - 2, n - 1], device="cuda")
+    for bc in (P.PERIODIC, P.DIRICHLET):
+        dd = 1.0 / n if bc == P.PERIODIC else 1.0 / (n - 1)
+        for direction in (0, 1, 2):
+            op = LineOp.deriv(P.PENTA10, direction, bc, (n, n, n), dd)
+            kname = op.kernel()
+            _say(f"650^3 field dir={direction} bc={bc} kernel={kname}")
+            if direction != 0:  # (ragged x-lines of more than 16 chunks take the LDS-tiled kernel)
+                assert not kname.startswith("lineop_kernel<"), kname
+            op.apply([f], [out])
+            torch.cuda.synchronize()
+            # the lines of direction `direction` through the end rows of the other two axes
+            if direction == 0:
+                sub, ref = f[ends][:, ends], out[ends][:, ends]
+            elif direction == 1:
+                sub, ref = f[ends][:, :, ends], out[ends][:, :, ends]
+            else:
+                sub, ref = f[:, ends][:, :, ends], out[:, ends][:, :, ends]
+            sub = sub.contiguous()
+            Nz, Ny, Nx = sub.shape
+            small = LineOp.deriv(P.PENTA10, direction, bc, (Nx, Ny, Nz), dd)
+            assert small.kernel() == kname, (small.kernel(), kname)
+            o2 = torch.empty_like(sub)
+            small.apply([sub], [o2])
+            torch.cuda.synchronize()
+            assert torch.equal(o2, ref.contiguous()), (direction, bc)
+            del op, small
+    del f, out
+    torch.cuda.empty_cache()
